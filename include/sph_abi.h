@@ -1,0 +1,319 @@
+/*
+ * sph_abi.h - C ABI of libsph_hip.so, the MI355X (gfx950) implementation of
+ * TinySPH's per-step compute: the Clavet double-density relaxation 2D SPH
+ * step of AdamSimpson/SPH.
+ *
+ * Three layers, all plain C linkage, plain pointers and sizes:
+ *
+ *   1. The reference's records, layout-identical, so AoS buffers that the
+ *      reference's host code fills can be handed over unchanged
+ *      (src/fluid.h:56-106, src/hash.h:35-48, src/geometry.h:33-40).
+ *   2. A device-resident context API (sph_create / sph_upload / sph_step /
+ *      sph_download ...).  The whole domain lives in one SoA particle array
+ *      in HBM, sorted by grid cell; it replaces the reference's per-rank
+ *      AoS + pointer array + MPI halo exchange (src/communication.c).
+ *   3. Reference-named entry points with the exact prototypes of
+ *      src/fluid.h:112-126 and src/hash.h:50-52, so the reference's main
+ *      loop (src/fluid.c:270-372) links against this library unchanged.
+ *
+ * Error convention of layer 2: 0 on success, a negative SPH_E* code on
+ * failure; sph_last_error() returns a message for the calling thread's last
+ * failure.  Layer 3 keeps the reference's void returns and records failures
+ * the same way.
+ *
+ * Threading: one host thread per context; every operation is ordered on the
+ * context's own HIP stream.  Calls on one context from several threads race.
+ */
+#ifndef SPH_ABI_H
+#define SPH_ABI_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#define SPH_STATIC_ASSERT(c, m) static_assert(c, m)
+#else
+#define SPH_STATIC_ASSERT(c, m) _Static_assert(c, m)
+#endif
+
+/* ------------------------------------------------------------------------
+ * 1. Reference records (layout-identical)
+ * ---------------------------------------------------------------------- */
+
+#define SPHERE_MOVER 0    /* src/fluid.h:48 */
+#define RECTANGLE_MOVER 1 /* src/fluid.h:49 */
+
+typedef struct FLUID_PARTICLE fluid_particle;
+typedef struct NEIGHBOR neighbor;
+typedef struct PARAM param;
+typedef struct TUNABLE_PARAMETERS tunable_parameters;
+typedef struct AABB_T AABB_t;
+typedef struct BUCKET_T bucket_t;
+typedef struct NEIGHBOR_GRID_T neighbor_grid_t;
+/* MPI-only records (src/communication.h:45-63); only passed by pointer. */
+typedef struct EDGE_T edge_t;
+typedef struct OOB_T oob_t;
+
+/* src/fluid.h:56-70 - 52 bytes. id is the particle's index in the
+ * reference's pointer array; on one domain it never changes. */
+struct FLUID_PARTICLE {
+    float x_prev;
+    float y_prev;
+    float x;
+    float y;
+    float v_x;
+    float v_y;
+    float a_x;
+    float a_y;
+    float density;
+    float density_near;
+    float pressure;
+    float pressure_near;
+    int id;
+};
+
+/* src/fluid.h:72-75 */
+struct NEIGHBOR {
+    fluid_particle **fluid_neighbors;
+    int number_fluid_neighbors;
+};
+
+/* src/fluid.h:78-97 - 64 bytes */
+struct TUNABLE_PARAMETERS {
+    float rest_density;
+    float smoothing_radius;
+    float g;
+    float k;
+    float k_near;
+    float k_spring;
+    float sigma;
+    float beta;
+    float time_step;
+    float node_start_x;
+    float node_end_x;
+    float mover_center_x;
+    float mover_center_y;
+    float mover_width;
+    float mover_height;
+    char mover_type;
+    char kill_sim;
+    char active;
+};
+
+/* src/fluid.h:100-106 - 80 bytes */
+struct PARAM {
+    tunable_parameters tunable_params;
+    int number_fluid_particles_global;
+    int number_fluid_particles_local;
+    int max_fluid_particle_index;
+    int number_halo_particles;
+};
+
+/* src/geometry.h:33-40 - 24 bytes */
+struct AABB_T {
+    float min_x;
+    float max_x;
+    float min_y;
+    float max_y;
+    float min_z;
+    float max_z;
+};
+
+/* src/hash.h:35-38 */
+struct BUCKET_T {
+    fluid_particle **fluid_particles;
+    unsigned int number_fluid;
+};
+
+/* src/hash.h:40-48 */
+struct NEIGHBOR_GRID_T {
+    float spacing;
+    unsigned int size_x;
+    unsigned int size_y;
+    neighbor *neighbors;
+    bucket_t *grid_buckets;
+    unsigned int max_neighbors;
+    unsigned int max_bucket_size;
+};
+
+SPH_STATIC_ASSERT(sizeof(fluid_particle) == 52, "fluid_particle must be 52 bytes");
+SPH_STATIC_ASSERT(offsetof(fluid_particle, x) == 8, "fluid_particle.x offset");
+SPH_STATIC_ASSERT(offsetof(fluid_particle, v_x) == 16, "fluid_particle.v_x offset");
+SPH_STATIC_ASSERT(offsetof(fluid_particle, density) == 32, "fluid_particle.density offset");
+SPH_STATIC_ASSERT(offsetof(fluid_particle, id) == 48, "fluid_particle.id offset");
+SPH_STATIC_ASSERT(sizeof(tunable_parameters) == 64, "tunable_parameters must be 64 bytes");
+SPH_STATIC_ASSERT(offsetof(tunable_parameters, mover_type) == 60, "mover_type offset");
+SPH_STATIC_ASSERT(offsetof(tunable_parameters, active) == 62, "active offset");
+SPH_STATIC_ASSERT(sizeof(param) == 80, "param must be 80 bytes");
+SPH_STATIC_ASSERT(offsetof(param, number_fluid_particles_global) == 64, "param.global offset");
+SPH_STATIC_ASSERT(offsetof(param, number_halo_particles) == 76, "param.halo offset");
+SPH_STATIC_ASSERT(sizeof(AABB_t) == 24, "AABB_t must be 24 bytes");
+SPH_STATIC_ASSERT(sizeof(bucket_t) == 16, "bucket_t must be 16 bytes");
+SPH_STATIC_ASSERT(sizeof(neighbor) == 16, "neighbor must be 16 bytes");
+SPH_STATIC_ASSERT(sizeof(neighbor_grid_t) == 40, "neighbor_grid_t must be 40 bytes");
+
+/* ------------------------------------------------------------------------
+ * 2. Device-resident context API
+ * ---------------------------------------------------------------------- */
+
+#define SPH_OK 0
+#define SPH_EINVAL (-1)    /* bad argument (null pointer, size, id set ...) */
+#define SPH_ENOMEM (-2)    /* device or host allocation failed             */
+#define SPH_EHIP (-3)      /* a HIP runtime call failed                     */
+#define SPH_ESTATE (-4)    /* call out of order (e.g. step before upload)   */
+#define SPH_ENODEV (-5)    /* no usable gfx950 device                       */
+
+/* Capacity limits the reference silently applies (src/fluid.c:173-175).
+ * The device path has no cap; it counts how often the reference's caps
+ * would have dropped work, because parity with the reference holds only
+ * while those counts stay zero. */
+#define SPH_REF_MAX_BUCKET 100
+#define SPH_REF_MAX_NEIGHBORS 400
+
+typedef struct sph_ctx sph_ctx;
+
+/* Device-side health counters, accumulated since the last reset. */
+typedef struct {
+    uint64_t steps;             /* full steps taken                               */
+    uint64_t hashes;            /* cell sorts performed                           */
+    uint64_t bucket_overflow;   /* cells that held more than 100 particles        */
+    uint64_t neighbor_overflow; /* particles with more than 400 half-neighbours   */
+    uint64_t key_out_of_range;  /* keys >= cell count (x==max_x on the top row,
+                                   y==max_y, NaN); clamped to the last cell       */
+    uint64_t nonfinite;         /* particles with a NaN/inf coordinate after a step */
+    uint64_t max_cell_count;    /* largest cell occupancy seen                    */
+    uint64_t max_neighbors;     /* largest full (both-way) neighbour count seen   */
+} sph_counters_t;
+
+/* Create a context for up to `capacity` particles on the current device.
+ * `params` supplies the tunables (smoothing radius = grid spacing, as in
+ * src/fluid.c:173-176); `boundary` is the simulation box.  The box must
+ * start at the origin (the reference assumes so: src/fluid.c:117, hash.c:41). */
+int sph_create(sph_ctx **ctx, const param *params, const AABB_t *boundary, int capacity);
+void sph_destroy(sph_ctx *ctx);
+
+/* Upload n particles (AoS, reference layout).  Particle ids must be a
+ * permutation of 0..n-1; id k is the particle's pointer-array index in the
+ * reference.  Marks the neighbour lists invalid, as a fresh reference run
+ * has them (src/fluid.c:202 calloc): the next step's viscosity is a no-op. */
+int sph_upload(sph_ctx *ctx, const fluid_particle *particles, int n);
+/* Same, from the reference's pointer array (src/fluid.c:197). */
+int sph_upload_pointers(sph_ctx *ctx, fluid_particle *const *fluid_particle_pointers, int n);
+
+/* Replace the tunables used by subsequent steps (the per-frame
+ * MPI_Scatterv of src/fluid.c:293-294).  Never triggers graph re-capture. */
+int sph_set_tunables(sph_ctx *ctx, const tunable_parameters *tunables);
+
+/* Run nsteps full steps (src/fluid.c:273-348 with MPI removed) through the
+ * captured HIP graph.  Returns once the work is enqueued; see sph_sync. */
+int sph_step(sph_ctx *ctx, int nsteps);
+/* Same, launching kernels one by one instead of replaying the graph. */
+int sph_step_eager(sph_ctx *ctx, int nsteps);
+int sph_sync(sph_ctx *ctx);
+
+/* Copy the state back into reference-layout AoS, indexed by id:
+ * out[id] for id in 0..n-1.  Fills every field the reference's loop leaves
+ * behind after a step (x_prev/density/pressure from that step). */
+int sph_download(sph_ctx *ctx, fluid_particle *out);
+/* Pack coordinates as the reference does for its renderer
+ * (src/fluid.c:356-363): coords[2*id] = (2x/max_x - 1)*SHRT_MAX, same for y. */
+int sph_pack_coords_i16(sph_ctx *ctx, short *coords);
+
+int sph_get_counters(sph_ctx *ctx, sph_counters_t *out);
+int sph_reset_counters(sph_ctx *ctx);
+int sph_num_particles(const sph_ctx *ctx);
+const char *sph_last_error(void);
+
+/* Phase-level calls.  A full step is exactly
+ *   gravity, viscosity, predict, hash(1), relax, update_velocities, hash(0)
+ * matching src/fluid.c:273-341.  The fused step computes the same values
+ * with fewer kernels. */
+int sph_phase_gravity(sph_ctx *ctx);            /* src/fluid.c:398-413 */
+int sph_phase_viscosity(sph_ctx *ctx);          /* src/fluid.c:416-478 */
+int sph_phase_predict(sph_ctx *ctx);            /* src/fluid.c:507-523 */
+int sph_phase_hash(sph_ctx *ctx, int compute_density); /* src/hash.c:127-242 */
+int sph_phase_relax(sph_ctx *ctx);              /* src/fluid.c:541-611 */
+int sph_phase_update_velocities(sph_ctx *ctx);  /* src/fluid.c:642-653 */
+
+/* Read-back of the device state in its current (cell, id) sorted order,
+ * for tests and tools. field: one of SPH_FIELD_*; out has n elements
+ * (float, or int for SPH_FIELD_ID / unsigned for SPH_FIELD_KEY). */
+#define SPH_FIELD_X 0
+#define SPH_FIELD_Y 1
+#define SPH_FIELD_VX 2
+#define SPH_FIELD_VY 3
+#define SPH_FIELD_XPREV 4
+#define SPH_FIELD_YPREV 5
+#define SPH_FIELD_DENSITY 6
+#define SPH_FIELD_DENSITY_NEAR 7
+#define SPH_FIELD_ID 8
+#define SPH_FIELD_KEY 9
+int sph_read_sorted(sph_ctx *ctx, int field, void *out);
+/* Cell table of the last hash: start[c], count[c] for c < cells. */
+int sph_read_cells(sph_ctx *ctx, unsigned int *start, unsigned int *count);
+int sph_num_cells(const sph_ctx *ctx, unsigned int *size_x, unsigned int *size_y);
+
+/* Per-kernel timing: run nsteps full steps launching kernel by kernel with
+ * a HIP event pair around every launch on the context stream, and return
+ * the mean duration (ms) of each kernel kind.  names/ms have room for
+ * max_kinds entries; *nkinds receives the count. */
+int sph_time_kernels(sph_ctx *ctx, int nsteps, int max_kinds, const char **names,
+                     float *ms, int *nkinds);
+
+/* Host-side scene builder: the reference's lattice (src/geometry.c:29-67,
+ * row-major, y outer, id = index) of nx*ny particles at `spacing`, lower
+ * left corner (x0, y0), plus a seeded uniform jitter of +-jitter per
+ * coordinate (SplitMix64, see scene.c).  v = a = 0.  out has nx*ny slots. */
+int sph_scene_lattice(fluid_particle *out, int nx, int ny, float x0, float y0, float spacing,
+                      float jitter, uint64_t seed);
+
+/* ------------------------------------------------------------------------
+ * 3. Reference-named entry points (drop-in for src/fluid.h:112-126 and
+ *    src/hash.h:50-52).  They drive one process-global context that is
+ *    created from the first call's arguments and uploads lazily from the
+ *    caller's pointer array.  After hash_fluid(..., false) (the last call
+ *    of a step, src/fluid.c:341) the state is copied back into the caller's
+ *    particles every sph_shim_set_sync_interval() steps (default 1), so the
+ *    caller's coordinate packing (src/fluid.c:356-363) reads current data.
+ *    neighbor / neighbor_grid_t list storage is accepted and left untouched:
+ *    neighbour sets live on the device.
+ * ---------------------------------------------------------------------- */
+void boundaryConditions(fluid_particle *p, AABB_t *boundary, param *params);
+void initParticles(fluid_particle **fluid_particle_pointers, fluid_particle *fluid_particles,
+                   AABB_t *water, int start_x, int number_particles_x, edge_t *edges,
+                   int max_fluid_particles_local, float spacing, param *params);
+void calculate_density(fluid_particle *p, fluid_particle *q, float ratio);
+void apply_gravity(fluid_particle **fluid_particle_pointers, param *params);
+void viscosity_impluses(fluid_particle **fluid_particle_pointers, neighbor *neighbors,
+                        param *params);
+void predict_positions(fluid_particle **fluid_particle_pointers, AABB_t *boundary_global,
+                       param *params);
+void double_density_relaxation(fluid_particle **fluid_particle_pointers, neighbor *neighbors,
+                               param *params);
+void updateVelocity(fluid_particle *p, param *params);
+void updateVelocities(fluid_particle **fluid_particle_pointers, edge_t *edges,
+                      AABB_t *boundary_global, param *params);
+void checkVelocity(float *v_x, float *v_y);
+void identify_oob_particles(fluid_particle **fluid_particle_pointers,
+                            fluid_particle *fluid_particles, oob_t *out_of_bounds,
+                            AABB_t *boundary_global, param *params);
+unsigned int hash_val(float x, float y, neighbor_grid_t *grid, param *params);
+void hash_fluid(fluid_particle **fluid_particle_pointers, neighbor_grid_t *grid, param *params,
+                bool compute_density);
+void hash_halo(fluid_particle **fluid_particle_pointers, neighbor_grid_t *grid, param *params,
+               bool compute_density);
+
+/* Shim controls (no reference counterpart). */
+int sph_shim_set_sync_interval(int steps); /* 0 = never copy back automatically */
+int sph_shim_sync(void);                   /* copy device state to the caller now */
+int sph_shim_mark_host_dirty(void);        /* re-upload from the caller on next call */
+void sph_shim_release(void);               /* destroy the process-global context */
+sph_ctx *sph_shim_context(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPH_ABI_H */

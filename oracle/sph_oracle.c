@@ -1,0 +1,907 @@
+/*
+ * sph_oracle.c - CPU oracle for the TinySPH step (AdamSimpson/SPH).
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (libsph_hip.so, the
+ * sph_amd package's solver path) links, loads or calls this file; only the
+ * tests, the smoke check and the bench's CPU-baseline leg do, as a checker.
+ *
+ * Built twice from this one source: REAL=float (suffix _f32, the parity
+ * definition) and REAL=double (suffix _f64, for error reporting).  Built
+ * with -O2 -ffp-contract=off and no fast-math: every float expression is
+ * evaluated exactly as written, operation by operation.
+ *
+ * Two restatements of the same step:
+ *
+ *  (A) "ref" - the reference's own semantics (scatter / Gauss-Seidel):
+ *      pointer buckets capped at max_bucket, half ("forward") neighbour lists
+ *      capped at max_neighbors, density scattered during the hash walk, and
+ *      viscosity / relaxation applied in place in reverse index order.
+ *      Follows src/hash.c:35-242 and src/fluid.c:398-653, 656-744 operation
+ *      by operation.  Pinned bit-for-bit against the compiled reference
+ *      (oracle/_ref) through the committed golden fixtures.
+ *
+ *  (B) "gather" - the same physics in gather / Jacobi form, the form the
+ *      GPU kernels compute: every pair term is evaluated from the state at
+ *      the start of the pass, and each particle sums its own terms over its
+ *      neighbours in one canonical order: the 3x3 cells around its cell in
+ *      ascending cell index, and within a cell ascending id.  Because every
+ *      pair term of the reference is exactly antisymmetric in floating
+ *      point (a-b == -(b-a), x*(-y) == -(x*y)), a particle's update in
+ *      gather form is "its own sequence of reference updates" with the
+ *      partner's contributions taken from the pre-pass state.  Only the
+ *      1e-6 nudge of nearly coincident pairs (src/fluid.c:583-586) depends
+ *      on which particle of a pair owns the half-list entry; the owner rule
+ *      is restated from src/hash.c:176-237 (same cell: lower id; otherwise
+ *      the particle whose cell has the other in a forward cell
+ *      (0,+1),(+1,-1),(+1,0),(+1,+1)).
+ *
+ * Cell keys (src/hash.c:35-47): key = floor(y/h)*size_x + floor(x/h), float
+ * division, floor, then unsigned.  A key >= size_x*size_y is undefined
+ * behaviour in the reference (out-of-bounds bucket write); here it is
+ * counted and clamped to the last cell, exactly as the device code does.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/sph_abi.h"
+
+#ifndef REAL_IS_DOUBLE
+typedef float REAL;
+#define RC(x) x##f
+#define OFN(name) oracle_##name##_f32
+#define RSQRT(v) sqrtf(v)
+#else
+typedef double REAL;
+#define RC(x) x
+#define OFN(name) oracle_##name##_f64
+#define RSQRT(v) sqrt(v)
+#endif
+
+/* Particle record: for REAL=float this is layout-identical to the
+ * reference's fluid_particle (src/fluid.h:56-70). */
+typedef struct {
+    REAL x_prev, y_prev, x, y, v_x, v_y, a_x, a_y;
+    REAL density, density_near, pressure, pressure_near;
+    int id;
+} o_particle;
+
+/* Tunables converted to REAL once (the reference reads them as float). */
+typedef struct {
+    REAL rest_density, h, g, k, k_near, k_spring, sigma, beta, dt;
+    REAL mover_cx, mover_cy, mover_w, mover_h;
+    int mover_type;
+} o_tun;
+
+static void tun_from(o_tun *t, const tunable_parameters *p)
+{
+    t->rest_density = (REAL)p->rest_density;
+    t->h = (REAL)p->smoothing_radius;
+    t->g = (REAL)p->g;
+    t->k = (REAL)p->k;
+    t->k_near = (REAL)p->k_near;
+    t->k_spring = (REAL)p->k_spring;
+    t->sigma = (REAL)p->sigma;
+    t->beta = (REAL)p->beta;
+    t->dt = (REAL)p->time_step;
+    t->mover_cx = (REAL)p->mover_center_x;
+    t->mover_cy = (REAL)p->mover_center_y;
+    t->mover_w = (REAL)p->mover_width;
+    t->mover_h = (REAL)p->mover_height;
+    t->mover_type = p->mover_type;
+}
+
+typedef struct {
+    REAL min_x, max_x, min_y, max_y;
+} o_box;
+
+/* Grid extent, src/fluid.c:214-215: ceil(float extent / float spacing).
+ * Computed in float in both builds so the grid is the same discrete object. */
+static void grid_size(const AABB_t *b, float h, unsigned *sx, unsigned *sy)
+{
+    *sx = (unsigned)ceil((b->max_x - b->min_x) / h);
+    *sy = (unsigned)ceil((b->max_y - b->min_y) / h);
+}
+
+/* src/hash.c:35-47 (+ clamp of the undefined out-of-range case).  The key
+ * is formed in 64 bits so that only keys the reference could index are kept. */
+static unsigned cell_key(REAL x, REAL y, REAL h, unsigned sx, unsigned ncells, uint64_t *oor)
+{
+    REAL qx = x / h, qy = y / h;
+    if (!(qx >= RC(0.0) && qy >= RC(0.0) && qx < RC(4294967296.0) && qy < RC(4294967296.0))) {
+        ++*oor; /* negative, NaN or huge: undefined in the reference */
+        return ncells - 1;
+    }
+    unsigned long long k =
+        (unsigned long long)(unsigned)floor(qy) * sx + (unsigned long long)(unsigned)floor(qx);
+    if (k >= ncells) {
+        ++*oor;
+        return ncells - 1;
+    }
+    return (unsigned)k;
+}
+
+/* src/fluid.c:613-625 */
+static void check_velocity(REAL *vx, REAL *vy)
+{
+    const REAL v_max = RC(5.0);
+    if (*vx > v_max)
+        *vx = v_max;
+    else if (*vx < -v_max)
+        *vx = -v_max;
+    if (*vy > v_max)
+        *vy = v_max;
+    else if (*vy < -v_max)
+        *vy = -v_max;
+}
+
+/* src/fluid.c:656-744: mover push-out, then clamp into the box. */
+static void boundary(REAL *px, REAL *py, const o_box *b, const o_tun *t)
+{
+    REAL cx = t->mover_cx, cy = t->mover_cy;
+    if (t->mover_type == SPHERE_MOVER) {
+        REAL radius = t->mover_w * RC(0.5);
+        REAL d2 = (*px - cx) * (*px - cx) + (*py - cy) * (*py - cy);
+        if (d2 <= radius * radius && d2 > RC(0.0)) {
+            REAL d = RSQRT(d2);
+            REAL nx = (cx - *px) / d;
+            REAL ny = (cy - *py) / d;
+            REAL pen = radius - d;
+            *px -= pen * nx;
+            *py -= pen * ny;
+        }
+    } else if (t->mover_type == RECTANGLE_MOVER) {
+        REAL hw = (REAL)(t->mover_w * 0.5); /* double literal, exact halving */
+        REAL hh = (REAL)(t->mover_h * 0.5);
+        REAL rx = *px - cx, ry = *py - cy;
+        REAL ax = fabs(rx), ay = fabs(ry);
+        if (ax < hw && ay < hh) {
+            REAL penx = hw - ax, peny = hh - ay;
+            if (penx < peny) {
+                if (rx < RC(0.0))
+                    *px -= penx;
+                else
+                    *px += penx;
+            } else {
+                if (ry < RC(0.0))
+                    *py -= peny;
+                else
+                    *py += peny;
+            }
+        }
+    }
+    if (*px < b->min_x)
+        *px = b->min_x;
+    else if (*px > b->max_x)
+        *px = b->max_x - RC(0.001);
+    if (*py < b->min_y)
+        *py = b->min_y;
+    else if (*py > b->max_y)
+        *py = b->max_y - RC(0.001);
+}
+
+/* Relaxation displacement magnitude, src/fluid.c:591.  The trailing 0.5 is
+ * a double literal, so the spring term, the sum with it and the product
+ * with dt*dt are evaluated in double and rounded to REAL once. */
+static REAL relax_D(REAL dt, REAL Psum, REAL Pnsum, REAL OmR, REAL k_spring, REAL h, REAL r)
+{
+    REAL a = Psum * OmR + Pnsum * OmR * OmR;
+    REAL s = k_spring * (h - r);
+    REAL dt2 = dt * dt;
+    return (REAL)((double)dt2 * ((double)a + (double)s * 0.5));
+}
+
+/* Viscosity impulse of pair (p,q) evaluated with p in the owner role,
+ * src/fluid.c:442-459.  Returns 0 when no impulse applies (u <= 0). */
+static int visc_impulse(REAL px, REAL py, REAL pvx, REAL pvy, REAL qx, REAL qy, REAL qvx, REAL qvy,
+                        const o_tun *t, REAL h_recip, REAL *imp_x, REAL *imp_y)
+{
+    REAL QmP_x = qx - px;
+    REAL QmP_y = qy - py;
+    REAL r = RSQRT(QmP_x * QmP_x + QmP_y * QmP_y);
+    REAL r_recip = RC(1.0) / r;
+    REAL ratio = r * h_recip;
+    REAL u = ((pvx - qvx) * QmP_x + (pvy - qvy) * QmP_y) * r_recip;
+    if (u > RC(0.0)) {
+        REAL imp = t->dt * (RC(1.0) - ratio) * (t->sigma * u + t->beta * u * u);
+        *imp_x = imp * QmP_x * r_recip;
+        *imp_y = imp * QmP_y * r_recip;
+        check_velocity(imp_x, imp_y);
+        return 1;
+    }
+    return 0;
+}
+
+/* ======================================================================
+ * (A) Reference semantics: scatter / Gauss-Seidel
+ * ==================================================================== */
+
+typedef struct {
+    int n;
+    o_box box;
+    o_tun t;
+    float h_f; /* spacing as float, for grid sizing */
+    unsigned sx, sy, ncells;
+    unsigned max_bucket, max_neighbors;
+    unsigned *bucket_count; /* ncells            */
+    int *bucket;            /* ncells*max_bucket */
+    int *nb_count;          /* n                 */
+    int *nb;                /* n*max_neighbors   */
+    uint64_t bucket_overflow, neighbor_overflow, key_oor;
+} oref;
+
+oref *OFN(ref_create)(const AABB_t *box, const tunable_parameters *tp, int n, unsigned max_bucket,
+                      unsigned max_neighbors)
+{
+    oref *s = (oref *)calloc(1, sizeof(oref));
+    if (!s)
+        return NULL;
+    s->n = n;
+    s->box.min_x = box->min_x;
+    s->box.max_x = box->max_x;
+    s->box.min_y = box->min_y;
+    s->box.max_y = box->max_y;
+    tun_from(&s->t, tp);
+    s->h_f = tp->smoothing_radius;
+    grid_size(box, s->h_f, &s->sx, &s->sy);
+    s->ncells = s->sx * s->sy;
+    s->max_bucket = max_bucket;
+    s->max_neighbors = max_neighbors;
+    s->bucket_count = (unsigned *)calloc(s->ncells, sizeof(unsigned));
+    s->bucket = (int *)calloc((size_t)s->ncells * max_bucket, sizeof(int));
+    /* lists start empty (src/fluid.c:202 calloc): first viscosity is a no-op */
+    s->nb_count = (int *)calloc((size_t)n, sizeof(int));
+    s->nb = (int *)calloc((size_t)n * max_neighbors, sizeof(int));
+    if (!s->bucket_count || !s->bucket || !s->nb_count || !s->nb) {
+        free(s->bucket_count);
+        free(s->bucket);
+        free(s->nb_count);
+        free(s->nb);
+        free(s);
+        return NULL;
+    }
+    return s;
+}
+
+void OFN(ref_destroy)(oref *s)
+{
+    if (!s)
+        return;
+    free(s->bucket_count);
+    free(s->bucket);
+    free(s->nb_count);
+    free(s->nb);
+    free(s);
+}
+
+void OFN(ref_set_tunables)(oref *s, const tunable_parameters *tp) { tun_from(&s->t, tp); }
+
+/* src/fluid.c:398-413 */
+static void ref_gravity(oref *s, o_particle *P)
+{
+    REAL dt = s->t.dt, g = -s->t.g;
+    for (int i = 0; i < s->n; i++) {
+        P[i].v_y += g * dt;
+        P[i].density = RC(0.0);
+        P[i].density_near = RC(0.0);
+    }
+}
+
+/* src/fluid.c:416-478 (no halo on one domain) */
+static void ref_viscosity(oref *s, o_particle *P)
+{
+    REAL h_recip = RC(1.0) / s->t.h;
+    for (int i = s->n; i-- > 0;) {
+        o_particle *p = &P[i];
+        REAL px = p->x, py = p->y;
+        for (int j = 0; j < s->nb_count[i]; j++) {
+            o_particle *q = &P[s->nb[(size_t)i * s->max_neighbors + j]];
+            REAL ix, iy;
+            if (visc_impulse(px, py, p->v_x, p->v_y, q->x, q->y, q->v_x, q->v_y, &s->t, h_recip,
+                             &ix, &iy)) {
+                p->v_x -= ix * RC(0.5);
+                p->v_y -= iy * RC(0.5);
+                q->v_x += ix * RC(0.5);
+                q->v_y += iy * RC(0.5);
+            }
+        }
+    }
+}
+
+/* src/fluid.c:507-523 */
+static void ref_predict(oref *s, o_particle *P)
+{
+    REAL dt = s->t.dt;
+    for (int i = 0; i < s->n; i++) {
+        o_particle *p = &P[i];
+        p->x_prev = p->x;
+        p->y_prev = p->y;
+        p->x += p->v_x * dt;
+        p->y += p->v_y * dt;
+        boundary(&p->x, &p->y, &s->box, &s->t);
+    }
+}
+
+/* src/fluid.c:527-539 */
+static void ref_density_pair(o_particle *p, o_particle *q, REAL ratio)
+{
+    REAL OmR2 = (RC(1.0) - ratio) * (RC(1.0) - ratio);
+    if (ratio < RC(1.0)) {
+        p->density += OmR2;
+        p->density_near += OmR2 * (RC(1.0) - ratio);
+        q->density += OmR2;
+        q->density_near += OmR2 * (RC(1.0) - ratio);
+    }
+}
+
+/* Append q to p's half list, src/hash.c:188-197 / 223-232. */
+static void ref_append(oref *s, o_particle *P, int p, int q, REAL r2, int compute_density,
+                       REAL h_recip, int p_first)
+{
+    if ((unsigned)s->nb_count[p] < s->max_neighbors) {
+        s->nb[(size_t)p * s->max_neighbors + s->nb_count[p]++] = q;
+        if (compute_density) {
+            REAL r = RSQRT(r2);
+            REAL ratio = r * h_recip;
+            /* argument order as at hash.c:193 (p,q) and hash.c:228 (q_neighbor,q) */
+            if (p_first)
+                ref_density_pair(&P[p], &P[q], ratio);
+            else
+                ref_density_pair(&P[q], &P[p], ratio);
+        }
+    } else {
+        s->neighbor_overflow++;
+    }
+}
+
+/* src/hash.c:127-242 */
+static void ref_hash(oref *s, o_particle *P, int compute_density)
+{
+    REAL h = s->t.h;
+    REAL h_recip = RC(1.0) / h;
+    REAL h2 = h * h;
+    unsigned sx = s->sx, sy = s->sy, mb = s->max_bucket;
+
+    memset(s->bucket_count, 0, sizeof(unsigned) * s->ncells);
+    for (int i = 0; i < s->n; i++) {
+        s->nb_count[i] = 0;
+        unsigned key = cell_key(P[i].x, P[i].y, h, sx, s->ncells, &s->key_oor);
+        if (s->bucket_count[key] < mb)
+            s->bucket[(size_t)key * mb + s->bucket_count[key]++] = i;
+        else
+            s->bucket_overflow++;
+    }
+
+    for (unsigned j = 0; j < sy; j++) {
+        for (unsigned i = 0; i < sx; i++) {
+            unsigned index = j * sx + i;
+            unsigned cnt = s->bucket_count[index];
+            if (cnt == 0)
+                continue;
+            const int *b = &s->bucket[(size_t)index * mb];
+            /* own bucket: pairs c < n, appended to the earlier particle */
+            for (unsigned c = 0; c < cnt; c++) {
+                int p = b[c];
+                for (unsigned m = c + 1; m < cnt; m++) {
+                    int q = b[m];
+                    REAL dx = P[p].x - P[q].x, dy = P[p].y - P[q].y;
+                    REAL r2 = dx * dx + dy * dy;
+                    if (r2 > h2)
+                        continue;
+                    ref_append(s, P, p, q, r2, compute_density, h_recip, 1);
+                }
+            }
+            /* forward cells (0,+1), (+1,-1), (+1,0), (+1,+1) */
+            for (int dx = 0; dx <= 1; dx++) {
+                for (int dy = (dx ? -1 : 1); dy <= 1; dy++) {
+                    long ni = (long)i + dx, nj = (long)j + dy;
+                    if (nj < 0 || ni < 0 || ni >= (long)sx || nj >= (long)sy)
+                        continue;
+                    unsigned nidx = (unsigned)nj * sx + (unsigned)ni;
+                    const int *nbk = &s->bucket[(size_t)nidx * mb];
+                    unsigned ncnt = s->bucket_count[nidx];
+                    for (unsigned c = 0; c < cnt; c++) {
+                        int q = b[c];
+                        for (unsigned m = 0; m < ncnt; m++) {
+                            int qn = nbk[m];
+                            REAL ddx = P[qn].x - P[q].x, ddy = P[qn].y - P[q].y;
+                            REAL r2 = ddx * ddx + ddy * ddy;
+                            if (r2 > h2)
+                                continue;
+                            ref_append(s, P, q, qn, r2, compute_density, h_recip, 0);
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* src/fluid.c:541-611 */
+static void ref_relax(oref *s, o_particle *P)
+{
+    const o_tun *t = &s->t;
+    REAL h = t->h, h_recip = RC(1.0) / h, dt = t->dt;
+    for (int i = 0; i < s->n; i++) {
+        P[i].pressure = t->k * (P[i].density - t->rest_density);
+        P[i].pressure_near = t->k_near * P[i].density_near;
+    }
+    for (int i = s->n; i-- > 0;) {
+        o_particle *p = &P[i];
+        REAL pP = p->pressure, pPn = p->pressure_near;
+        for (int j = 0; j < s->nb_count[i]; j++) {
+            o_particle *q = &P[s->nb[(size_t)i * s->max_neighbors + j]];
+            REAL r = RSQRT((p->x - q->x) * (p->x - q->x) + (p->y - q->y) * (p->y - q->y));
+            REAL r_recip = RC(1.0) / r;
+            REAL ratio = r * h_recip;
+            REAL OmR = RC(1.0) - ratio;
+            if (r <= RC(0.000001)) {
+                p->x += RC(0.000001);
+                p->y += RC(0.000001);
+            }
+            if (ratio < RC(1.0) && r > RC(0.0)) {
+                REAL D = relax_D(dt, pP + q->pressure, pPn + q->pressure_near, OmR, t->k_spring, h, r);
+                REAL Dx = D * (q->x - p->x) * r_recip;
+                REAL Dy = D * (q->y - p->y) * r_recip;
+                q->x += Dx;
+                q->y += Dy;
+                p->x -= Dx;
+                p->y -= Dy;
+            }
+        }
+    }
+}
+
+/* src/fluid.c:627-653 */
+static void ref_update_velocities(oref *s, o_particle *P)
+{
+    REAL dt = s->t.dt;
+    for (int i = 0; i < s->n; i++) {
+        o_particle *p = &P[i];
+        boundary(&p->x, &p->y, &s->box, &s->t);
+        REAL vx = (p->x - p->x_prev) / dt;
+        REAL vy = (p->y - p->y_prev) / dt;
+        check_velocity(&vx, &vy);
+        p->v_x = vx;
+        p->v_y = vy;
+    }
+}
+
+#define PH_GRAVITY 0
+#define PH_VISCOSITY 1
+#define PH_PREDICT 2
+#define PH_HASH_DENSITY 3
+#define PH_RELAX 4
+#define PH_UPDATE_VELOCITIES 5
+#define PH_HASH 6
+
+int OFN(ref_phase)(oref *s, o_particle *P, int phase)
+{
+    switch (phase) {
+    case PH_GRAVITY: ref_gravity(s, P); break;
+    case PH_VISCOSITY: ref_viscosity(s, P); break;
+    case PH_PREDICT: ref_predict(s, P); break;
+    case PH_HASH_DENSITY: ref_hash(s, P, 1); break;
+    case PH_RELAX: ref_relax(s, P); break;
+    case PH_UPDATE_VELOCITIES: ref_update_velocities(s, P); break;
+    case PH_HASH: ref_hash(s, P, 0); break;
+    default: return -1;
+    }
+    return 0;
+}
+
+/* One step = src/fluid.c:273-341 with MPI and the one-domain no-ops
+ * (identify_oob_particles, halo exchange, hash_halo) removed. */
+void OFN(ref_step)(oref *s, o_particle *P, int nsteps)
+{
+    for (int it = 0; it < nsteps; it++)
+        for (int ph = PH_GRAVITY; ph <= PH_HASH; ph++)
+            OFN(ref_phase)(s, P, ph);
+}
+
+void OFN(ref_counters)(const oref *s, uint64_t out[3])
+{
+    out[0] = s->bucket_overflow;
+    out[1] = s->neighbor_overflow;
+    out[2] = s->key_oor;
+}
+
+/* Half lists after the last hash: counts[i], and ids into lists[i*stride+j]. */
+void OFN(ref_lists)(const oref *s, int *counts, int *lists, int stride)
+{
+    for (int i = 0; i < s->n; i++) {
+        int c = s->nb_count[i];
+        counts[i] = c;
+        if (lists)
+            for (int j = 0; j < c && j < stride; j++)
+                lists[(size_t)i * stride + j] = s->nb[(size_t)i * s->max_neighbors + j];
+    }
+}
+
+/* ======================================================================
+ * (B) Gather / Jacobi form (what the device computes)
+ * ==================================================================== */
+
+typedef struct {
+    int n;
+    o_box box;
+    o_tun t;
+    unsigned sx, sy, ncells;
+    REAL *x, *y, *vx, *vy, *xp, *yp, *rho, *rhon; /* by id */
+    REAL *tx, *ty;                                /* pass outputs */
+    unsigned *key;                                /* by id */
+    int *sorted;                                  /* sorted position -> id */
+    unsigned *start;                              /* ncells + 1 */
+    int lists_valid;
+    uint64_t bucket_overflow, neighbor_overflow, key_oor;
+    unsigned max_cell, max_full_neighbors;
+} ogat;
+
+ogat *OFN(gather_create)(const AABB_t *box, const tunable_parameters *tp, int n)
+{
+    ogat *s = (ogat *)calloc(1, sizeof(ogat));
+    if (!s)
+        return NULL;
+    s->n = n;
+    s->box.min_x = box->min_x;
+    s->box.max_x = box->max_x;
+    s->box.min_y = box->min_y;
+    s->box.max_y = box->max_y;
+    tun_from(&s->t, tp);
+    grid_size(box, tp->smoothing_radius, &s->sx, &s->sy);
+    s->ncells = s->sx * s->sy;
+    size_t nn = (size_t)(n > 0 ? n : 1);
+    REAL **f[] = {&s->x, &s->y, &s->vx, &s->vy, &s->xp, &s->yp, &s->rho, &s->rhon, &s->tx, &s->ty};
+    int ok = 1;
+    for (unsigned i = 0; i < sizeof(f) / sizeof(f[0]); i++)
+        ok &= (*f[i] = (REAL *)calloc(nn, sizeof(REAL))) != NULL;
+    s->key = (unsigned *)calloc(nn, sizeof(unsigned));
+    s->sorted = (int *)calloc(nn, sizeof(int));
+    s->start = (unsigned *)calloc((size_t)s->ncells + 1, sizeof(unsigned));
+    if (!ok || !s->key || !s->sorted || !s->start) {
+        for (unsigned i = 0; i < sizeof(f) / sizeof(f[0]); i++)
+            free(*f[i]);
+        free(s->key);
+        free(s->sorted);
+        free(s->start);
+        free(s);
+        return NULL;
+    }
+    return s;
+}
+
+void OFN(gather_destroy)(ogat *s)
+{
+    if (!s)
+        return;
+    REAL *f[] = {s->x, s->y, s->vx, s->vy, s->xp, s->yp, s->rho, s->rhon, s->tx, s->ty};
+    for (unsigned i = 0; i < sizeof(f) / sizeof(f[0]); i++)
+        free(f[i]);
+    free(s->key);
+    free(s->sorted);
+    free(s->start);
+    free(s);
+}
+
+void OFN(gather_set_tunables)(ogat *s, const tunable_parameters *tp) { tun_from(&s->t, tp); }
+
+/* State in/out as AoS indexed by id (P[id]).  set_state invalidates the
+ * neighbour lists, like a fresh reference run. */
+void OFN(gather_set_state)(ogat *s, const o_particle *P)
+{
+    for (int i = 0; i < s->n; i++) {
+        int id = P[i].id;
+        s->x[id] = P[i].x;
+        s->y[id] = P[i].y;
+        s->vx[id] = P[i].v_x;
+        s->vy[id] = P[i].v_y;
+        s->xp[id] = P[i].x_prev;
+        s->yp[id] = P[i].y_prev;
+        s->rho[id] = P[i].density;
+        s->rhon[id] = P[i].density_near;
+    }
+    s->lists_valid = 0;
+}
+
+void OFN(gather_get_state)(const ogat *s, o_particle *P)
+{
+    for (int id = 0; id < s->n; id++) {
+        o_particle *p = &P[id];
+        p->x = s->x[id];
+        p->y = s->y[id];
+        p->v_x = s->vx[id];
+        p->v_y = s->vy[id];
+        p->x_prev = s->xp[id];
+        p->y_prev = s->yp[id];
+        p->a_x = p->a_y = RC(0.0);
+        p->density = s->rho[id];
+        p->density_near = s->rhon[id];
+        p->pressure = s->t.k * (s->rho[id] - s->t.rest_density);
+        p->pressure_near = s->t.k_near * s->rhon[id];
+        p->id = id;
+    }
+}
+
+/* Counting sort of ids by key, stable in id order: the reference's bucket
+ * order is insertion order = pointer order = id (src/hash.c:153-163). */
+static void gather_sort(ogat *s)
+{
+    unsigned nc = s->ncells;
+    memset(s->start, 0, sizeof(unsigned) * (nc + 1));
+    for (int i = 0; i < s->n; i++) {
+        s->key[i] = cell_key(s->x[i], s->y[i], s->t.h, s->sx, nc, &s->key_oor);
+        s->start[s->key[i] + 1]++;
+    }
+    for (unsigned c = 0; c < nc; c++) {
+        unsigned cnt = s->start[c + 1];
+        if (cnt > SPH_REF_MAX_BUCKET)
+            s->bucket_overflow++;
+        if (cnt > s->max_cell)
+            s->max_cell = cnt;
+        s->start[c + 1] += s->start[c];
+    }
+    unsigned *fill = (unsigned *)malloc(sizeof(unsigned) * (nc ? nc : 1));
+    memcpy(fill, s->start, sizeof(unsigned) * nc);
+    for (int i = 0; i < s->n; i++)
+        s->sorted[fill[s->key[i]]++] = i;
+    free(fill);
+    s->lists_valid = 1;
+}
+
+/* Row ranges of the 3x3 neighbourhood of cell `key` in sorted order:
+ * rows j-1..j+1, cells i-1..i+1 clipped to the grid.  Returns row count. */
+static int gather_ranges(const ogat *s, unsigned key, unsigned lo[3], unsigned hi[3])
+{
+    unsigned sx = s->sx, sy = s->sy;
+    long i = key % sx, j = key / sx;
+    long i0 = i > 0 ? i - 1 : 0, i1 = i + 1 < (long)sx ? i + 1 : (long)sx - 1;
+    int nr = 0;
+    for (long r = j - 1; r <= j + 1; r++) {
+        if (r < 0 || r >= (long)sy)
+            continue;
+        lo[nr] = s->start[(unsigned)r * sx + (unsigned)i0];
+        hi[nr] = s->start[(unsigned)r * sx + (unsigned)i1 + 1];
+        nr++;
+    }
+    return nr;
+}
+
+/* Does particle a own the half-list entry of pair (a,b)?  src/hash.c:176-237 */
+static int owns(const ogat *s, int a, int b)
+{
+    unsigned ka = s->key[a], kb = s->key[b];
+    if (ka == kb)
+        return a < b;
+    long ia = ka % s->sx, ja = ka / s->sx, ib = kb % s->sx, jb = kb / s->sx;
+    long dx = ib - ia, dy = jb - ja;
+    return (dx == 0 && dy == 1) || (dx == 1 && dy >= -1 && dy <= 1);
+}
+
+static void gather_gravity(ogat *s)
+{
+    REAL dt = s->t.dt, g = -s->t.g;
+    for (int i = 0; i < s->n; i++) {
+        s->vy[i] += g * dt;
+        s->rho[i] = RC(0.0);
+        s->rhon[i] = RC(0.0);
+    }
+}
+
+static void gather_viscosity(ogat *s)
+{
+    if (!s->lists_valid)
+        return; /* empty lists before the first hash (src/fluid.c:202) */
+    REAL h = s->t.h, h2 = h * h, h_recip = RC(1.0) / h;
+    for (int a = 0; a < s->n; a++) {
+        unsigned lo[3], hi[3];
+        int nr = gather_ranges(s, s->key[a], lo, hi);
+        REAL ax = s->x[a], ay = s->y[a], avx = s->vx[a], avy = s->vy[a];
+        REAL nvx = avx, nvy = avy;
+        for (int r = 0; r < nr; r++)
+            for (unsigned k = lo[r]; k < hi[r]; k++) {
+                int b = s->sorted[k];
+                if (b == a)
+                    continue;
+                REAL dx = s->x[b] - ax, dy = s->y[b] - ay;
+                if (dx * dx + dy * dy > h2)
+                    continue;
+                REAL ix, iy;
+                if (visc_impulse(ax, ay, avx, avy, s->x[b], s->y[b], s->vx[b], s->vy[b], &s->t,
+                                 h_recip, &ix, &iy)) {
+                    nvx -= ix * RC(0.5);
+                    nvy -= iy * RC(0.5);
+                }
+            }
+        s->tx[a] = nvx;
+        s->ty[a] = nvy;
+    }
+    REAL *t;
+    t = s->vx; s->vx = s->tx; s->tx = t;
+    t = s->vy; s->vy = s->ty; s->ty = t;
+}
+
+static void gather_predict(ogat *s)
+{
+    REAL dt = s->t.dt;
+    for (int i = 0; i < s->n; i++) {
+        s->xp[i] = s->x[i];
+        s->yp[i] = s->y[i];
+        s->x[i] += s->vx[i] * dt;
+        s->y[i] += s->vy[i] * dt;
+        boundary(&s->x[i], &s->y[i], &s->box, &s->t);
+    }
+}
+
+static void gather_hash(ogat *s, int compute_density)
+{
+    gather_sort(s);
+    REAL h = s->t.h, h2 = h * h, h_recip = RC(1.0) / h;
+    for (int a = 0; a < s->n; a++) {
+        unsigned lo[3], hi[3];
+        int nr = gather_ranges(s, s->key[a], lo, hi);
+        REAL ax = s->x[a], ay = s->y[a];
+        REAL rho = RC(0.0), rhon = RC(0.0);
+        unsigned full = 0, half = 0;
+        for (int r = 0; r < nr; r++)
+            for (unsigned k = lo[r]; k < hi[r]; k++) {
+                int b = s->sorted[k];
+                if (b == a)
+                    continue;
+                REAL dx = ax - s->x[b], dy = ay - s->y[b];
+                REAL r2 = dx * dx + dy * dy;
+                if (r2 > h2)
+                    continue;
+                full++;
+                half += owns(s, a, b);
+                if (!compute_density)
+                    continue;
+                REAL ratio = RSQRT(r2) * h_recip;
+                REAL OmR2 = (RC(1.0) - ratio) * (RC(1.0) - ratio);
+                if (ratio < RC(1.0)) {
+                    rho += OmR2;
+                    rhon += OmR2 * (RC(1.0) - ratio);
+                }
+            }
+        if (compute_density) {
+            s->rho[a] = rho;
+            s->rhon[a] = rhon;
+        }
+        if (half > SPH_REF_MAX_NEIGHBORS)
+            s->neighbor_overflow++;
+        if (full > s->max_full_neighbors)
+            s->max_full_neighbors = full;
+    }
+}
+
+/* Relaxation + (optionally) boundary and velocity update for one particle,
+ * shared by the separate and fused forms. */
+static void gather_relax(ogat *s, int fuse_update)
+{
+    const o_tun *t = &s->t;
+    REAL h = t->h, h2 = h * h, h_recip = RC(1.0) / h, dt = t->dt;
+    for (int a = 0; a < s->n; a++) {
+        unsigned lo[3], hi[3];
+        int nr = gather_ranges(s, s->key[a], lo, hi);
+        REAL ax = s->x[a], ay = s->y[a];
+        REAL aP = t->k * (s->rho[a] - t->rest_density), aPn = t->k_near * s->rhon[a];
+        REAL nx = ax, ny = ay;
+        for (int r = 0; r < nr; r++)
+            for (unsigned k = lo[r]; k < hi[r]; k++) {
+                int b = s->sorted[k];
+                if (b == a)
+                    continue;
+                REAL bx = s->x[b], by = s->y[b];
+                REAL dx = ax - bx, dy = ay - by;
+                REAL r2 = dx * dx + dy * dy;
+                if (r2 > h2)
+                    continue;
+                REAL rr = RSQRT(r2);
+                REAL r_recip = RC(1.0) / rr;
+                REAL ratio = rr * h_recip;
+                REAL OmR = RC(1.0) - ratio;
+                if (rr <= RC(0.000001) && owns(s, a, b)) {
+                    nx += RC(0.000001);
+                    ny += RC(0.000001);
+                }
+                if (ratio < RC(1.0) && rr > RC(0.0)) {
+                    REAL bP = t->k * (s->rho[b] - t->rest_density), bPn = t->k_near * s->rhon[b];
+                    REAL D = relax_D(dt, aP + bP, aPn + bPn, OmR, t->k_spring, h, rr);
+                    REAL Dx = D * (bx - ax) * r_recip;
+                    REAL Dy = D * (by - ay) * r_recip;
+                    nx -= Dx;
+                    ny -= Dy;
+                }
+            }
+        if (fuse_update) {
+            boundary(&nx, &ny, &s->box, t);
+            REAL vx = (nx - s->xp[a]) / dt, vy = (ny - s->yp[a]) / dt;
+            check_velocity(&vx, &vy);
+            s->vx[a] = vx;
+            s->vy[a] = vy;
+        }
+        s->tx[a] = nx;
+        s->ty[a] = ny;
+    }
+    REAL *tmp;
+    tmp = s->x; s->x = s->tx; s->tx = tmp;
+    tmp = s->y; s->y = s->ty; s->ty = tmp;
+}
+
+static void gather_update_velocities(ogat *s)
+{
+    REAL dt = s->t.dt;
+    for (int i = 0; i < s->n; i++) {
+        boundary(&s->x[i], &s->y[i], &s->box, &s->t);
+        REAL vx = (s->x[i] - s->xp[i]) / dt, vy = (s->y[i] - s->yp[i]) / dt;
+        check_velocity(&vx, &vy);
+        s->vx[i] = vx;
+        s->vy[i] = vy;
+    }
+}
+
+int OFN(gather_phase)(ogat *s, int phase)
+{
+    switch (phase) {
+    case PH_GRAVITY: gather_gravity(s); break;
+    case PH_VISCOSITY: gather_viscosity(s); break;
+    case PH_PREDICT: gather_predict(s); break;
+    case PH_HASH_DENSITY: gather_hash(s, 1); break;
+    case PH_RELAX: gather_relax(s, 0); break;
+    case PH_UPDATE_VELOCITIES: gather_update_velocities(s); break;
+    case PH_HASH: gather_hash(s, 0); break;
+    default: return -1;
+    }
+    return 0;
+}
+
+void OFN(gather_step)(ogat *s, int nsteps)
+{
+    for (int it = 0; it < nsteps; it++)
+        for (int ph = PH_GRAVITY; ph <= PH_HASH; ph++)
+            OFN(gather_phase)(s, ph);
+}
+
+/* Cell structure of the last hash: keys by id, sorted ids, start[ncells+1]. */
+void OFN(gather_cells)(const ogat *s, unsigned *key_by_id, int *sorted_ids, unsigned *start)
+{
+    if (key_by_id)
+        memcpy(key_by_id, s->key, sizeof(unsigned) * s->n);
+    if (sorted_ids)
+        memcpy(sorted_ids, s->sorted, sizeof(int) * s->n);
+    if (start)
+        memcpy(start, s->start, sizeof(unsigned) * (s->ncells + 1));
+}
+
+void OFN(gather_counters)(const ogat *s, uint64_t out[5])
+{
+    out[0] = s->bucket_overflow;
+    out[1] = s->neighbor_overflow;
+    out[2] = s->key_oor;
+    out[3] = s->max_cell;
+    out[4] = s->max_full_neighbors;
+}
+
+/* Full (symmetric) neighbour sets at the current positions, from the last
+ * hash's cells: counts[id] and ascending ids in lists[id*stride ...]. */
+void OFN(gather_neighbor_sets)(const ogat *s, int *counts, int *lists, int stride)
+{
+    REAL h2 = s->t.h * s->t.h;
+    for (int a = 0; a < s->n; a++) {
+        unsigned lo[3], hi[3];
+        int nr = gather_ranges(s, s->key[a], lo, hi), c = 0;
+        for (int r = 0; r < nr; r++)
+            for (unsigned k = lo[r]; k < hi[r]; k++) {
+                int b = s->sorted[k];
+                if (b == a)
+                    continue;
+                REAL dx = s->x[a] - s->x[b], dy = s->y[a] - s->y[b];
+                if (dx * dx + dy * dy > h2)
+                    continue;
+                if (c < stride)
+                    lists[(size_t)a * stride + c] = b;
+                c++;
+            }
+        counts[a] = c;
+    }
+}

@@ -1,0 +1,264 @@
+"""sph_amd - Python host binding of libsph_hip.so, the MI355X (gfx950)
+implementation of TinySPH's step (AdamSimpson/SPH).
+
+The library's C ABI (include/sph_abi.h) is the product boundary; this module
+is a thin ctypes mirror of it for tests, the bench and Python callers.  It
+loads the in-tree ``sph_amd/libsph_hip.so`` and raises if that library is
+missing: there is no CPU fallback.
+
+Records keep the reference's layouts: particles are numpy structured arrays
+of ``PARTICLE`` (= fluid_particle, src/fluid.h:56-70, 52 bytes), tunables are
+``Tunables`` (src/fluid.h:78-97), boxes ``AABB`` (src/geometry.h:33-40).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["PARTICLE", "Tunables", "Param", "AABB", "Counters", "Solver", "SphError",
+           "make_tunables", "make_box", "scene_lattice", "lib", "LIB_PATH"]
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libsph_hip.so")
+
+_FIELDS = ("x_prev", "y_prev", "x", "y", "v_x", "v_y", "a_x", "a_y",
+           "density", "density_near", "pressure", "pressure_near")
+PARTICLE = np.dtype([(f, np.float32) for f in _FIELDS] + [("id", np.int32)])
+assert PARTICLE.itemsize == 52
+
+SPH_FIELD = dict(x=0, y=1, v_x=2, v_y=3, x_prev=4, y_prev=5, density=6, density_near=7,
+                 id=8, key=9)
+
+
+class Tunables(ctypes.Structure):
+    """tunable_parameters (src/fluid.h:78-97)."""
+    _fields_ = [(n, ctypes.c_float) for n in (
+        "rest_density", "smoothing_radius", "g", "k", "k_near", "k_spring", "sigma", "beta",
+        "time_step", "node_start_x", "node_end_x", "mover_center_x", "mover_center_y",
+        "mover_width", "mover_height")] + [
+        ("mover_type", ctypes.c_char), ("kill_sim", ctypes.c_char), ("active", ctypes.c_char)]
+
+
+class Param(ctypes.Structure):
+    """param (src/fluid.h:100-106)."""
+    _fields_ = [("tunable_params", Tunables), ("number_fluid_particles_global", ctypes.c_int),
+                ("number_fluid_particles_local", ctypes.c_int),
+                ("max_fluid_particle_index", ctypes.c_int),
+                ("number_halo_particles", ctypes.c_int)]
+
+
+class AABB(ctypes.Structure):
+    """AABB_t (src/geometry.h:33-40)."""
+    _fields_ = [(n, ctypes.c_float) for n in ("min_x", "max_x", "min_y", "max_y", "min_z", "max_z")]
+
+
+class Counters(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "steps", "hashes", "bucket_overflow", "neighbor_overflow", "key_out_of_range",
+        "nonfinite", "max_cell_count", "max_neighbors")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+assert ctypes.sizeof(Tunables) == 64 and ctypes.sizeof(Param) == 80 and ctypes.sizeof(AABB) == 24
+
+
+class SphError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libsph_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SphError(f"{LIB_PATH} is missing: build it with `python -m sph_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ip, cp = ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p
+        sigs = {
+            "sph_create": (ip, [ctypes.POINTER(vp), ctypes.POINTER(Param), ctypes.POINTER(AABB), ip]),
+            "sph_destroy": (None, [vp]),
+            "sph_upload": (ip, [vp, vp, ip]),
+            "sph_set_tunables": (ip, [vp, ctypes.POINTER(Tunables)]),
+            "sph_step": (ip, [vp, ip]),
+            "sph_step_eager": (ip, [vp, ip]),
+            "sph_sync": (ip, [vp]),
+            "sph_download": (ip, [vp, vp]),
+            "sph_pack_coords_i16": (ip, [vp, vp]),
+            "sph_get_counters": (ip, [vp, ctypes.POINTER(Counters)]),
+            "sph_reset_counters": (ip, [vp]),
+            "sph_num_particles": (ip, [vp]),
+            "sph_last_error": (cp, []),
+            "sph_phase_gravity": (ip, [vp]),
+            "sph_phase_viscosity": (ip, [vp]),
+            "sph_phase_predict": (ip, [vp]),
+            "sph_phase_hash": (ip, [vp, ip]),
+            "sph_phase_relax": (ip, [vp]),
+            "sph_phase_update_velocities": (ip, [vp]),
+            "sph_read_sorted": (ip, [vp, ip, vp]),
+            "sph_read_cells": (ip, [vp, vp, vp]),
+            "sph_read_neighbors": (ip, [vp, ip, vp, vp, vp]),
+            "sph_num_cells": (ip, [vp, ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_uint)]),
+            "sph_time_kernels": (ip, [vp, ip, ip, ctypes.POINTER(cp), ctypes.POINTER(ctypes.c_float),
+                                      ctypes.POINTER(ip)]),
+            "sph_scene_lattice": (ip, [vp, ip, ip, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                       ctypes.c_float, ctypes.c_uint64]),
+        }
+        for name, (res, args) in sigs.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = lib().sph_last_error().decode(errors="replace")
+        raise SphError(f"{what} failed ({rc}): {msg}")
+
+
+def make_tunables(**kw) -> Tunables:
+    t = Tunables()
+    for k, v in kw.items():
+        if k in ("mover_type", "kill_sim", "active"):
+            setattr(t, k, bytes([int(v)]))
+        else:
+            setattr(t, k, float(v))
+    return t
+
+
+def make_box(max_x, max_y) -> AABB:
+    return AABB(0.0, float(max_x), 0.0, float(max_y), 0.0, 0.0)
+
+
+def scene_lattice(nx, ny, spacing, jitter=0.0, seed=0, x0=0.0, y0=0.0) -> np.ndarray:
+    """The reference's lattice (src/geometry.c:29-67) + seeded jitter."""
+    out = np.zeros(nx * ny, PARTICLE)
+    _check(lib().sph_scene_lattice(out.ctypes.data, nx, ny, x0, y0, spacing, jitter, seed),
+           "sph_scene_lattice")
+    return out
+
+
+class Solver:
+    """One device-resident scene (sph_create ... sph_destroy)."""
+
+    def __init__(self, tunables: Tunables, box: AABB, capacity: int):
+        self._h = ctypes.c_void_p()
+        p = Param()
+        p.tunable_params = tunables
+        p.number_fluid_particles_global = capacity
+        p.number_fluid_particles_local = capacity
+        p.max_fluid_particle_index = capacity - 1
+        self.box = box
+        self.tunables = tunables
+        _check(lib().sph_create(ctypes.byref(self._h), ctypes.byref(p), ctypes.byref(box), capacity),
+               "sph_create")
+
+    def close(self):
+        if self._h:
+            lib().sph_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def n(self):
+        return lib().sph_num_particles(self._h)
+
+    def upload(self, particles: np.ndarray):
+        p = np.ascontiguousarray(particles, dtype=PARTICLE)
+        _check(lib().sph_upload(self._h, p.ctypes.data, len(p)), "sph_upload")
+
+    def set_tunables(self, t: Tunables):
+        _check(lib().sph_set_tunables(self._h, ctypes.byref(t)), "sph_set_tunables")
+        self.tunables = t
+
+    def step(self, nsteps=1, graph=True):
+        f = lib().sph_step if graph else lib().sph_step_eager
+        _check(f(self._h, nsteps), "sph_step")
+
+    def sync(self):
+        _check(lib().sph_sync(self._h), "sph_sync")
+
+    def download(self) -> np.ndarray:
+        out = np.zeros(self.n, PARTICLE)
+        _check(lib().sph_download(self._h, out.ctypes.data), "sph_download")
+        return out
+
+    def pack_coords(self) -> np.ndarray:
+        out = np.zeros((self.n, 2), np.int16)
+        _check(lib().sph_pack_coords_i16(self._h, out.ctypes.data), "sph_pack_coords_i16")
+        return out
+
+    def phase(self, name: str, **kw):
+        L = lib()
+        if name == "gravity":
+            rc = L.sph_phase_gravity(self._h)
+        elif name == "viscosity":
+            rc = L.sph_phase_viscosity(self._h)
+        elif name == "predict":
+            rc = L.sph_phase_predict(self._h)
+        elif name == "hash_density":
+            rc = L.sph_phase_hash(self._h, 1)
+        elif name == "hash":
+            rc = L.sph_phase_hash(self._h, 0)
+        elif name == "relax":
+            rc = L.sph_phase_relax(self._h)
+        elif name == "update_velocities":
+            rc = L.sph_phase_update_velocities(self._h)
+        else:
+            raise ValueError(name)
+        _check(rc, "sph_phase_" + name)
+
+    def read_sorted(self, field: str) -> np.ndarray:
+        dt = {"id": np.int32, "key": np.uint32}.get(field, np.float32)
+        out = np.zeros(self.n, dt)
+        _check(lib().sph_read_sorted(self._h, SPH_FIELD[field], out.ctypes.data), "sph_read_sorted")
+        return out
+
+    def cells(self):
+        sx, sy = ctypes.c_uint(), ctypes.c_uint()
+        _check(lib().sph_num_cells(self._h, ctypes.byref(sx), ctypes.byref(sy)), "sph_num_cells")
+        nc = sx.value * sy.value
+        start = np.zeros(nc, np.uint32)
+        count = np.zeros(nc, np.uint32)
+        _check(lib().sph_read_cells(self._h, start.ctypes.data, count.ctypes.data), "sph_read_cells")
+        return start, count, (sx.value, sy.value)
+
+    def neighbors(self, stride=64):
+        """(full count, half-list length, first `stride` neighbour ids), by id."""
+        n = self.n
+        full = np.zeros(n, np.int32)
+        half = np.zeros(n, np.int32)
+        ids = np.full((n, stride), -1, np.int32)
+        _check(lib().sph_read_neighbors(self._h, stride, full.ctypes.data, half.ctypes.data,
+                                        ids.ctypes.data), "sph_read_neighbors")
+        return full, half, ids
+
+    def counters(self) -> dict:
+        c = Counters()
+        _check(lib().sph_get_counters(self._h, ctypes.byref(c)), "sph_get_counters")
+        return c.as_dict()
+
+    def reset_counters(self):
+        _check(lib().sph_reset_counters(self._h), "sph_reset_counters")
+
+    def time_kernels(self, nsteps: int) -> dict:
+        K = 16
+        names = (ctypes.c_char_p * K)()
+        ms = (ctypes.c_float * K)()
+        nk = ctypes.c_int()
+        _check(lib().sph_time_kernels(self._h, nsteps, K, names, ms, ctypes.byref(nk)),
+               "sph_time_kernels")
+        return {names[i].decode(): float(ms[i]) for i in range(nk.value)}

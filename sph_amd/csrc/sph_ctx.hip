@@ -1,0 +1,851 @@
+// sph_ctx.hip - host side of libsph_hip.so: the device-resident context
+// behind the C ABI of include/sph_abi.h.
+//
+// One context = one scene on one GPU: a SoA particle array sorted by
+// (cell, id), the cell table of the last hash, parameter and counter
+// blocks, and a HIP stream.  sph_step replays a captured HIP graph of the
+// fused step; the phase calls run the reference's functions one by one.
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "sph_kernels.h"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int set_err(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(call)                                                                             \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return set_err(SPH_EHIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),      \
+                           __FILE__, __LINE__);                                                  \
+    } while (0)
+
+// Kernel kinds of the fused step, for per-kernel timing.
+enum Kind { KV = 0, KSR, KSA, KSC, KRO, KDE, KRE, NKIND };
+const char *kKindNames[NKIND] = {"visc_predict", "scan_reduce", "scan_apply", "scatter",
+                                 "reorder",      "density",     "relax"};
+
+constexpr int kGraphSteps = 16; // steps per replay of the long graph
+
+} // namespace
+
+struct sph_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    int cap = 0, n = 0;
+    unsigned sx = 0, sy = 0, ncells = 0;
+    tunable_parameters tun{};
+    AABB_t box{};
+    sph_dev_params hp{};
+    sph_dev_params *dP = nullptr;
+    sph_dev_counters *dC = nullptr;
+    // state (canonical: sorted by (cell, id); ID0/K0 give id and key)
+    float *X0 = nullptr, *Y0 = nullptr, *VX = nullptr, *VY = nullptr;
+    float *XP = nullptr, *YP = nullptr, *RHO = nullptr, *RHON = nullptr;
+    // work buffers
+    float *X1 = nullptr, *Y1 = nullptr, *UX = nullptr, *UY = nullptr, *UVX = nullptr,
+          *UVY = nullptr, *T0 = nullptr, *T1 = nullptr;
+    int *ID0 = nullptr, *ID1 = nullptr, *perm_src = nullptr, *perm_id = nullptr,
+        *pos_of_id = nullptr;
+    unsigned *K0 = nullptr, *K1 = nullptr, *ukey = nullptr, *uslot = nullptr, *cnt = nullptr,
+             *start = nullptr, *bsum = nullptr;
+    short *coords = nullptr;
+    std::vector<float> ax_by_id, ay_by_id; // a_x/a_y: carried, never changed (src/fluid.c)
+    bool lists_valid = false; // a hash ran since upload: K0/start describe X0/Y0
+    int aux_order = 0;        // XP,YP,RHO,RHON ordered like ID0 (0) or ID1 (1)
+    hipGraphExec_t g1 = nullptr, gK = nullptr;
+    int graph_n = -1;
+    uint64_t steps = 0, hashes = 0;
+    // per-kernel timing (only while sph_time_kernels runs)
+    std::vector<hipEvent_t> *tev = nullptr;
+    std::vector<int> *tkind = nullptr;
+};
+
+namespace {
+
+void derive_params(sph_ctx *c)
+{
+    const tunable_parameters &t = c->tun;
+    sph_dev_params &p = c->hp;
+    p.h = t.smoothing_radius;
+    p.h_recip = 1.0f / t.smoothing_radius;
+    p.h2 = t.smoothing_radius * t.smoothing_radius;
+    p.dt = t.time_step;
+    p.g_neg = -t.g;
+    p.rest_density = t.rest_density;
+    p.k = t.k;
+    p.k_near = t.k_near;
+    p.k_spring = t.k_spring;
+    p.sigma = t.sigma;
+    p.beta = t.beta;
+    p.min_x = c->box.min_x;
+    p.max_x = c->box.max_x;
+    p.min_y = c->box.min_y;
+    p.max_y = c->box.max_y;
+    p.mover_cx = t.mover_center_x;
+    p.mover_cy = t.mover_center_y;
+    p.mover_w = t.mover_width;
+    p.mover_h = t.mover_height;
+    p.mover_type = t.mover_type;
+    p.sx = c->sx;
+    p.sy = c->sy;
+    p.ncells = c->ncells;
+    p.n = c->n;
+}
+
+int push_params(sph_ctx *c)
+{
+    derive_params(c);
+    HIPCHK(hipMemcpy(c->dP, &c->hp, sizeof c->hp, hipMemcpyHostToDevice));
+    return SPH_OK;
+}
+
+void free_graphs(sph_ctx *c)
+{
+    if (c->g1)
+        (void)hipGraphExecDestroy(c->g1);
+    if (c->gK)
+        (void)hipGraphExecDestroy(c->gK);
+    c->g1 = c->gK = nullptr;
+    c->graph_n = -1;
+}
+
+// Timing hooks around each launch (no-ops unless sph_time_kernels runs).
+struct Tick {
+    sph_ctx *c;
+    Tick(sph_ctx *c_, int kind) : c(c_)
+    {
+        if (c->tev) {
+            hipEvent_t e;
+            (void)hipEventCreate(&e);
+            (void)hipEventRecord(e, c->st);
+            c->tev->push_back(e);
+            c->tkind->push_back(kind);
+        }
+    }
+    ~Tick()
+    {
+        if (c->tev) {
+            hipEvent_t e;
+            (void)hipEventCreate(&e);
+            (void)hipEventRecord(e, c->st);
+            c->tev->push_back(e);
+        }
+    }
+};
+
+void enqueue_sort_tail(sph_ctx *c, const int *id_src)
+{
+    const unsigned m = c->ncells + 1;
+    {
+        Tick t(c, KSR);
+        sph_launch_scan_reduce(c->st, c->cnt, m, c->bsum, c->dC);
+    }
+    {
+        Tick t(c, KSA);
+        sph_launch_scan_apply(c->st, c->cnt, m, c->bsum, c->start);
+    }
+    {
+        Tick t(c, KSC);
+        sph_launch_scatter(c->st, c->n, c->ukey, c->uslot, c->start, id_src, c->perm_src,
+                           c->perm_id);
+    }
+}
+
+// The fused step: 11 launches, no host synchronisation.
+//   visc_predict: gravity + viscosity + predict + boundary + key/slot
+//   scan, scan, scatter, reorder   -> sorted predicted state (X1, Y1, XP, YP)
+//   density                        -> RHO, RHON
+//   relax: relaxation + boundary + velocity update + key/slot
+//   scan, scan, scatter, reorder   -> sorted relaxed state (X0, Y0, VX, VY)
+void enqueue_fused_step(sph_ctx *c, bool visc)
+{
+    const int n = c->n;
+    {
+        Tick t(c, KV);
+        sph_launch_visc_predict(c->st, n, true, visc, true, c->dP, c->X0, c->Y0, c->VX, c->VY,
+                                c->K0, c->start, c->UX, c->UY, c->ukey, c->uslot, c->cnt, c->dC);
+    }
+    enqueue_sort_tail(c, c->ID0);
+    {
+        Tick t(c, KRO);
+        sph_reorder_fields<4> f{{c->UX, c->UY, c->X0, c->Y0}, {c->X1, c->Y1, c->XP, c->YP}};
+        sph_launch_reorder4(c->st, n, c->start, c->perm_src, c->perm_id, c->ukey, f, c->ID1,
+                            c->K1);
+    }
+    {
+        Tick t(c, KDE);
+        sph_launch_density(c->st, n, c->dP, c->X1, c->Y1, c->K1, c->start, c->RHO, c->RHON,
+                           c->dC);
+    }
+    {
+        Tick t(c, KRE);
+        sph_launch_relax(c->st, n, true, c->dP, c->X1, c->Y1, c->XP, c->YP, c->RHO, c->RHON,
+                         c->K1, c->start, c->UX, c->UY, c->UVX, c->UVY, c->ukey, c->uslot,
+                         c->cnt, c->dC);
+    }
+    enqueue_sort_tail(c, c->ID1);
+    {
+        Tick t(c, KRO);
+        sph_reorder_fields<4> f{{c->UX, c->UY, c->UVX, c->UVY}, {c->X0, c->Y0, c->VX, c->VY}};
+        sph_launch_reorder4(c->st, n, c->start, c->perm_src, c->perm_id, c->ukey, f, c->ID0,
+                            c->K0);
+    }
+}
+
+int build_graph(sph_ctx *c, int steps, hipGraphExec_t *out)
+{
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < steps; i++)
+        enqueue_fused_step(c, true);
+    HIPCHK(hipStreamEndCapture(c->st, &g));
+    hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess)
+        return set_err(SPH_EHIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    return SPH_OK;
+}
+
+int ensure_graphs(sph_ctx *c)
+{
+    if (c->graph_n == c->n && c->g1 && c->gK)
+        return SPH_OK;
+    free_graphs(c);
+    int rc = build_graph(c, 1, &c->g1);
+    if (rc)
+        return rc;
+    rc = build_graph(c, kGraphSteps, &c->gK);
+    if (rc)
+        return rc;
+    c->graph_n = c->n;
+    return SPH_OK;
+}
+
+// Bring XP/YP/RHO/RHON into ID0 order (after fused steps they follow ID1).
+int canonicalize(sph_ctx *c)
+{
+    if (c->aux_order == 0 || c->n == 0) {
+        c->aux_order = 0;
+        return SPH_OK;
+    }
+    const int n = c->n;
+    sph_launch_pos_of_id(c->st, n, c->ID0, c->pos_of_id);
+    sph_reorder_fields<4> f{{c->XP, c->YP, c->RHO, c->RHON}, {c->T0, c->T1, c->UVX, c->UVY}};
+    sph_launch_permute_by_id(c->st, n, c->ID1, c->pos_of_id, f);
+    size_t b = sizeof(float) * (size_t)n;
+    HIPCHK(hipMemcpyAsync(c->XP, c->T0, b, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->YP, c->T1, b, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->RHO, c->UVX, b, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->RHON, c->UVY, b, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipGetLastError());
+    c->aux_order = 0;
+    return SPH_OK;
+}
+
+int check_ctx(const sph_ctx *c)
+{
+    if (!c)
+        return set_err(SPH_EINVAL, "null context");
+    return SPH_OK;
+}
+
+template <class T>
+int dalloc(T **p, size_t count)
+{
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * (count ? count : 1));
+    if (e != hipSuccess)
+        return set_err(SPH_ENOMEM, "hipMalloc(%zu x %zu B) failed: %s", count, sizeof(T),
+                       hipGetErrorString(e));
+    return SPH_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *sph_last_error(void) { return g_err; }
+
+int sph_create(sph_ctx **out, const param *params, const AABB_t *boundary, int capacity)
+{
+    if (!out || !params || !boundary || capacity < 0)
+        return set_err(SPH_EINVAL, "sph_create: null argument or negative capacity");
+    *out = nullptr;
+    const float h = params->tunable_params.smoothing_radius;
+    if (!(h > 0.0f))
+        return set_err(SPH_EINVAL, "sph_create: smoothing_radius must be > 0");
+    if (boundary->min_x != 0.0f || boundary->min_y != 0.0f)
+        return set_err(SPH_EINVAL, "sph_create: the box must start at the origin");
+    if (!(boundary->max_x > 0.0f && boundary->max_y > 0.0f))
+        return set_err(SPH_EINVAL, "sph_create: empty box");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return set_err(SPH_ENODEV, "no HIP device available");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return set_err(SPH_ENODEV, "device %d is %s; this library is built for gfx950", dev,
+                       prop.gcnArchName);
+
+    // grid extent, src/fluid.c:214-215
+    const double gx = ceil((boundary->max_x - boundary->min_x) / h);
+    const double gy = ceil((boundary->max_y - boundary->min_y) / h);
+    if (gx * gy >= 4294967295.0)
+        return set_err(SPH_EINVAL, "sph_create: %g x %g cells exceed 32-bit keys", gx, gy);
+
+    sph_ctx *c = new (std::nothrow) sph_ctx();
+    if (!c)
+        return set_err(SPH_ENOMEM, "host allocation failed");
+    c->device = dev;
+    c->cap = capacity;
+    c->tun = params->tunable_params;
+    c->box = *boundary;
+    c->sx = (unsigned)gx;
+    c->sy = (unsigned)gy;
+    c->ncells = c->sx * c->sy;
+    int rc = SPH_OK;
+    const size_t N = (size_t)capacity, M = (size_t)c->ncells + 1;
+#define A(p, cnt)                                                                                \
+    if (!rc)                                                                                     \
+    rc = dalloc(&c->p, cnt)
+    A(X0, N); A(Y0, N); A(VX, N); A(VY, N); A(XP, N); A(YP, N); A(RHO, N); A(RHON, N);
+    A(X1, N); A(Y1, N); A(UX, N); A(UY, N); A(UVX, N); A(UVY, N); A(T0, N); A(T1, N);
+    A(ID0, N); A(ID1, N); A(perm_src, N); A(perm_id, N); A(pos_of_id, N);
+    A(K0, N); A(K1, N); A(ukey, N); A(uslot, N);
+    A(cnt, M); A(start, M); A(bsum, sph_scan_blocks((unsigned)M));
+    A(coords, 2 * N); A(dP, 1); A(dC, 1);
+#undef A
+    if (!rc) {
+        hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+        if (e != hipSuccess)
+            rc = set_err(SPH_EHIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+    }
+    if (!rc) {
+        hipError_t e = hipMemset(c->cnt, 0, sizeof(unsigned) * M);
+        if (e == hipSuccess)
+            e = hipMemset(c->start, 0, sizeof(unsigned) * M);
+        if (e == hipSuccess)
+            e = hipMemset(c->dC, 0, sizeof(sph_dev_counters));
+        if (e != hipSuccess)
+            rc = set_err(SPH_EHIP, "hipMemset failed: %s", hipGetErrorString(e));
+    }
+    if (!rc)
+        rc = push_params(c);
+    if (rc) {
+        sph_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return SPH_OK;
+}
+
+void sph_destroy(sph_ctx *c)
+{
+    if (!c)
+        return;
+    if (c->st)
+        (void)hipStreamSynchronize(c->st);
+    free_graphs(c);
+    void *ptrs[] = {c->X0,  c->Y0,   c->VX,   c->VY,       c->XP,      c->YP,    c->RHO,
+                    c->RHON, c->X1,  c->Y1,   c->UX,       c->UY,      c->UVX,   c->UVY,
+                    c->T0,  c->T1,   c->ID0,  c->ID1,      c->perm_src, c->perm_id,
+                    c->pos_of_id, c->K0, c->K1, c->ukey,   c->uslot,   c->cnt,   c->start,
+                    c->bsum, c->coords, c->dP, c->dC};
+    for (void *p : ptrs)
+        if (p)
+            (void)hipFree(p);
+    if (c->st)
+        (void)hipStreamDestroy(c->st);
+    delete c;
+}
+
+int sph_num_particles(const sph_ctx *c) { return c ? c->n : 0; }
+
+int sph_num_cells(const sph_ctx *c, unsigned *size_x, unsigned *size_y)
+{
+    if (!c)
+        return set_err(SPH_EINVAL, "null context");
+    if (size_x)
+        *size_x = c->sx;
+    if (size_y)
+        *size_y = c->sy;
+    return SPH_OK;
+}
+
+static int upload_impl(sph_ctx *c, const fluid_particle *const *pp, const fluid_particle *arr,
+                       int n)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (n < 0 || n > c->cap)
+        return set_err(SPH_EINVAL, "sph_upload: n=%d outside [0, capacity=%d]", n, c->cap);
+    if (n > 0 && !pp && !arr)
+        return set_err(SPH_EINVAL, "sph_upload: null particles");
+    std::vector<float> f[8];
+    for (auto &v : f)
+        v.resize((size_t)n);
+    std::vector<int> ids((size_t)n);
+    std::vector<char> seen((size_t)n, 0);
+    c->ax_by_id.assign((size_t)n, 0.0f);
+    c->ay_by_id.assign((size_t)n, 0.0f);
+    for (int i = 0; i < n; i++) {
+        const fluid_particle *p = pp ? pp[i] : &arr[i];
+        if (!p)
+            return set_err(SPH_EINVAL, "sph_upload: null particle pointer at %d", i);
+        if (p->id < 0 || p->id >= n || seen[p->id])
+            return set_err(SPH_EINVAL, "sph_upload: ids must be a permutation of 0..%d (bad id %d at %d)",
+                           n - 1, p->id, i);
+        seen[p->id] = 1;
+        ids[i] = p->id;
+        f[0][i] = p->x;
+        f[1][i] = p->y;
+        f[2][i] = p->v_x;
+        f[3][i] = p->v_y;
+        f[4][i] = p->x_prev;
+        f[5][i] = p->y_prev;
+        f[6][i] = p->density;
+        f[7][i] = p->density_near;
+        c->ax_by_id[p->id] = p->a_x;
+        c->ay_by_id[p->id] = p->a_y;
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    float *dst[8] = {c->X0, c->Y0, c->VX, c->VY, c->XP, c->YP, c->RHO, c->RHON};
+    const size_t b = sizeof(float) * (size_t)n;
+    if (n > 0) {
+        for (int k = 0; k < 8; k++)
+            HIPCHK(hipMemcpy(dst[k], f[k].data(), b, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->ID0, ids.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice));
+    }
+    c->n = n;
+    c->lists_valid = false;
+    c->aux_order = 0;
+    return push_params(c);
+}
+
+int sph_upload(sph_ctx *c, const fluid_particle *particles, int n)
+{
+    return upload_impl(c, nullptr, particles, n);
+}
+
+int sph_upload_pointers(sph_ctx *c, fluid_particle *const *fluid_particle_pointers, int n)
+{
+    return upload_impl(c, (const fluid_particle *const *)fluid_particle_pointers, nullptr, n);
+}
+
+int sph_set_tunables(sph_ctx *c, const tunable_parameters *t)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (!t)
+        return set_err(SPH_EINVAL, "sph_set_tunables: null");
+    if (t->smoothing_radius != c->tun.smoothing_radius)
+        return set_err(SPH_EINVAL, "sph_set_tunables: smoothing_radius fixes the grid and cannot change");
+    if (memcmp(t, &c->tun, sizeof *t) == 0)
+        return SPH_OK;
+    c->tun = *t;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return push_params(c);
+}
+
+int sph_step_eager(sph_ctx *c, int nsteps)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (nsteps < 0)
+        return set_err(SPH_EINVAL, "nsteps < 0");
+    if (c->n == 0)
+        return SPH_OK;
+    for (int i = 0; i < nsteps; i++) {
+        enqueue_fused_step(c, c->lists_valid);
+        c->lists_valid = true;
+        c->aux_order = 1;
+        c->steps++;
+        c->hashes += 2;
+    }
+    HIPCHK(hipGetLastError());
+    return SPH_OK;
+}
+
+int sph_step(sph_ctx *c, int nsteps)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (nsteps < 0)
+        return set_err(SPH_EINVAL, "nsteps < 0");
+    if (c->n == 0 || nsteps == 0)
+        return SPH_OK;
+    if (!c->lists_valid) { // first step after upload: empty lists, no viscosity
+        int rc = sph_step_eager(c, 1);
+        if (rc)
+            return rc;
+        nsteps--;
+    }
+    if (nsteps == 0)
+        return SPH_OK;
+    if (int rc = ensure_graphs(c))
+        return rc;
+    while (nsteps >= kGraphSteps) {
+        HIPCHK(hipGraphLaunch(c->gK, c->st));
+        nsteps -= kGraphSteps;
+        c->steps += kGraphSteps;
+        c->hashes += 2 * kGraphSteps;
+    }
+    while (nsteps-- > 0) {
+        HIPCHK(hipGraphLaunch(c->g1, c->st));
+        c->steps++;
+        c->hashes += 2;
+    }
+    c->aux_order = 1;
+    return SPH_OK;
+}
+
+int sph_sync(sph_ctx *c)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return SPH_OK;
+}
+
+// ---- phase-level calls (reference functions one by one) -------------------
+
+int sph_phase_gravity(sph_ctx *c)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (int rc = canonicalize(c))
+        return rc;
+    if (c->n)
+        sph_launch_gravity(c->st, c->n, c->dP, c->VY, c->RHO, c->RHON);
+    HIPCHK(hipGetLastError());
+    return SPH_OK;
+}
+
+int sph_phase_viscosity(sph_ctx *c)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (int rc = canonicalize(c))
+        return rc;
+    if (!c->lists_valid || c->n == 0)
+        return SPH_OK; // empty lists (src/fluid.c:202)
+    sph_launch_visc_predict(c->st, c->n, false, true, false, c->dP, c->X0, c->Y0, c->VX, c->VY,
+                            c->K0, c->start, c->UVX, c->UVY, nullptr, nullptr, c->cnt, c->dC);
+    const size_t b = sizeof(float) * (size_t)c->n;
+    HIPCHK(hipMemcpyAsync(c->VX, c->UVX, b, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->VY, c->UVY, b, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipGetLastError());
+    return SPH_OK;
+}
+
+int sph_phase_predict(sph_ctx *c)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (int rc = canonicalize(c))
+        return rc;
+    if (c->n)
+        sph_launch_predict(c->st, c->n, c->dP, c->X0, c->Y0, c->VX, c->VY, c->XP, c->YP);
+    HIPCHK(hipGetLastError());
+    return SPH_OK;
+}
+
+int sph_phase_hash(sph_ctx *c, int compute_density)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (int rc = canonicalize(c))
+        return rc;
+    const int n = c->n;
+    if (n == 0)
+        return SPH_OK;
+    sph_launch_key_hist(c->st, n, c->dP, c->X0, c->Y0, c->ukey, c->uslot, c->cnt, c->dC);
+    enqueue_sort_tail(c, c->ID0);
+    sph_reorder_fields<8> f{{c->X0, c->Y0, c->VX, c->VY, c->XP, c->YP, c->RHO, c->RHON},
+                            {c->X1, c->Y1, c->UX, c->UY, c->UVX, c->UVY, c->T0, c->T1}};
+    sph_launch_reorder8(c->st, n, c->start, c->perm_src, c->perm_id, c->ukey, f, c->ID1, c->K1);
+    const size_t b = sizeof(float) * (size_t)n;
+    for (int k = 0; k < 8; k++)
+        HIPCHK(hipMemcpyAsync((void *)f.src[k], f.dst[k], b, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->ID0, c->ID1, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->K0, c->K1, sizeof(unsigned) * (size_t)n, hipMemcpyDeviceToDevice,
+                          c->st));
+    if (compute_density)
+        sph_launch_density(c->st, n, c->dP, c->X0, c->Y0, c->K0, c->start, c->RHO, c->RHON, c->dC);
+    HIPCHK(hipGetLastError());
+    c->lists_valid = true;
+    c->hashes++;
+    return SPH_OK;
+}
+
+int sph_phase_relax(sph_ctx *c)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (int rc = canonicalize(c))
+        return rc;
+    if (c->n == 0)
+        return SPH_OK;
+    if (!c->lists_valid)
+        return set_err(SPH_ESTATE, "sph_phase_relax: no hash since upload");
+    sph_launch_relax(c->st, c->n, false, c->dP, c->X0, c->Y0, c->XP, c->YP, c->RHO, c->RHON,
+                     c->K0, c->start, c->UX, c->UY, nullptr, nullptr, nullptr, nullptr, nullptr,
+                     c->dC);
+    const size_t b = sizeof(float) * (size_t)c->n;
+    HIPCHK(hipMemcpyAsync(c->X0, c->UX, b, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->Y0, c->UY, b, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipGetLastError());
+    return SPH_OK;
+}
+
+int sph_phase_update_velocities(sph_ctx *c)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (int rc = canonicalize(c))
+        return rc;
+    if (c->n)
+        sph_launch_update_velocities(c->st, c->n, c->dP, c->X0, c->Y0, c->VX, c->VY, c->XP, c->YP);
+    HIPCHK(hipGetLastError());
+    return SPH_OK;
+}
+
+// ---- read-back -------------------------------------------------------------
+
+int sph_download(sph_ctx *c, fluid_particle *out)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    const int n = c->n;
+    if (n > 0 && !out)
+        return set_err(SPH_EINVAL, "sph_download: null output");
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (n == 0)
+        return SPH_OK;
+    std::vector<float> f[8];
+    const float *src[8] = {c->X0, c->Y0, c->VX, c->VY, c->XP, c->YP, c->RHO, c->RHON};
+    const size_t b = sizeof(float) * (size_t)n;
+    for (int k = 0; k < 8; k++) {
+        f[k].resize((size_t)n);
+        HIPCHK(hipMemcpy(f[k].data(), src[k], b, hipMemcpyDeviceToHost));
+    }
+    std::vector<int> id0((size_t)n), idaux((size_t)n);
+    HIPCHK(hipMemcpy(id0.data(), c->ID0, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(idaux.data(), c->aux_order ? c->ID1 : c->ID0, sizeof(int) * (size_t)n,
+                     hipMemcpyDeviceToHost));
+    const tunable_parameters &t = c->tun;
+    for (int s = 0; s < n; s++) {
+        fluid_particle *p = &out[id0[s]];
+        p->x = f[0][s];
+        p->y = f[1][s];
+        p->v_x = f[2][s];
+        p->v_y = f[3][s];
+        p->id = id0[s];
+        p->a_x = c->ax_by_id[id0[s]];
+        p->a_y = c->ay_by_id[id0[s]];
+    }
+    for (int s = 0; s < n; s++) {
+        fluid_particle *p = &out[idaux[s]];
+        p->x_prev = f[4][s];
+        p->y_prev = f[5][s];
+        p->density = f[6][s];
+        p->density_near = f[7][s];
+        // src/fluid.c:563-564
+        p->pressure = t.k * (f[6][s] - t.rest_density);
+        p->pressure_near = t.k_near * f[7][s];
+    }
+    return SPH_OK;
+}
+
+int sph_pack_coords_i16(sph_ctx *c, short *coords)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (c->n > 0 && !coords)
+        return set_err(SPH_EINVAL, "sph_pack_coords_i16: null output");
+    if (c->n == 0)
+        return SPH_OK;
+    sph_launch_pack_coords(c->st, c->n, c->X0, c->Y0, c->ID0, c->box.max_x, c->box.max_y,
+                           c->coords);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(coords, c->coords, sizeof(short) * 2 * (size_t)c->n,
+                          hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return SPH_OK;
+}
+
+int sph_read_sorted(sph_ctx *c, int field, void *out)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (!out && c->n)
+        return set_err(SPH_EINVAL, "sph_read_sorted: null output");
+    if (int rc = canonicalize(c))
+        return rc;
+    const void *src = nullptr;
+    switch (field) {
+    case SPH_FIELD_X: src = c->X0; break;
+    case SPH_FIELD_Y: src = c->Y0; break;
+    case SPH_FIELD_VX: src = c->VX; break;
+    case SPH_FIELD_VY: src = c->VY; break;
+    case SPH_FIELD_XPREV: src = c->XP; break;
+    case SPH_FIELD_YPREV: src = c->YP; break;
+    case SPH_FIELD_DENSITY: src = c->RHO; break;
+    case SPH_FIELD_DENSITY_NEAR: src = c->RHON; break;
+    case SPH_FIELD_ID: src = c->ID0; break;
+    case SPH_FIELD_KEY: src = c->K0; break;
+    default: return set_err(SPH_EINVAL, "sph_read_sorted: unknown field %d", field);
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (c->n)
+        HIPCHK(hipMemcpy(out, src, 4 * (size_t)c->n, hipMemcpyDeviceToHost));
+    return SPH_OK;
+}
+
+int sph_read_cells(sph_ctx *c, unsigned *start, unsigned *count)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    std::vector<unsigned> s((size_t)c->ncells + 1);
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipMemcpy(s.data(), c->start, sizeof(unsigned) * s.size(), hipMemcpyDeviceToHost));
+    for (unsigned i = 0; i < c->ncells; i++) {
+        if (start)
+            start[i] = s[i];
+        if (count)
+            count[i] = s[i + 1] - s[i];
+    }
+    return SPH_OK;
+}
+
+int sph_read_neighbors(sph_ctx *c, int stride, int *full_by_id, int *half_by_id, int *ids_by_id)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (stride < 0 || !full_by_id || !half_by_id || (stride > 0 && !ids_by_id))
+        return set_err(SPH_EINVAL, "sph_read_neighbors: bad arguments");
+    if (!c->lists_valid)
+        return set_err(SPH_ESTATE, "sph_read_neighbors: no hash since upload");
+    const int n = c->n;
+    if (n == 0)
+        return SPH_OK;
+    int *dfull = nullptr, *dhalf = nullptr, *dids = nullptr;
+    int rc = dalloc(&dfull, (size_t)n);
+    if (!rc)
+        rc = dalloc(&dhalf, (size_t)n);
+    if (!rc)
+        rc = dalloc(&dids, (size_t)n * (size_t)(stride ? stride : 1));
+    if (!rc) {
+        sph_launch_neighbor_lists(c->st, n, c->dP, c->X0, c->Y0, c->K0, c->ID0, c->start, stride,
+                                  dfull, dhalf, dids);
+        hipError_t e = hipStreamSynchronize(c->st);
+        if (e == hipSuccess)
+            e = hipMemcpy(full_by_id, dfull, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost);
+        if (e == hipSuccess)
+            e = hipMemcpy(half_by_id, dhalf, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && stride > 0)
+            e = hipMemcpy(ids_by_id, dids, sizeof(int) * (size_t)n * stride, hipMemcpyDeviceToHost);
+        if (e != hipSuccess)
+            rc = set_err(SPH_EHIP, "sph_read_neighbors: %s", hipGetErrorString(e));
+    }
+    if (dfull)
+        (void)hipFree(dfull);
+    if (dhalf)
+        (void)hipFree(dhalf);
+    if (dids)
+        (void)hipFree(dids);
+    return rc;
+}
+
+int sph_get_counters(sph_ctx *c, sph_counters_t *out)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (!out)
+        return set_err(SPH_EINVAL, "sph_get_counters: null output");
+    sph_dev_counters d;
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipMemcpy(&d, c->dC, sizeof d, hipMemcpyDeviceToHost));
+    out->steps = c->steps;
+    out->hashes = c->hashes;
+    out->bucket_overflow = d.bucket_overflow;
+    out->neighbor_overflow = d.neighbor_overflow;
+    out->key_out_of_range = d.key_out_of_range;
+    out->nonfinite = d.nonfinite;
+    out->max_cell_count = d.max_cell_count;
+    out->max_neighbors = d.max_neighbors;
+    return SPH_OK;
+}
+
+int sph_reset_counters(sph_ctx *c)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    HIPCHK(hipMemsetAsync(c->dC, 0, sizeof(sph_dev_counters), c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->steps = c->hashes = 0;
+    return SPH_OK;
+}
+
+int sph_time_kernels(sph_ctx *c, int nsteps, int max_kinds, const char **names, float *ms,
+                     int *nkinds)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (nsteps <= 0 || max_kinds < NKIND || !names || !ms || !nkinds)
+        return set_err(SPH_EINVAL, "sph_time_kernels: need nsteps > 0 and room for %d kinds",
+                       (int)NKIND);
+    if (c->n == 0)
+        return set_err(SPH_ESTATE, "sph_time_kernels: no particles");
+    std::vector<hipEvent_t> ev;
+    std::vector<int> kind;
+    c->tev = &ev;
+    c->tkind = &kind;
+    int rc = sph_step_eager(c, nsteps);
+    c->tev = nullptr;
+    c->tkind = nullptr;
+    hipError_t se = hipStreamSynchronize(c->st);
+    double tot[NKIND] = {0};
+    int cntk[NKIND] = {0};
+    for (size_t i = 0; i < kind.size(); i++) {
+        float t = 0.0f;
+        if (hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]) == hipSuccess) {
+            tot[kind[i]] += t;
+            cntk[kind[i]]++;
+        }
+    }
+    for (hipEvent_t e : ev)
+        (void)hipEventDestroy(e);
+    if (rc)
+        return rc;
+    if (se != hipSuccess)
+        return set_err(SPH_EHIP, "hipStreamSynchronize failed: %s", hipGetErrorString(se));
+    for (int k = 0; k < NKIND; k++) {
+        names[k] = kKindNames[k];
+        ms[k] = cntk[k] ? (float)(tot[k] / cntk[k]) : 0.0f;
+    }
+    *nkinds = NKIND;
+    return SPH_OK;
+}
+
+} // extern "C"

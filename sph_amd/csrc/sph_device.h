@@ -1,0 +1,197 @@
+// sph_device.h - device-side math of the TinySPH step, shared by every
+// kernel of libsph_hip.so.  Each helper restates one reference expression
+// with the same operation order, so that with -ffp-contract=off and
+// IEEE-correct division / sqrt (HIP's defaults) the device reproduces the
+// reference's float arithmetic bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/sph_abi.h"
+
+// Step parameters as the kernels read them, derived on the host from
+// tunable_parameters exactly as the reference derives them
+// (h_recip = 1.0f/h: src/fluid.c:427, 555; g = -g: src/fluid.c:403;
+// h2 = h*h: src/hash.c:133).
+struct sph_dev_params {
+    float h, h_recip, h2, dt, g_neg;
+    float rest_density, k, k_near, k_spring, sigma, beta;
+    float min_x, max_x, min_y, max_y;
+    float mover_cx, mover_cy, mover_w, mover_h;
+    int mover_type;
+    unsigned sx, sy, ncells;
+    int n;
+};
+
+// Device health counters (mirrors sph_counters_t from the key counters on).
+struct sph_dev_counters {
+    unsigned long long bucket_overflow;
+    unsigned long long neighbor_overflow;
+    unsigned long long key_out_of_range;
+    unsigned long long nonfinite;
+    unsigned long long max_cell_count;
+    unsigned long long max_neighbors;
+};
+
+// src/hash.c:35-47.  key = floor(y/h)*size_x + floor(x/h); keys the
+// reference could not index (>= cells, negative, NaN: undefined behaviour
+// there) are counted and clamped to the last cell.
+__device__ __forceinline__ unsigned sph_cell_key(float x, float y, const sph_dev_params &P,
+                                                 unsigned long long *oor)
+{
+    float qx = x / P.h, qy = y / P.h;
+    if (!(qx >= 0.0f && qy >= 0.0f && qx < 4294967296.0f && qy < 4294967296.0f)) {
+        atomicAdd(oor, 1ull);
+        return P.ncells - 1;
+    }
+    unsigned long long k = (unsigned long long)(unsigned)floorf(qy) * P.sx +
+                           (unsigned long long)(unsigned)floorf(qx);
+    if (k >= P.ncells) {
+        atomicAdd(oor, 1ull);
+        return P.ncells - 1;
+    }
+    return (unsigned)k;
+}
+
+// src/fluid.c:613-625
+__device__ __forceinline__ void sph_check_velocity(float &vx, float &vy)
+{
+    const float v_max = 5.0f;
+    if (vx > v_max)
+        vx = v_max;
+    else if (vx < -v_max)
+        vx = -v_max;
+    if (vy > v_max)
+        vy = v_max;
+    else if (vy < -v_max)
+        vy = -v_max;
+}
+
+// src/fluid.c:656-744: mover push-out then clamp into the box.
+__device__ __forceinline__ void sph_boundary(float &px, float &py, const sph_dev_params &P)
+{
+    const float cx = P.mover_cx, cy = P.mover_cy;
+    if (P.mover_type == SPHERE_MOVER) {
+        float radius = P.mover_w * 0.5f;
+        float d2 = (px - cx) * (px - cx) + (py - cy) * (py - cy);
+        if (d2 <= radius * radius && d2 > 0.0f) {
+            float d = sqrtf(d2);
+            float nx = (cx - px) / d;
+            float ny = (cy - py) / d;
+            float pen = radius - d;
+            px -= pen * nx;
+            py -= pen * ny;
+        }
+    } else if (P.mover_type == RECTANGLE_MOVER) {
+        float hw = (float)((double)P.mover_w * 0.5);
+        float hh = (float)((double)P.mover_h * 0.5);
+        float rx = px - cx, ry = py - cy;
+        float ax = fabsf(rx), ay = fabsf(ry);
+        if (ax < hw && ay < hh) {
+            float penx = hw - ax, peny = hh - ay;
+            if (penx < peny) {
+                if (rx < 0.0f)
+                    px -= penx;
+                else
+                    px += penx;
+            } else {
+                if (ry < 0.0f)
+                    py -= peny;
+                else
+                    py += peny;
+            }
+        }
+    }
+    if (px < P.min_x)
+        px = P.min_x;
+    else if (px > P.max_x)
+        px = P.max_x - 0.001f;
+    if (py < P.min_y)
+        py = P.min_y;
+    else if (py > P.max_y)
+        py = P.max_y - 0.001f;
+}
+
+// src/fluid.c:591.  The literal 0.5 is a double, so the spring term, its
+// sum with the pressure terms and the product with dt*dt are double
+// operations, rounded to float once.
+__device__ __forceinline__ float sph_relax_D(float dt, float Psum, float Pnsum, float OmR,
+                                             float k_spring, float h, float r)
+{
+    float a = Psum * OmR + Pnsum * OmR * OmR;
+    float s = k_spring * (h - r);
+    float dt2 = dt * dt;
+    return (float)((double)dt2 * ((double)a + (double)s * 0.5));
+}
+
+// src/fluid.c:442-459: impulse of pair (p, q) with p in the owner role.
+// Exactly antisymmetric in (p, q), so a particle in either role subtracts
+// half of the impulse computed with itself as p.
+__device__ __forceinline__ bool sph_visc_impulse(float px, float py, float pvx, float pvy, float qx,
+                                                 float qy, float qvx, float qvy,
+                                                 const sph_dev_params &P, float &ix, float &iy)
+{
+    float QmP_x = qx - px;
+    float QmP_y = qy - py;
+    float r = sqrtf(QmP_x * QmP_x + QmP_y * QmP_y);
+    float r_recip = 1.0f / r;
+    float ratio = r * P.h_recip;
+    float u = ((pvx - qvx) * QmP_x + (pvy - qvy) * QmP_y) * r_recip;
+    if (u > 0.0f) {
+        float imp = P.dt * (1.0f - ratio) * (P.sigma * u + P.beta * u * u);
+        ix = imp * QmP_x * r_recip;
+        iy = imp * QmP_y * r_recip;
+        sph_check_velocity(ix, iy);
+        return true;
+    }
+    return false;
+}
+
+// The 3x3 neighbourhood of cell `key` as three sorted-order row ranges.
+// For row r (dy = r - j) the candidates in [b0, b3) lie in cell i-1 for
+// k < b1, cell i for b1 <= k < b2, and cell i+1 for k >= b2.
+struct sph_rows {
+    int nr;
+    int dy[3];
+    unsigned b0[3], b1[3], b2[3], b3[3];
+};
+
+__device__ __forceinline__ void sph_row_ranges(unsigned key, const sph_dev_params &P,
+                                               const unsigned *__restrict__ start, sph_rows &R)
+{
+    const unsigned sx = P.sx, sy = P.sy;
+    const unsigned i = key % sx, j = key / sx;
+    const unsigned i0 = i > 0 ? i - 1 : 0;
+    const unsigned i1 = i + 1 < sx ? i + 1 : sx - 1;
+    R.nr = 0;
+#pragma unroll
+    for (int d = -1; d <= 1; d++) {
+        long r = (long)j + d;
+        if (r < 0 || r >= (long)sy)
+            continue;
+        unsigned base = (unsigned)r * sx;
+        R.dy[R.nr] = d;
+        R.b0[R.nr] = start[base + i0];
+        R.b1[R.nr] = start[base + i];
+        R.b2[R.nr] = start[base + i + 1];
+        R.b3[R.nr] = start[base + i1 + 1];
+        R.nr++;
+    }
+}
+
+// Half-list owner rule of src/hash.c:176-237 for the pair (self at sorted
+// position s, candidate at sorted position k, row offset dy): the pair is in
+// self's half list when the candidate's cell is a forward cell
+// (0,+1), (+1,-1), (+1,0), (+1,+1), or shares self's cell with a higher id
+// (cells are sorted by id, so: a later sorted position).
+__device__ __forceinline__ bool sph_owns(unsigned k, unsigned s, int dy, unsigned b1, unsigned b2)
+{
+    if (k >= b2)
+        return true; // dx = +1
+    if (k < b1)
+        return false; // dx = -1
+    if (dy == 1)
+        return true; // (0, +1)
+    return dy == 0 && k > s;
+}

@@ -1,0 +1,337 @@
+/*
+ * sph_shim.c - the reference's step-function API (src/fluid.h:112-126,
+ * src/hash.h:50-52) implemented on the device context, so the reference's
+ * main loop (src/fluid.c:270-372) links against libsph_hip.so in place of
+ * src/hash.c and the step functions of src/fluid.c.
+ *
+ * One process-global context serves the calls, as the reference keeps one
+ * simulation per process.  It is created by the first call that carries the
+ * box (predict_positions), uploading the caller's particles through the
+ * pointer array; calls before that (the first apply_gravity of a run) run
+ * on the host with the reference's arithmetic, and the first viscosity pass
+ * is a no-op exactly as in the reference, whose lists start empty.
+ * hash_fluid(..., false) ends a step (src/fluid.c:341); every
+ * sync_interval steps it copies the device state back into the caller's
+ * particles, so code that reads p->x afterwards (src/fluid.c:356-363) sees
+ * the current state.  neighbor / bucket storage is left untouched: the
+ * neighbour sets live on the device.
+ *
+ * Errors keep the reference's void returns; the message is available from
+ * sph_last_error() and is also printed once to stderr.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/sph_abi.h"
+
+static sph_ctx *g_ctx;
+static fluid_particle **g_ptrs;
+static int g_n;
+static int g_dirty = 1;
+static int g_sync_interval = 1;
+static int g_steps_since_sync;
+static int g_reported;
+
+static void shim_fail(const char *where, int rc)
+{
+    if (rc && !g_reported) {
+        fprintf(stderr, "libsph_hip: %s failed (%d): %s\n", where, rc, sph_last_error());
+        g_reported = 1;
+    }
+}
+
+/* Make the context match the caller's particles and tunables. */
+static int shim_ready(fluid_particle **ptrs, param *params, AABB_t *box)
+{
+    int n = params->number_fluid_particles_local;
+    int rc = SPH_OK;
+    if (!g_ctx) {
+        if (!box)
+            return SPH_ESTATE;
+        rc = sph_create(&g_ctx, params, box, n);
+        if (rc)
+            return rc;
+        g_dirty = 1;
+    }
+    if (n != g_n || ptrs != g_ptrs)
+        g_dirty = 1;
+    if (g_dirty) {
+        rc = sph_upload_pointers(g_ctx, ptrs, n);
+        if (rc)
+            return rc;
+        g_ptrs = ptrs;
+        g_n = n;
+        g_dirty = 0;
+        g_steps_since_sync = 0;
+    }
+    return sph_set_tunables(g_ctx, &params->tunable_params);
+}
+
+int sph_shim_sync(void)
+{
+    if (!g_ctx || !g_ptrs)
+        return SPH_OK;
+    fluid_particle *tmp = (fluid_particle *)malloc(sizeof(fluid_particle) * (size_t)(g_n ? g_n : 1));
+    if (!tmp)
+        return SPH_ENOMEM;
+    int rc = sph_download(g_ctx, tmp);
+    if (!rc)
+        for (int i = 0; i < g_n; i++)
+            *g_ptrs[i] = tmp[i]; /* pointer index == id on one domain */
+    free(tmp);
+    g_steps_since_sync = 0;
+    return rc;
+}
+
+int sph_shim_set_sync_interval(int steps)
+{
+    if (steps < 0)
+        return SPH_EINVAL;
+    g_sync_interval = steps;
+    return SPH_OK;
+}
+
+int sph_shim_mark_host_dirty(void)
+{
+    g_dirty = 1;
+    return SPH_OK;
+}
+
+void sph_shim_release(void)
+{
+    sph_destroy(g_ctx);
+    g_ctx = NULL;
+    g_ptrs = NULL;
+    g_n = 0;
+    g_dirty = 1;
+    g_reported = 0;
+}
+
+sph_ctx *sph_shim_context(void) { return g_ctx; }
+
+/* ---- host helpers with the reference's arithmetic ---------------------- */
+
+/* src/fluid.c:613-625 */
+void checkVelocity(float *v_x, float *v_y)
+{
+    const float v_max = 5.0f;
+    if (*v_x > v_max)
+        *v_x = v_max;
+    else if (*v_x < -v_max)
+        *v_x = -v_max;
+    if (*v_y > v_max)
+        *v_y = v_max;
+    else if (*v_y < -v_max)
+        *v_y = -v_max;
+}
+
+/* src/fluid.c:656-744 */
+void boundaryConditions(fluid_particle *p, AABB_t *boundary, param *params)
+{
+    const tunable_parameters *t = &params->tunable_params;
+    float cx = t->mover_center_x, cy = t->mover_center_y;
+    if (t->mover_type == SPHERE_MOVER) {
+        float radius = t->mover_width * 0.5f;
+        float d2 = (p->x - cx) * (p->x - cx) + (p->y - cy) * (p->y - cy);
+        if (d2 <= radius * radius && d2 > 0.0f) {
+            float d = sqrtf(d2);
+            float nx = (cx - p->x) / d;
+            float ny = (cy - p->y) / d;
+            float pen = radius - d;
+            p->x -= pen * nx;
+            p->y -= pen * ny;
+        }
+    } else if (t->mover_type == RECTANGLE_MOVER) {
+        float hw = (float)(t->mover_width * 0.5);
+        float hh = (float)(t->mover_height * 0.5);
+        float rx = p->x - cx, ry = p->y - cy;
+        float ax = fabsf(rx), ay = fabsf(ry);
+        if (ax < hw && ay < hh) {
+            float penx = hw - ax, peny = hh - ay;
+            if (penx < peny)
+                p->x += rx < 0.0f ? -penx : penx;
+            else
+                p->y += ry < 0.0f ? -peny : peny;
+        }
+    }
+    if (p->x < boundary->min_x)
+        p->x = boundary->min_x;
+    else if (p->x > boundary->max_x)
+        p->x = boundary->max_x - 0.001f;
+    if (p->y < boundary->min_y)
+        p->y = boundary->min_y;
+    else if (p->y > boundary->max_y)
+        p->y = boundary->max_y - 0.001f;
+}
+
+/* src/fluid.c:527-539 */
+void calculate_density(fluid_particle *p, fluid_particle *q, float ratio)
+{
+    float OmR2 = (1.0f - ratio) * (1.0f - ratio);
+    if (ratio < 1.0f) {
+        p->density += OmR2;
+        p->density_near += OmR2 * (1.0f - ratio);
+        q->density += OmR2;
+        q->density_near += OmR2 * (1.0f - ratio);
+    }
+}
+
+/* src/fluid.c:627-639 */
+void updateVelocity(fluid_particle *p, param *params)
+{
+    float dt = params->tunable_params.time_step;
+    float v_x = (p->x - p->x_prev) / dt;
+    float v_y = (p->y - p->y_prev) / dt;
+    checkVelocity(&v_x, &v_y);
+    p->v_x = v_x;
+    p->v_y = v_y;
+}
+
+/* src/hash.c:35-47 */
+unsigned int hash_val(float x, float y, neighbor_grid_t *grid, param *params)
+{
+    (void)params;
+    unsigned gx = (unsigned)floorf(x / grid->spacing);
+    unsigned gy = (unsigned)floorf(y / grid->spacing);
+    return gy * grid->size_x + gx;
+}
+
+/* src/fluid.c:747-768 with src/geometry.c:29-67 on one domain. */
+void initParticles(fluid_particle **fluid_particle_pointers, fluid_particle *fluid_particles,
+                   AABB_t *water, int start_x, int number_particles_x, edge_t *edges,
+                   int max_fluid_particles_local, float spacing, param *params)
+{
+    (void)edges; /* halo bookkeeping: MPI only */
+    int num_y = (int)floor((water->max_y - water->min_y) / spacing);
+    int i = 0;
+    for (int ny = 0; ny < num_y; ny++) {
+        float y = water->min_y + ny * spacing;
+        for (int nx = 0; nx < number_particles_x; nx++) {
+            fluid_particle *p = fluid_particles + i;
+            p->x = water->min_x + (start_x + nx) * spacing;
+            p->y = y;
+            fluid_particle_pointers[i] = p;
+            p->id = i;
+            i++;
+        }
+    }
+    params->number_fluid_particles_local = i;
+    params->max_fluid_particle_index = i - 1;
+    for (int k = i; k < max_fluid_particles_local; k++)
+        fluid_particle_pointers[k] = NULL;
+    for (int k = 0; k < i; k++) {
+        fluid_particle_pointers[k]->a_x = 0.0f;
+        fluid_particle_pointers[k]->a_y = 0.0f;
+        fluid_particle_pointers[k]->v_x = 0.0f;
+        fluid_particle_pointers[k]->v_y = 0.0f;
+    }
+    g_dirty = 1;
+}
+
+/* ---- step functions on the device -------------------------------------- */
+
+/* src/fluid.c:398-413 */
+void apply_gravity(fluid_particle **fluid_particle_pointers, param *params)
+{
+    if (!g_ctx) { /* before the context exists: host pass, reference arithmetic */
+        float dt = params->tunable_params.time_step, g = -params->tunable_params.g;
+        for (int i = 0; i < params->number_fluid_particles_local; i++) {
+            fluid_particle *p = fluid_particle_pointers[i];
+            p->v_y += g * dt;
+            p->density = 0.0f;
+            p->density_near = 0.0f;
+        }
+        return;
+    }
+    int rc = shim_ready(fluid_particle_pointers, params, NULL);
+    if (!rc)
+        rc = sph_phase_gravity(g_ctx);
+    shim_fail("apply_gravity", rc);
+}
+
+/* src/fluid.c:416-478 */
+void viscosity_impluses(fluid_particle **fluid_particle_pointers, neighbor *neighbors,
+                        param *params)
+{
+    (void)neighbors;
+    if (!g_ctx)
+        return; /* no hash yet: the reference's lists are empty */
+    int rc = shim_ready(fluid_particle_pointers, params, NULL);
+    if (!rc)
+        rc = sph_phase_viscosity(g_ctx);
+    shim_fail("viscosity_impluses", rc);
+}
+
+/* src/fluid.c:507-523 */
+void predict_positions(fluid_particle **fluid_particle_pointers, AABB_t *boundary_global,
+                       param *params)
+{
+    int rc = shim_ready(fluid_particle_pointers, params, boundary_global);
+    if (!rc)
+        rc = sph_phase_predict(g_ctx);
+    shim_fail("predict_positions", rc);
+}
+
+/* src/fluid.c:481-502: on one domain nothing leaves the box (every
+ * particle was clamped into it by predict_positions) and the pointer array
+ * stays compact with id = index, so there is nothing to do. */
+void identify_oob_particles(fluid_particle **fluid_particle_pointers,
+                            fluid_particle *fluid_particles, oob_t *out_of_bounds,
+                            AABB_t *boundary_global, param *params)
+{
+    (void)fluid_particle_pointers;
+    (void)fluid_particles;
+    (void)out_of_bounds;
+    (void)boundary_global;
+    (void)params;
+}
+
+/* src/hash.c:127-242 */
+void hash_fluid(fluid_particle **fluid_particle_pointers, neighbor_grid_t *grid, param *params,
+                bool compute_density)
+{
+    (void)grid;
+    int rc = shim_ready(fluid_particle_pointers, params, NULL);
+    if (!rc)
+        rc = sph_phase_hash(g_ctx, compute_density ? 1 : 0);
+    if (!rc && !compute_density) { /* end of a step (src/fluid.c:341) */
+        ++g_steps_since_sync;
+        if (g_sync_interval > 0 && g_steps_since_sync >= g_sync_interval)
+            rc = sph_shim_sync();
+    }
+    shim_fail("hash_fluid", rc);
+}
+
+/* src/hash.c:51-122: there are no halo particles on one domain. */
+void hash_halo(fluid_particle **fluid_particle_pointers, neighbor_grid_t *grid, param *params,
+               bool compute_density)
+{
+    (void)fluid_particle_pointers;
+    (void)grid;
+    (void)params;
+    (void)compute_density;
+}
+
+/* src/fluid.c:541-611 */
+void double_density_relaxation(fluid_particle **fluid_particle_pointers, neighbor *neighbors,
+                               param *params)
+{
+    (void)neighbors;
+    int rc = shim_ready(fluid_particle_pointers, params, NULL);
+    if (!rc)
+        rc = sph_phase_relax(g_ctx);
+    shim_fail("double_density_relaxation", rc);
+}
+
+/* src/fluid.c:642-653 */
+void updateVelocities(fluid_particle **fluid_particle_pointers, edge_t *edges,
+                      AABB_t *boundary_global, param *params)
+{
+    (void)edges;
+    int rc = shim_ready(fluid_particle_pointers, params, boundary_global);
+    if (!rc)
+        rc = sph_phase_update_velocities(g_ctx);
+    shim_fail("updateVelocities", rc);
+}

@@ -1,0 +1,162 @@
+"""Write the golden fixtures of tests/golden/*.npz.
+
+Runs the reference's own step functions (src/hash.c, src/fluid.c of
+AdamSimpson/SPH, compiled unmodified into oracle/_ref by oracle/Makefile with
+strict IEEE float: -O2 -ffp-contract=off, no fast-math) on small seeded
+scenes and records inputs and outputs.  Only data is stored.
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Fixtures:
+  c1_dambreak.npz  N=4096 dam break (64x64 lattice at 0.5h, box 60x40,
+                   rho0=10, k=0.004, k_near=0.01, k_spring=10, sigma=0,
+                   beta=0.3, g=9.8, dt=1/60): state after steps 1, 2, 10, 100
+                   and the half-list lengths after step 2.
+  phases64.npz     N=64 scene with viscosity on (sigma=0.1): state after every
+                   phase of 8 steps, and the half lists after each hash.
+  movers.npz       N=400 with a sphere mover, then with a rectangle mover:
+                   state after each of 6 steps.
+  caps.npz         dense clusters: cells above 100 particles and half lists
+                   above 400, where the reference drops work.
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from oracle import pyoracle as po  # noqa: E402
+
+FIELDS = ("x", "y", "v_x", "v_y", "x_prev", "y_prev", "density", "density_near")
+
+
+def lattice(nx, ny, spacing, jitter, seed, x0=0.0, y0=0.0):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(nx * ny, po.P32)
+    j, i = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    jx = rng.uniform(-jitter, jitter, nx * ny).astype(np.float32)
+    jy = rng.uniform(-jitter, jitter, nx * ny).astype(np.float32)
+    p["x"] = np.float32(x0) + i.ravel().astype(np.float32) * np.float32(spacing) + jx
+    p["y"] = np.float32(y0) + j.ravel().astype(np.float32) * np.float32(spacing) + jy
+    p["x_prev"], p["y_prev"] = p["x"], p["y"]
+    p["id"] = np.arange(nx * ny)
+    return p
+
+
+def tun(**kw):
+    base = dict(rest_density=10.0, smoothing_radius=1.0, g=9.8, k=0.004, k_near=0.01,
+                k_spring=10.0, sigma=0.0, beta=0.3, time_step=1.0 / 60.0, mover_type=0,
+                mover_width=0.0, mover_height=0.0, active=1)
+    base.update(kw)
+    return po.make_tunables(**base)
+
+
+def tun_array(t):
+    return np.frombuffer(bytes(t), np.uint8).copy()
+
+
+def snap(parts):
+    return np.stack([parts[f] for f in FIELDS])
+
+
+def c1():
+    box = po.make_box(60, 40)
+    t = tun()
+    p0 = lattice(64, 64, 0.5, 0.01, seed=20260101)
+    ref = po.CompiledRef(box, t, len(p0))
+    ref.load(p0)
+    out = {"input": snap(p0), "tunables": tun_array(t), "box": np.array([60, 40], np.float32)}
+    done = 0
+    for s in (1, 2, 10, 100):
+        ref.step(s - done)
+        done = s
+        out[f"step{s}"] = snap(ref.store())
+        if s == 2:
+            out["halflist_len_step2"] = ref.lists()[0]
+    np.savez_compressed(os.path.join(HERE, "c1_dambreak.npz"), **out)
+
+
+def phases64():
+    box = po.make_box(6, 6)
+    t = tun(sigma=0.1, beta=0.3, k=0.2, k_near=0.5, rest_density=3.0)
+    p0 = lattice(8, 8, 0.5, 0.05, seed=7, x0=0.5, y0=0.5)
+    p0["v_x"] = np.float32(0.5)
+    ref = po.CompiledRef(box, t, len(p0))
+    ref.load(p0)
+    snaps, lists = [], []
+    for s in range(8):
+        for ph in range(7):
+            ref.phase(ph)
+            snaps.append(snap(ref.store()))
+            if ph in (3, 6):
+                c, l = ref.lists(stride=64)
+                lists.append((c, l))
+    np.savez_compressed(os.path.join(HERE, "phases64.npz"), input=snap(p0),
+                        input_v=np.stack([p0["v_x"], p0["v_y"]]), tunables=tun_array(t),
+                        box=np.array([6, 6], np.float32), states=np.stack(snaps),
+                        list_len=np.stack([c for c, _ in lists]),
+                        lists=np.stack([l for _, l in lists]))
+
+
+def movers():
+    box = po.make_box(12, 10)
+    out = {"box": np.array([12, 10], np.float32)}
+    p0 = lattice(20, 20, 0.5, 0.01, seed=11, x0=0.25, y0=0.25)
+    out["input"] = snap(p0)
+    for name, kw in (("sphere", dict(mover_type=0, mover_center_x=5.0, mover_center_y=3.0,
+                                     mover_width=3.0, mover_height=3.0)),
+                     ("rect", dict(mover_type=1, mover_center_x=4.0, mover_center_y=2.5,
+                                   mover_width=2.0, mover_height=3.0))):
+        t = tun(**kw)
+        ref = po.CompiledRef(box, t, len(p0))
+        ref.load(p0)
+        steps = []
+        for _ in range(6):
+            ref.step(1)
+            steps.append(snap(ref.store()))
+        out[name] = np.stack(steps)
+        out[name + "_tunables"] = tun_array(t)
+    np.savez_compressed(os.path.join(HERE, "movers.npz"), **out)
+
+
+def caps():
+    """Five 101-particle clusters around the right edge of cell (0,1): each
+    cluster overflows its bucket (101 > 100) and the owners in cell (0,1)
+    see 100 own + 4x100 forward candidates within h (> 400 half slots)."""
+    box = po.make_box(4, 4)
+    t = tun()
+    rng = np.random.default_rng(5)
+    centres = [(0.96, 1.5), (1.04, 0.96), (1.04, 1.5), (1.04, 2.04), (0.96, 2.04)]
+    pts = []
+    for cx, cy in centres:
+        r = np.sqrt(rng.uniform(0, 1, 101)) * 0.03
+        a = rng.uniform(0, 2 * np.pi, 101)
+        pts.append(np.stack([cx + r * np.cos(a), cy + r * np.sin(a)], 1))
+    xy = np.concatenate(pts).astype(np.float32)
+    p0 = np.zeros(len(xy), po.P32)
+    p0["x"], p0["y"] = xy[:, 0], xy[:, 1]
+    p0["x_prev"], p0["y_prev"] = p0["x"], p0["y"]
+    p0["id"] = np.arange(len(xy))
+    ref = po.CompiledRef(box, t, len(p0))
+    ref.load(p0)
+    out = {"input": snap(p0), "tunables": tun_array(t), "box": np.array([4, 4], np.float32)}
+    for s in (1, 2):
+        ref.step(1)
+        out[f"step{s}"] = snap(ref.store())
+        out[f"halflist_len_step{s}"] = ref.lists()[0]
+        out[f"bucket_counts_step{s}"] = ref.bucket_counts()
+    np.savez_compressed(os.path.join(HERE, "caps.npz"), **out)
+
+
+if __name__ == "__main__":
+    if not po.have_compiled_ref():
+        sys.exit("oracle/_ref/libtinysph_ref.so is not built (make -C oracle ref)")
+    c1()
+    phases64()
+    movers()
+    caps()
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")
