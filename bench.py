@@ -80,12 +80,32 @@ def cpu_baseline(tun, box, parts, budget_s=15.0):
                       f"{dt:.1f} s on one host core ({os.cpu_count()} visible)"}
 
 
+def calibration_copy(mbytes):
+    """A device-to-device copy of known size through the HIP runtime (a blit
+    kernel), run after the timed region: its FETCH_SIZE / WRITE_SIZE counter
+    readings calibrate the counters' units against known bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = int(mbytes * 1e6)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(n))
+    for _ in range(3):
+        hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calib-copy-mb", type=float, default=0.0,
+                    help="also run a device copy of this many MB (profiler calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs one scene on one GPU; --gpus must be 1"}))
@@ -144,6 +164,8 @@ def main():
         "kernels": kernel_bw,
         "counters": counters,
     }
+    if args.calib_copy_mb > 0:
+        calibration_copy(args.calib_copy_mb)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(tun, box, parts)
     print(json.dumps(out))

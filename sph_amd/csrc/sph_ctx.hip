@@ -57,20 +57,20 @@ struct sph_ctx {
     sph_dev_params hp{};
     sph_dev_params *dP = nullptr;
     sph_dev_counters *dC = nullptr;
-    // state (canonical: sorted by (cell, id); ID0/K0 give id and key)
-    float *X0 = nullptr, *Y0 = nullptr, *VX = nullptr, *VY = nullptr;
-    float *XP = nullptr, *YP = nullptr, *RHO = nullptr, *RHON = nullptr;
+    // state (canonical: sorted by (cell, id); ID0/K0 give id and key);
+    // pairs interleaved: POS=(x,y) VEL=(v_x,v_y) PREV=(x_prev,y_prev)
+    // RHO=(density,density_near)
+    float2 *POS0 = nullptr, *VEL = nullptr, *PREV = nullptr, *RHO = nullptr;
     // work buffers
-    float *X1 = nullptr, *Y1 = nullptr, *UX = nullptr, *UY = nullptr, *UVX = nullptr,
-          *UVY = nullptr, *T0 = nullptr, *T1 = nullptr;
+    float2 *POS1 = nullptr, *UPOS = nullptr, *UVEL = nullptr, *T2a = nullptr, *T2b = nullptr;
     int *ID0 = nullptr, *ID1 = nullptr, *perm_src = nullptr, *perm_id = nullptr,
         *pos_of_id = nullptr;
     unsigned *K0 = nullptr, *K1 = nullptr, *ukey = nullptr, *uslot = nullptr, *cnt = nullptr,
              *start = nullptr, *bsum = nullptr;
     short *coords = nullptr;
     std::vector<float> ax_by_id, ay_by_id; // a_x/a_y: carried, never changed (src/fluid.c)
-    bool lists_valid = false; // a hash ran since upload: K0/start describe X0/Y0
-    int aux_order = 0;        // XP,YP,RHO,RHON ordered like ID0 (0) or ID1 (1)
+    bool lists_valid = false; // a hash ran since upload: K0/start describe POS0
+    int aux_order = 0;        // PREV, RHO ordered like ID0 (0) or ID1 (1)
     hipGraphExec_t g1 = nullptr, gK = nullptr;
     int graph_n = -1;
     uint64_t steps = 0, hashes = 0;
@@ -172,41 +172,39 @@ void enqueue_sort_tail(sph_ctx *c, const int *id_src)
 
 // The fused step: 11 launches, no host synchronisation.
 //   visc_predict: gravity + viscosity + predict + boundary + key/slot
-//   scan, scan, scatter, reorder   -> sorted predicted state (X1, Y1, XP, YP)
-//   density                        -> RHO, RHON
+//   scan, scan, scatter, reorder   -> sorted predicted state (POS1, PREV)
+//   density                        -> RHO
 //   relax: relaxation + boundary + velocity update + key/slot
-//   scan, scan, scatter, reorder   -> sorted relaxed state (X0, Y0, VX, VY)
+//   scan, scan, scatter, reorder   -> sorted relaxed state (POS0, VEL)
 void enqueue_fused_step(sph_ctx *c, bool visc)
 {
     const int n = c->n;
     {
         Tick t(c, KV);
-        sph_launch_visc_predict(c->st, n, true, visc, true, c->dP, c->X0, c->Y0, c->VX, c->VY,
-                                c->K0, c->start, c->UX, c->UY, c->ukey, c->uslot, c->cnt, c->dC);
+        sph_launch_visc_predict(c->st, n, true, visc, true, c->dP, c->POS0, c->VEL, c->K0,
+                                c->start, c->UPOS, c->ukey, c->uslot, c->cnt, c->dC);
     }
     enqueue_sort_tail(c, c->ID0);
     {
         Tick t(c, KRO);
-        sph_reorder_fields<4> f{{c->UX, c->UY, c->X0, c->Y0}, {c->X1, c->Y1, c->XP, c->YP}};
-        sph_launch_reorder4(c->st, n, c->start, c->perm_src, c->perm_id, c->ukey, f, c->ID1,
+        sph_reorder_fields<2> f{{c->UPOS, c->POS0}, {c->POS1, c->PREV}};
+        sph_launch_reorder2(c->st, n, c->start, c->perm_src, c->perm_id, c->ukey, f, c->ID1,
                             c->K1);
     }
     {
         Tick t(c, KDE);
-        sph_launch_density(c->st, n, c->dP, c->X1, c->Y1, c->K1, c->start, c->RHO, c->RHON,
-                           c->dC);
+        sph_launch_density(c->st, n, c->dP, c->POS1, c->K1, c->start, c->RHO, c->dC);
     }
     {
         Tick t(c, KRE);
-        sph_launch_relax(c->st, n, true, c->dP, c->X1, c->Y1, c->XP, c->YP, c->RHO, c->RHON,
-                         c->K1, c->start, c->UX, c->UY, c->UVX, c->UVY, c->ukey, c->uslot,
-                         c->cnt, c->dC);
+        sph_launch_relax(c->st, n, true, c->dP, c->POS1, c->PREV, c->RHO, c->K1, c->start,
+                         c->UPOS, c->UVEL, c->ukey, c->uslot, c->cnt, c->dC);
     }
     enqueue_sort_tail(c, c->ID1);
     {
         Tick t(c, KRO);
-        sph_reorder_fields<4> f{{c->UX, c->UY, c->UVX, c->UVY}, {c->X0, c->Y0, c->VX, c->VY}};
-        sph_launch_reorder4(c->st, n, c->start, c->perm_src, c->perm_id, c->ukey, f, c->ID0,
+        sph_reorder_fields<2> f{{c->UPOS, c->UVEL}, {c->POS0, c->VEL}};
+        sph_launch_reorder2(c->st, n, c->start, c->perm_src, c->perm_id, c->ukey, f, c->ID0,
                             c->K0);
     }
 }
@@ -240,7 +238,7 @@ int ensure_graphs(sph_ctx *c)
     return SPH_OK;
 }
 
-// Bring XP/YP/RHO/RHON into ID0 order (after fused steps they follow ID1).
+// Bring PREV/RHO into ID0 order (after fused steps they follow ID1).
 int canonicalize(sph_ctx *c)
 {
     if (c->aux_order == 0 || c->n == 0) {
@@ -249,13 +247,11 @@ int canonicalize(sph_ctx *c)
     }
     const int n = c->n;
     sph_launch_pos_of_id(c->st, n, c->ID0, c->pos_of_id);
-    sph_reorder_fields<4> f{{c->XP, c->YP, c->RHO, c->RHON}, {c->T0, c->T1, c->UVX, c->UVY}};
+    sph_reorder_fields<2> f{{c->PREV, c->RHO}, {c->T2a, c->T2b}};
     sph_launch_permute_by_id(c->st, n, c->ID1, c->pos_of_id, f);
-    size_t b = sizeof(float) * (size_t)n;
-    HIPCHK(hipMemcpyAsync(c->XP, c->T0, b, hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(c->YP, c->T1, b, hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(c->RHO, c->UVX, b, hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(c->RHON, c->UVY, b, hipMemcpyDeviceToDevice, c->st));
+    size_t b = sizeof(float2) * (size_t)n;
+    HIPCHK(hipMemcpyAsync(c->PREV, c->T2a, b, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->RHO, c->T2b, b, hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipGetLastError());
     c->aux_order = 0;
     return SPH_OK;
@@ -328,8 +324,8 @@ int sph_create(sph_ctx **out, const param *params, const AABB_t *boundary, int c
 #define A(p, cnt)                                                                                \
     if (!rc)                                                                                     \
     rc = dalloc(&c->p, cnt)
-    A(X0, N); A(Y0, N); A(VX, N); A(VY, N); A(XP, N); A(YP, N); A(RHO, N); A(RHON, N);
-    A(X1, N); A(Y1, N); A(UX, N); A(UY, N); A(UVX, N); A(UVY, N); A(T0, N); A(T1, N);
+    A(POS0, N); A(VEL, N); A(PREV, N); A(RHO, N);
+    A(POS1, N); A(UPOS, N); A(UVEL, N); A(T2a, N); A(T2b, N);
     A(ID0, N); A(ID1, N); A(perm_src, N); A(perm_id, N); A(pos_of_id, N);
     A(K0, N); A(K1, N); A(ukey, N); A(uslot, N);
     A(cnt, M); A(start, M); A(bsum, sph_scan_blocks((unsigned)M));
@@ -366,11 +362,10 @@ void sph_destroy(sph_ctx *c)
     if (c->st)
         (void)hipStreamSynchronize(c->st);
     free_graphs(c);
-    void *ptrs[] = {c->X0,  c->Y0,   c->VX,   c->VY,       c->XP,      c->YP,    c->RHO,
-                    c->RHON, c->X1,  c->Y1,   c->UX,       c->UY,      c->UVX,   c->UVY,
-                    c->T0,  c->T1,   c->ID0,  c->ID1,      c->perm_src, c->perm_id,
-                    c->pos_of_id, c->K0, c->K1, c->ukey,   c->uslot,   c->cnt,   c->start,
-                    c->bsum, c->coords, c->dP, c->dC};
+    void *ptrs[] = {c->POS0, c->VEL,  c->PREV,     c->RHO,     c->POS1,      c->UPOS,
+                    c->UVEL, c->T2a,  c->T2b,      c->ID0,     c->ID1,       c->perm_src,
+                    c->perm_id, c->pos_of_id, c->K0, c->K1,    c->ukey,      c->uslot,
+                    c->cnt,  c->start, c->bsum,    c->coords,  c->dP,        c->dC};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -401,7 +396,7 @@ static int upload_impl(sph_ctx *c, const fluid_particle *const *pp, const fluid_
         return set_err(SPH_EINVAL, "sph_upload: n=%d outside [0, capacity=%d]", n, c->cap);
     if (n > 0 && !pp && !arr)
         return set_err(SPH_EINVAL, "sph_upload: null particles");
-    std::vector<float> f[8];
+    std::vector<float2> f[4];
     for (auto &v : f)
         v.resize((size_t)n);
     std::vector<int> ids((size_t)n);
@@ -417,22 +412,18 @@ static int upload_impl(sph_ctx *c, const fluid_particle *const *pp, const fluid_
                            n - 1, p->id, i);
         seen[p->id] = 1;
         ids[i] = p->id;
-        f[0][i] = p->x;
-        f[1][i] = p->y;
-        f[2][i] = p->v_x;
-        f[3][i] = p->v_y;
-        f[4][i] = p->x_prev;
-        f[5][i] = p->y_prev;
-        f[6][i] = p->density;
-        f[7][i] = p->density_near;
+        f[0][i] = make_float2(p->x, p->y);
+        f[1][i] = make_float2(p->v_x, p->v_y);
+        f[2][i] = make_float2(p->x_prev, p->y_prev);
+        f[3][i] = make_float2(p->density, p->density_near);
         c->ax_by_id[p->id] = p->a_x;
         c->ay_by_id[p->id] = p->a_y;
     }
     HIPCHK(hipStreamSynchronize(c->st));
-    float *dst[8] = {c->X0, c->Y0, c->VX, c->VY, c->XP, c->YP, c->RHO, c->RHON};
-    const size_t b = sizeof(float) * (size_t)n;
+    float2 *dst[4] = {c->POS0, c->VEL, c->PREV, c->RHO};
+    const size_t b = sizeof(float2) * (size_t)n;
     if (n > 0) {
-        for (int k = 0; k < 8; k++)
+        for (int k = 0; k < 4; k++)
             HIPCHK(hipMemcpy(dst[k], f[k].data(), b, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(c->ID0, ids.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice));
     }
@@ -536,7 +527,7 @@ int sph_phase_gravity(sph_ctx *c)
     if (int rc = canonicalize(c))
         return rc;
     if (c->n)
-        sph_launch_gravity(c->st, c->n, c->dP, c->VY, c->RHO, c->RHON);
+        sph_launch_gravity(c->st, c->n, c->dP, c->VEL, c->RHO);
     HIPCHK(hipGetLastError());
     return SPH_OK;
 }
@@ -549,11 +540,10 @@ int sph_phase_viscosity(sph_ctx *c)
         return rc;
     if (!c->lists_valid || c->n == 0)
         return SPH_OK; // empty lists (src/fluid.c:202)
-    sph_launch_visc_predict(c->st, c->n, false, true, false, c->dP, c->X0, c->Y0, c->VX, c->VY,
-                            c->K0, c->start, c->UVX, c->UVY, nullptr, nullptr, c->cnt, c->dC);
-    const size_t b = sizeof(float) * (size_t)c->n;
-    HIPCHK(hipMemcpyAsync(c->VX, c->UVX, b, hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(c->VY, c->UVY, b, hipMemcpyDeviceToDevice, c->st));
+    sph_launch_visc_predict(c->st, c->n, false, true, false, c->dP, c->POS0, c->VEL, c->K0,
+                            c->start, c->UVEL, nullptr, nullptr, c->cnt, c->dC);
+    HIPCHK(hipMemcpyAsync(c->VEL, c->UVEL, sizeof(float2) * (size_t)c->n,
+                          hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipGetLastError());
     return SPH_OK;
 }
@@ -565,7 +555,7 @@ int sph_phase_predict(sph_ctx *c)
     if (int rc = canonicalize(c))
         return rc;
     if (c->n)
-        sph_launch_predict(c->st, c->n, c->dP, c->X0, c->Y0, c->VX, c->VY, c->XP, c->YP);
+        sph_launch_predict(c->st, c->n, c->dP, c->POS0, c->VEL, c->PREV);
     HIPCHK(hipGetLastError());
     return SPH_OK;
 }
@@ -579,19 +569,19 @@ int sph_phase_hash(sph_ctx *c, int compute_density)
     const int n = c->n;
     if (n == 0)
         return SPH_OK;
-    sph_launch_key_hist(c->st, n, c->dP, c->X0, c->Y0, c->ukey, c->uslot, c->cnt, c->dC);
+    sph_launch_key_hist(c->st, n, c->dP, c->POS0, c->ukey, c->uslot, c->cnt, c->dC);
     enqueue_sort_tail(c, c->ID0);
-    sph_reorder_fields<8> f{{c->X0, c->Y0, c->VX, c->VY, c->XP, c->YP, c->RHO, c->RHON},
-                            {c->X1, c->Y1, c->UX, c->UY, c->UVX, c->UVY, c->T0, c->T1}};
-    sph_launch_reorder8(c->st, n, c->start, c->perm_src, c->perm_id, c->ukey, f, c->ID1, c->K1);
-    const size_t b = sizeof(float) * (size_t)n;
-    for (int k = 0; k < 8; k++)
+    sph_reorder_fields<4> f{{c->POS0, c->VEL, c->PREV, c->RHO},
+                            {c->POS1, c->UVEL, c->T2a, c->T2b}};
+    sph_launch_reorder4(c->st, n, c->start, c->perm_src, c->perm_id, c->ukey, f, c->ID1, c->K1);
+    const size_t b = sizeof(float2) * (size_t)n;
+    for (int k = 0; k < 4; k++)
         HIPCHK(hipMemcpyAsync((void *)f.src[k], f.dst[k], b, hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->ID0, c->ID1, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->K0, c->K1, sizeof(unsigned) * (size_t)n, hipMemcpyDeviceToDevice,
                           c->st));
     if (compute_density)
-        sph_launch_density(c->st, n, c->dP, c->X0, c->Y0, c->K0, c->start, c->RHO, c->RHON, c->dC);
+        sph_launch_density(c->st, n, c->dP, c->POS0, c->K0, c->start, c->RHO, c->dC);
     HIPCHK(hipGetLastError());
     c->lists_valid = true;
     c->hashes++;
@@ -608,12 +598,10 @@ int sph_phase_relax(sph_ctx *c)
         return SPH_OK;
     if (!c->lists_valid)
         return set_err(SPH_ESTATE, "sph_phase_relax: no hash since upload");
-    sph_launch_relax(c->st, c->n, false, c->dP, c->X0, c->Y0, c->XP, c->YP, c->RHO, c->RHON,
-                     c->K0, c->start, c->UX, c->UY, nullptr, nullptr, nullptr, nullptr, nullptr,
-                     c->dC);
-    const size_t b = sizeof(float) * (size_t)c->n;
-    HIPCHK(hipMemcpyAsync(c->X0, c->UX, b, hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(c->Y0, c->UY, b, hipMemcpyDeviceToDevice, c->st));
+    sph_launch_relax(c->st, c->n, false, c->dP, c->POS0, c->PREV, c->RHO, c->K0, c->start,
+                     c->UPOS, nullptr, nullptr, nullptr, nullptr, c->dC);
+    HIPCHK(hipMemcpyAsync(c->POS0, c->UPOS, sizeof(float2) * (size_t)c->n,
+                          hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipGetLastError());
     return SPH_OK;
 }
@@ -625,7 +613,7 @@ int sph_phase_update_velocities(sph_ctx *c)
     if (int rc = canonicalize(c))
         return rc;
     if (c->n)
-        sph_launch_update_velocities(c->st, c->n, c->dP, c->X0, c->Y0, c->VX, c->VY, c->XP, c->YP);
+        sph_launch_update_velocities(c->st, c->n, c->dP, c->POS0, c->VEL, c->PREV);
     HIPCHK(hipGetLastError());
     return SPH_OK;
 }
@@ -642,10 +630,10 @@ int sph_download(sph_ctx *c, fluid_particle *out)
     HIPCHK(hipStreamSynchronize(c->st));
     if (n == 0)
         return SPH_OK;
-    std::vector<float> f[8];
-    const float *src[8] = {c->X0, c->Y0, c->VX, c->VY, c->XP, c->YP, c->RHO, c->RHON};
-    const size_t b = sizeof(float) * (size_t)n;
-    for (int k = 0; k < 8; k++) {
+    std::vector<float2> f[4];
+    const float2 *src[4] = {c->POS0, c->VEL, c->PREV, c->RHO};
+    const size_t b = sizeof(float2) * (size_t)n;
+    for (int k = 0; k < 4; k++) {
         f[k].resize((size_t)n);
         HIPCHK(hipMemcpy(f[k].data(), src[k], b, hipMemcpyDeviceToHost));
     }
@@ -656,23 +644,23 @@ int sph_download(sph_ctx *c, fluid_particle *out)
     const tunable_parameters &t = c->tun;
     for (int s = 0; s < n; s++) {
         fluid_particle *p = &out[id0[s]];
-        p->x = f[0][s];
-        p->y = f[1][s];
-        p->v_x = f[2][s];
-        p->v_y = f[3][s];
+        p->x = f[0][s].x;
+        p->y = f[0][s].y;
+        p->v_x = f[1][s].x;
+        p->v_y = f[1][s].y;
         p->id = id0[s];
         p->a_x = c->ax_by_id[id0[s]];
         p->a_y = c->ay_by_id[id0[s]];
     }
     for (int s = 0; s < n; s++) {
         fluid_particle *p = &out[idaux[s]];
-        p->x_prev = f[4][s];
-        p->y_prev = f[5][s];
-        p->density = f[6][s];
-        p->density_near = f[7][s];
+        p->x_prev = f[2][s].x;
+        p->y_prev = f[2][s].y;
+        p->density = f[3][s].x;
+        p->density_near = f[3][s].y;
         // src/fluid.c:563-564
-        p->pressure = t.k * (f[6][s] - t.rest_density);
-        p->pressure_near = t.k_near * f[7][s];
+        p->pressure = t.k * (f[3][s].x - t.rest_density);
+        p->pressure_near = t.k_near * f[3][s].y;
     }
     return SPH_OK;
 }
@@ -685,8 +673,7 @@ int sph_pack_coords_i16(sph_ctx *c, short *coords)
         return set_err(SPH_EINVAL, "sph_pack_coords_i16: null output");
     if (c->n == 0)
         return SPH_OK;
-    sph_launch_pack_coords(c->st, c->n, c->X0, c->Y0, c->ID0, c->box.max_x, c->box.max_y,
-                           c->coords);
+    sph_launch_pack_coords(c->st, c->n, c->POS0, c->ID0, c->box.max_x, c->box.max_y, c->coords);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(coords, c->coords, sizeof(short) * 2 * (size_t)c->n,
                           hipMemcpyDeviceToHost, c->st));
@@ -703,22 +690,32 @@ int sph_read_sorted(sph_ctx *c, int field, void *out)
     if (int rc = canonicalize(c))
         return rc;
     const void *src = nullptr;
+    int comp = -1; // component of an interleaved pair, -1 for 4-byte fields
     switch (field) {
-    case SPH_FIELD_X: src = c->X0; break;
-    case SPH_FIELD_Y: src = c->Y0; break;
-    case SPH_FIELD_VX: src = c->VX; break;
-    case SPH_FIELD_VY: src = c->VY; break;
-    case SPH_FIELD_XPREV: src = c->XP; break;
-    case SPH_FIELD_YPREV: src = c->YP; break;
-    case SPH_FIELD_DENSITY: src = c->RHO; break;
-    case SPH_FIELD_DENSITY_NEAR: src = c->RHON; break;
+    case SPH_FIELD_X: src = c->POS0; comp = 0; break;
+    case SPH_FIELD_Y: src = c->POS0; comp = 1; break;
+    case SPH_FIELD_VX: src = c->VEL; comp = 0; break;
+    case SPH_FIELD_VY: src = c->VEL; comp = 1; break;
+    case SPH_FIELD_XPREV: src = c->PREV; comp = 0; break;
+    case SPH_FIELD_YPREV: src = c->PREV; comp = 1; break;
+    case SPH_FIELD_DENSITY: src = c->RHO; comp = 0; break;
+    case SPH_FIELD_DENSITY_NEAR: src = c->RHO; comp = 1; break;
     case SPH_FIELD_ID: src = c->ID0; break;
     case SPH_FIELD_KEY: src = c->K0; break;
     default: return set_err(SPH_EINVAL, "sph_read_sorted: unknown field %d", field);
     }
     HIPCHK(hipStreamSynchronize(c->st));
-    if (c->n)
+    if (c->n == 0)
+        return SPH_OK;
+    if (comp < 0) {
         HIPCHK(hipMemcpy(out, src, 4 * (size_t)c->n, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float2> tmp((size_t)c->n);
+        HIPCHK(hipMemcpy(tmp.data(), src, sizeof(float2) * (size_t)c->n, hipMemcpyDeviceToHost));
+        float *o = (float *)out;
+        for (int i = 0; i < c->n; i++)
+            o[i] = comp ? tmp[i].y : tmp[i].x;
+    }
     return SPH_OK;
 }
 
@@ -756,7 +753,7 @@ int sph_read_neighbors(sph_ctx *c, int stride, int *full_by_id, int *half_by_id,
     if (!rc)
         rc = dalloc(&dids, (size_t)n * (size_t)(stride ? stride : 1));
     if (!rc) {
-        sph_launch_neighbor_lists(c->st, n, c->dP, c->X0, c->Y0, c->K0, c->ID0, c->start, stride,
+        sph_launch_neighbor_lists(c->st, n, c->dP, c->POS0, c->K0, c->ID0, c->start, stride,
                                   dfull, dhalf, dids);
         hipError_t e = hipStreamSynchronize(c->st);
         if (e == hipSuccess)

@@ -54,6 +54,28 @@ __device__ __forceinline__ unsigned sph_cell_key(float x, float y, const sph_dev
     return (unsigned)k;
 }
 
+// Correctly rounded square root.  For 2^-96 <= x < inf, v_sqrt_f32 is
+// within one ulp of sqrt(x); the rounded result is the candidate among
+// s - ulp, s, s + ulp whose square brackets x: s - ulp when
+// x - (s - ulp)*s <= 0, s + ulp when x - (s + ulp)*s > 0 (each residual
+// formed with one rounding by an FMA), else s.  Smaller, zero and
+// non-finite inputs take the general sqrtf path.  Results equal sqrtf's
+// (IEEE round-to-nearest) for every input.
+__device__ __forceinline__ float sph_sqrt(float x)
+{
+    if (!(x >= 0x1p-96f && x < __builtin_inff()))
+        return sqrtf(x);
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __int_as_float(__float_as_int(s) - 1);
+    const float su = __int_as_float(__float_as_int(s) + 1);
+    float r = s;
+    if (__builtin_fmaf(-sd, s, x) <= 0.0f)
+        r = sd;
+    if (__builtin_fmaf(-su, s, x) > 0.0f)
+        r = su;
+    return r;
+}
+
 // src/fluid.c:613-625
 __device__ __forceinline__ void sph_check_velocity(float &vx, float &vy)
 {
@@ -134,7 +156,7 @@ __device__ __forceinline__ bool sph_visc_impulse(float px, float py, float pvx, 
 {
     float QmP_x = qx - px;
     float QmP_y = qy - py;
-    float r = sqrtf(QmP_x * QmP_x + QmP_y * QmP_y);
+    float r = sph_sqrt(QmP_x * QmP_x + QmP_y * QmP_y);
     float r_recip = 1.0f / r;
     float ratio = r * P.h_recip;
     float u = ((pvx - qvx) * QmP_x + (pvy - qvy) * QmP_y) * r_recip;
@@ -148,41 +170,10 @@ __device__ __forceinline__ bool sph_visc_impulse(float px, float py, float pvx, 
     return false;
 }
 
-// The 3x3 neighbourhood of cell `key` as three sorted-order row ranges.
-// For row r (dy = r - j) the candidates in [b0, b3) lie in cell i-1 for
-// k < b1, cell i for b1 <= k < b2, and cell i+1 for k >= b2.
-struct sph_rows {
-    int nr;
-    int dy[3];
-    unsigned b0[3], b1[3], b2[3], b3[3];
-};
-
-__device__ __forceinline__ void sph_row_ranges(unsigned key, const sph_dev_params &P,
-                                               const unsigned *__restrict__ start, sph_rows &R)
-{
-    const unsigned sx = P.sx, sy = P.sy;
-    const unsigned i = key % sx, j = key / sx;
-    const unsigned i0 = i > 0 ? i - 1 : 0;
-    const unsigned i1 = i + 1 < sx ? i + 1 : sx - 1;
-    R.nr = 0;
-#pragma unroll
-    for (int d = -1; d <= 1; d++) {
-        long r = (long)j + d;
-        if (r < 0 || r >= (long)sy)
-            continue;
-        unsigned base = (unsigned)r * sx;
-        R.dy[R.nr] = d;
-        R.b0[R.nr] = start[base + i0];
-        R.b1[R.nr] = start[base + i];
-        R.b2[R.nr] = start[base + i + 1];
-        R.b3[R.nr] = start[base + i1 + 1];
-        R.nr++;
-    }
-}
-
 // Half-list owner rule of src/hash.c:176-237 for the pair (self at sorted
-// position s, candidate at sorted position k, row offset dy): the pair is in
-// self's half list when the candidate's cell is a forward cell
+// position s, candidate at sorted position k in the row dy = r - j, where
+// cells i-1 / i / i+1 of that row occupy [b0,b1) / [b1,b2) / [b2,b3)): the
+// pair is in self's half list when the candidate's cell is a forward cell
 // (0,+1), (+1,-1), (+1,0), (+1,+1), or shares self's cell with a higher id
 // (cells are sorted by id, so: a later sorted position).
 __device__ __forceinline__ bool sph_owns(unsigned k, unsigned s, int dy, unsigned b1, unsigned b2)
@@ -194,4 +185,76 @@ __device__ __forceinline__ bool sph_owns(unsigned k, unsigned s, int dy, unsigne
     if (dy == 1)
         return true; // (0, +1)
     return dy == 0 && k > s;
+}
+
+// Walk the 3x3 neighbourhood of the particle at sorted position s (cell
+// `key`) in canonical order: rows j-1, j, j+1; in each row the contiguous
+// sorted range of cells i-1..i+1 (clipped to the grid), ascending.  Self
+// (k == s) is visited too; every body rejects it together with its distance
+// test, so the wave takes one branch per candidate.  `src.row(dy)` prepares
+// a row (an LDS offset for staged tiles); `src.at(k, row)` returns candidate
+// k's staged record; body(k, dy, b1, b2, rec).
+template <class Src, class Body>
+__device__ __forceinline__ void sph_walk(unsigned s, unsigned key, const sph_dev_params &P,
+                                         const unsigned *__restrict__ start, const Src &src,
+                                         Body &&body)
+{
+    const unsigned sx = P.sx, sy = P.sy;
+    const unsigned i = key % sx, j = key / sx;
+    const unsigned i0 = i > 0 ? i - 1 : 0;
+    const unsigned i1 = i + 1 < sx ? i + 1 : sx - 1;
+#pragma unroll
+    for (int dy = -1; dy <= 1; dy++) {
+        const long r = (long)j + dy;
+        if (r < 0 || r >= (long)sy)
+            continue;
+        const unsigned base = (unsigned)r * sx;
+        const unsigned b0 = start[base + i0], b1 = start[base + i];
+        const unsigned b2 = start[base + i + 1], b3 = start[base + i1 + 1];
+        const int row = src.row(dy);
+        for (unsigned k = b0; k < b3; k++)
+            body(k, dy, b1, b2, src.at(k, row));
+    }
+}
+
+// The same walk in two passes per row, for SIMD efficiency: a branch-free
+// pass tests up to 64 candidates at a time (test(k, rec) -> accepted) and
+// records the accepted ones in a 64-bit mask; a second pass runs body() on
+// the set bits only, lowest first.  The visiting order of accepted
+// candidates is exactly sph_walk's, so sums are bit-identical, but a wave no
+// longer executes the pair work for every candidate that any one of its
+// lanes accepts.  Returns the number of accepted candidates.
+template <class Src, class Test, class Body>
+__device__ __forceinline__ unsigned sph_walk_masked(unsigned key, const sph_dev_params &P,
+                                                    const unsigned *__restrict__ start,
+                                                    const Src &src, Test &&test, Body &&body)
+{
+    const unsigned sx = P.sx, sy = P.sy;
+    const unsigned i = key % sx, j = key / sx;
+    const unsigned i0 = i > 0 ? i - 1 : 0;
+    const unsigned i1 = i + 1 < sx ? i + 1 : sx - 1;
+    unsigned accepted = 0;
+#pragma unroll
+    for (int dy = -1; dy <= 1; dy++) {
+        const long r = (long)j + dy;
+        if (r < 0 || r >= (long)sy)
+            continue;
+        const unsigned base = (unsigned)r * sx;
+        const unsigned b0 = start[base + i0], b1 = start[base + i];
+        const unsigned b2 = start[base + i + 1], b3 = start[base + i1 + 1];
+        const int row = src.row(dy);
+        for (unsigned c = b0; c < b3; c += 64) {
+            const unsigned m = b3 - c < 64u ? b3 - c : 64u;
+            unsigned long long mask = 0;
+            for (unsigned t = 0; t < m; t++)
+                mask |= (unsigned long long)test(c + t, src.at(c + t, row)) << t;
+            accepted += (unsigned)__popcll(mask);
+            while (mask) {
+                const unsigned t = (unsigned)__builtin_ctzll(mask);
+                mask &= mask - 1;
+                body(c + t, dy, b1, b2, src.at(c + t, row));
+            }
+        }
+    }
+    return accepted;
 }

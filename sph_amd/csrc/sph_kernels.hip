@@ -1,24 +1,41 @@
 // sph_kernels.hip - gfx950 kernels of the TinySPH step (AdamSimpson/SPH).
 //
 // Data layout: one SoA particle array in HBM, kept sorted by (cell, id)
-// after every hash.  Every physics kernel is a gather over the 3x3 cells
-// around the particle's cell: the candidates are three contiguous row
-// ranges of the sorted arrays, visited in ascending sorted order, so each
-// particle sums its pair terms in one canonical order and no kernel scatters
-// into another particle's state (no atomics on particle data).
+// after every hash, with paired fields interleaved (pos = (x, y),
+// vel = (v_x, v_y), prev = (x_prev, y_prev), rho = (density,
+// density_near)) so a gather reads a pair with one 8-byte load.
+//
+// Physics kernels are gathers over the 3x3 cells around each particle's
+// cell.  A workgroup owns 256 consecutive sorted particles; their
+// candidates are at most six contiguous ranges of the sorted arrays (three
+// rows around each of the one or two cell rows the group touches), which
+// the group stages into LDS with coalesced loads before each particle walks
+// its own candidates in canonical order (rows ascending, sorted order within
+// a row).  A group whose tile would not fit (it spans more than two cell
+// rows, or the staged ranges exceed the tile) reads the same candidates from
+// global memory in the same order, so both paths give identical results.
+// No kernel writes another particle's state: no atomics on particle data.
 //
 // The cell sort is a counting sort: the producing kernel computes each
-// particle's new key and claims a slot in its cell with an atomic counter,
-// a two-kernel scan turns the counts into cell starts (and clears them for
-// the next sort), a scatter drops (id, source) pairs into the cell ranges,
-// and the reorder kernel ranks each particle by id inside its cell and
+// particle's new key and claims a slot in its cell with an atomic counter;
+// a two-kernel scan turns counts into cell starts (and clears the counts
+// for the next sort); a scatter drops (id, source) pairs into the cell
+// ranges; the reorder kernel ranks each particle by id inside its cell and
 // gathers its fields there - the canonical (cell, id) order of
 // src/hash.c:153-163.
+//
+// Workgroups are numbered XCD-aware: the hardware deals workgroups to the
+// 8 XCDs round-robin, so logical tile b is remapped such that each XCD
+// processes one contiguous eighth of the sorted array and its neighbours
+// stay in that XCD's L2.
 #include "sph_kernels.h"
 
 namespace {
 
 constexpr int kWave = 64;
+constexpr int kTPB = 256;
+constexpr int kTileCap = 1536; // staged candidates per workgroup
+constexpr unsigned kXcds = 8;
 
 __device__ __forceinline__ unsigned wave_max_u32(unsigned v)
 {
@@ -38,6 +55,25 @@ __device__ __forceinline__ unsigned wave_sum_u32(unsigned v)
 
 __device__ __forceinline__ bool lane0() { return (threadIdx.x & (kWave - 1)) == 0; }
 
+// Logical workgroup index with each XCD's share contiguous.
+__device__ __forceinline__ unsigned xcd_block()
+{
+    const unsigned b = blockIdx.x, nb = gridDim.x;
+    const unsigned xcd = b % kXcds, local = b / kXcds;
+    const unsigned per = nb / kXcds, rem = nb % kXcds;
+    return xcd * per + min(xcd, rem) + local;
+}
+
+// Raise a running maximum.  Every wave of a launch reports to the same
+// address, so the atomic is issued only when the value would change it:
+// same-address atomics serialise in L2, and the maximum settles after the
+// first few waves.
+__device__ __forceinline__ void raise_max(unsigned long long *p, unsigned long long v)
+{
+    if (v > *(volatile unsigned long long *)p)
+        atomicMax(p, v);
+}
+
 // Count-only record of a particle's neighbourhood (overflow guard for the
 // reference's 400-slot half lists, and the largest full set seen).
 __device__ __forceinline__ void report_neighbors(unsigned full, unsigned half,
@@ -47,7 +83,109 @@ __device__ __forceinline__ void report_neighbors(unsigned full, unsigned half,
         atomicAdd(&C->neighbor_overflow, 1ull);
     unsigned m = wave_max_u32(full);
     if (lane0())
-        atomicMax(&C->max_neighbors, (unsigned long long)m);
+        raise_max(&C->max_neighbors, m);
+}
+
+// ---------------------------------------------------------------------------
+// Tiles: the candidate ranges of one workgroup of sorted particles.
+struct TilePlan {
+    int ok;           // 1: staged in LDS; 0: read candidates from global memory
+    unsigned jA;      // first cell row of the group
+    unsigned g0[6];   // segment sorted begin, segment = (row - jA) * 3 + dy + 1
+    unsigned len[6];  // segment length
+    int off[6];       // LDS index = k + off[segment]
+};
+
+// Thread 0 plans the tile of sorted particles [s0, s1).
+__device__ void plan_tile(unsigned s0, unsigned s1, const unsigned *__restrict__ K,
+                          const unsigned *__restrict__ start, const sph_dev_params &P,
+                          TilePlan &T)
+{
+    const unsigned sx = P.sx, sy = P.sy;
+    const unsigned kf = K[s0], kl = K[s1 - 1];
+    const unsigned jA = kf / sx, jB = kl / sx;
+    T.jA = jA;
+    T.ok = 0;
+    for (int q = 0; q < 6; q++) {
+        T.g0[q] = 0;
+        T.len[q] = 0;
+        T.off[q] = 0;
+    }
+    if (jB > jA + 1)
+        return;
+    unsigned total = 0;
+    for (unsigned rho = 0; rho <= jB - jA; rho++) {
+        const unsigned j = jA + rho;
+        const unsigned lo = rho == 0 ? kf % sx : 0;
+        const unsigned hi = j == jB ? kl % sx : sx - 1;
+        const unsigned c0 = lo > 0 ? lo - 1 : 0;
+        const unsigned c1 = hi + 1 < sx ? hi + 1 : sx - 1;
+        for (int dy = -1; dy <= 1; dy++) {
+            const long r = (long)j + dy;
+            if (r < 0 || r >= (long)sy)
+                continue;
+            const int q = (int)rho * 3 + dy + 1;
+            const unsigned a = start[(unsigned)r * sx + c0];
+            const unsigned b = start[(unsigned)r * sx + c1 + 1];
+            T.g0[q] = a;
+            T.len[q] = b - a;
+            T.off[q] = (int)total - (int)a;
+            total += b - a;
+        }
+    }
+    T.ok = total <= (unsigned)kTileCap;
+}
+
+// Copy the planned segments into LDS (all threads), then barrier.
+template <class Rec, class Load>
+__device__ __forceinline__ void stage_tile(const TilePlan &T, Rec *tile, Load load)
+{
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        const unsigned g0 = T.g0[q], len = T.len[q];
+        const int off = T.off[q];
+        for (unsigned t = threadIdx.x; t < len; t += kTPB)
+            tile[(int)(g0 + t) + off] = load(g0 + t);
+    }
+    __syncthreads();
+}
+
+// Candidate source reading the LDS tile.
+template <class Rec>
+struct LdsSrc {
+    const Rec *tile;
+    int offA[3], offB[3];
+    bool rowB;
+    __device__ int row(int dy) const { return rowB ? offB[dy + 1] : offA[dy + 1]; }
+    __device__ Rec at(unsigned k, int off) const { return tile[(int)k + off]; }
+};
+
+template <class Rec>
+__device__ __forceinline__ LdsSrc<Rec> lds_src(const Rec *tile, const TilePlan &T, unsigned key,
+                                              unsigned sx)
+{
+    LdsSrc<Rec> r;
+    r.tile = tile;
+    for (int q = 0; q < 3; q++) {
+        r.offA[q] = T.off[q];
+        r.offB[q] = T.off[3 + q];
+    }
+    r.rowB = key / sx != T.jA;
+    return r;
+}
+
+// Candidate source reading global memory.
+template <class Rec, class Load>
+struct GlobalSrc {
+    Load load;
+    __device__ int row(int) const { return 0; }
+    __device__ Rec at(unsigned k, int) const { return load(k); }
+};
+
+template <class Rec, class Load>
+__device__ __forceinline__ GlobalSrc<Rec, Load> global_src(Load l)
+{
+    return GlobalSrc<Rec, Load>{l};
 }
 
 } // namespace
@@ -62,65 +200,73 @@ __device__ __forceinline__ void report_neighbors(unsigned full, unsigned half,
 //       reference's half lists seen from both ends).
 // PRED: x_prev = x; x += v*dt; boundary; then the new cell key and a slot
 //       in that cell (histogram of the next sort).  Without PRED the
-//       post-viscosity velocity is written to out_vx/out_vy.
+//       post-viscosity velocity is written to out_pos.
 template <bool GRAV, bool VISC, bool PRED>
-__global__ void __launch_bounds__(256)
-k_visc_predict(const sph_dev_params *__restrict__ Pp, const float *__restrict__ X,
-               const float *__restrict__ Y, const float *__restrict__ VX,
-               const float *__restrict__ VY, const unsigned *__restrict__ K,
-               const unsigned *__restrict__ start, float *__restrict__ out_x,
-               float *__restrict__ out_y, unsigned *__restrict__ out_key,
-               unsigned *__restrict__ out_slot, unsigned *__restrict__ cnt,
-               sph_dev_counters *__restrict__ C)
+__global__ void __launch_bounds__(kTPB)
+k_visc_predict(const sph_dev_params *__restrict__ Pp, const float2 *__restrict__ POS,
+               const float2 *__restrict__ VEL, const unsigned *__restrict__ K,
+               const unsigned *__restrict__ start, float2 *__restrict__ out_pos,
+               unsigned *__restrict__ out_key, unsigned *__restrict__ out_slot,
+               unsigned *__restrict__ cnt, sph_dev_counters *__restrict__ C)
 {
+    __shared__ TilePlan T;
+    __shared__ float4 tile[VISC ? kTileCap : 1];
     const sph_dev_params P = *Pp;
-    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = s < (unsigned)P.n;
+    const unsigned n = (unsigned)P.n;
+    const unsigned s0 = xcd_block() * kTPB;
+    const unsigned s1 = min(s0 + kTPB, n);
+    const unsigned s = s0 + threadIdx.x;
     const float gdt = P.g_neg * P.dt;
+    if (VISC) {
+        if (threadIdx.x == 0)
+            plan_tile(s0, s1, K, start, P, T);
+        __syncthreads();
+        if (T.ok)
+            stage_tile(T, tile, [&](unsigned g) {
+                const float2 p = POS[g], v = VEL[g];
+                return make_float4(p.x, p.y, v.x, v.y);
+            });
+    }
     unsigned full = 0, half = 0;
-    if (live) {
-        const float ax = X[s], ay = Y[s];
-        const float avx = VX[s];
-        float avy = VY[s];
-        if (GRAV)
-            avy += gdt;
+    if (s < s1) {
+        const float2 a = POS[s];
+        const float2 av = VEL[s];
+        const float avx = av.x;
+        const float avy = GRAV ? av.y + gdt : av.y;
         float nvx = avx, nvy = avy;
         if (VISC) {
-            sph_rows R;
-            sph_row_ranges(K[s], P, start, R);
-            for (int r = 0; r < R.nr; r++) {
-                for (unsigned k = R.b0[r]; k < R.b3[r]; k++) {
-                    if (k == s)
-                        continue;
-                    const float bx = X[k], by = Y[k];
-                    const float dx = bx - ax, dy = by - ay;
-                    if (dx * dx + dy * dy > P.h2)
-                        continue;
-                    full++;
-                    half += sph_owns(k, s, R.dy[r], R.b1[r], R.b2[r]);
-                    float bvy = VY[k];
-                    if (GRAV)
-                        bvy += gdt;
-                    float ix, iy;
-                    if (sph_visc_impulse(ax, ay, avx, avy, bx, by, VX[k], bvy, P, ix, iy)) {
-                        nvx -= ix * 0.5f;
-                        nvy -= iy * 0.5f;
-                    }
+            auto test = [&](unsigned k, float4 b) {
+                const float dx = b.x - a.x, ddy = b.y - a.y;
+                return (k != s) & !(dx * dx + ddy * ddy > P.h2);
+            };
+            auto body = [&](unsigned k, int dy, unsigned b1, unsigned b2, float4 b) {
+                half += sph_owns(k, s, dy, b1, b2);
+                const float bvy = GRAV ? b.w + gdt : b.w;
+                float ix, iy;
+                if (sph_visc_impulse(a.x, a.y, avx, avy, b.x, b.y, b.z, bvy, P, ix, iy)) {
+                    nvx -= ix * 0.5f;
+                    nvy -= iy * 0.5f;
                 }
-            }
+            };
+            const unsigned key = K[s];
+            if (T.ok)
+                full = sph_walk_masked(key, P, start, lds_src(tile, T, key, P.sx), test, body);
+            else
+                full = sph_walk_masked(key, P, start, global_src<float4>([&](unsigned g) {
+                                           const float2 p = POS[g], v = VEL[g];
+                                           return make_float4(p.x, p.y, v.x, v.y);
+                                       }), test, body);
         }
         if (PRED) {
-            float x = ax + nvx * P.dt;
-            float y = ay + nvy * P.dt;
+            float x = a.x + nvx * P.dt;
+            float y = a.y + nvy * P.dt;
             sph_boundary(x, y, P);
-            out_x[s] = x;
-            out_y[s] = y;
+            out_pos[s] = make_float2(x, y);
             const unsigned key = sph_cell_key(x, y, P, &C->key_out_of_range);
             out_key[s] = key;
             out_slot[s] = atomicAdd(&cnt[key], 1u);
         } else {
-            out_x[s] = nvx;
-            out_y[s] = nvy;
+            out_pos[s] = make_float2(nvx, nvy);
         }
     }
     if (VISC)
@@ -128,72 +274,66 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const float *__restrict__ 
 }
 
 // src/fluid.c:398-413 as a separate pass (phase API).
-__global__ void __launch_bounds__(256)
-k_gravity(const sph_dev_params *__restrict__ Pp, float *__restrict__ VY,
-          float *__restrict__ RHO, float *__restrict__ RHON)
+__global__ void __launch_bounds__(kTPB)
+k_gravity(const sph_dev_params *__restrict__ Pp, float2 *__restrict__ VEL,
+          float2 *__restrict__ RHO)
 {
     const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= (unsigned)Pp->n)
         return;
-    VY[s] += Pp->g_neg * Pp->dt;
-    RHO[s] = 0.0f;
-    RHON[s] = 0.0f;
+    VEL[s].y += Pp->g_neg * Pp->dt;
+    RHO[s] = make_float2(0.0f, 0.0f);
 }
 
 // src/fluid.c:507-523 as a separate, in-place pass (phase API).
-__global__ void __launch_bounds__(256)
-k_predict(const sph_dev_params *__restrict__ Pp, float *__restrict__ X, float *__restrict__ Y,
-          const float *__restrict__ VX, const float *__restrict__ VY, float *__restrict__ XP,
-          float *__restrict__ YP)
+__global__ void __launch_bounds__(kTPB)
+k_predict(const sph_dev_params *__restrict__ Pp, float2 *__restrict__ POS,
+          const float2 *__restrict__ VEL, float2 *__restrict__ PREV)
 {
     const sph_dev_params P = *Pp;
     const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= (unsigned)P.n)
         return;
-    float x = X[s], y = Y[s];
-    XP[s] = x;
-    YP[s] = y;
-    x += VX[s] * P.dt;
-    y += VY[s] * P.dt;
+    const float2 p = POS[s], v = VEL[s];
+    PREV[s] = p;
+    float x = p.x + v.x * P.dt;
+    float y = p.y + v.y * P.dt;
     sph_boundary(x, y, P);
-    X[s] = x;
-    Y[s] = y;
+    POS[s] = make_float2(x, y);
 }
 
 // src/fluid.c:642-653 as a separate, in-place pass (phase API).
-__global__ void __launch_bounds__(256)
-k_update_velocities(const sph_dev_params *__restrict__ Pp, float *__restrict__ X,
-                    float *__restrict__ Y, float *__restrict__ VX, float *__restrict__ VY,
-                    const float *__restrict__ XP, const float *__restrict__ YP)
+__global__ void __launch_bounds__(kTPB)
+k_update_velocities(const sph_dev_params *__restrict__ Pp, float2 *__restrict__ POS,
+                    float2 *__restrict__ VEL, const float2 *__restrict__ PREV)
 {
     const sph_dev_params P = *Pp;
     const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= (unsigned)P.n)
         return;
-    float x = X[s], y = Y[s];
-    sph_boundary(x, y, P);
-    float vx = (x - XP[s]) / P.dt;
-    float vy = (y - YP[s]) / P.dt;
+    float2 p = POS[s];
+    const float2 q = PREV[s];
+    sph_boundary(p.x, p.y, P);
+    float vx = (p.x - q.x) / P.dt;
+    float vy = (p.y - q.y) / P.dt;
     sph_check_velocity(vx, vy);
-    X[s] = x;
-    Y[s] = y;
-    VX[s] = vx;
-    VY[s] = vy;
+    POS[s] = p;
+    VEL[s] = make_float2(vx, vy);
 }
 
 // Keys and slots of the current positions (histogram pass of a standalone
 // hash, phase API; the fused step folds this into its producers).
-__global__ void __launch_bounds__(256)
-k_key_hist(const sph_dev_params *__restrict__ Pp, const float *__restrict__ X,
-           const float *__restrict__ Y, unsigned *__restrict__ out_key,
-           unsigned *__restrict__ out_slot, unsigned *__restrict__ cnt,
-           sph_dev_counters *__restrict__ C)
+__global__ void __launch_bounds__(kTPB)
+k_key_hist(const sph_dev_params *__restrict__ Pp, const float2 *__restrict__ POS,
+           unsigned *__restrict__ out_key, unsigned *__restrict__ out_slot,
+           unsigned *__restrict__ cnt, sph_dev_counters *__restrict__ C)
 {
     const sph_dev_params P = *Pp;
     const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= (unsigned)P.n)
         return;
-    const unsigned key = sph_cell_key(X[s], Y[s], P, &C->key_out_of_range);
+    const float2 p = POS[s];
+    const unsigned key = sph_cell_key(p.x, p.y, P, &C->key_out_of_range);
     out_key[s] = key;
     out_slot[s] = atomicAdd(&cnt[key], 1u);
 }
@@ -241,7 +381,7 @@ k_scan_reduce(const unsigned *__restrict__ cnt, unsigned m, unsigned *__restrict
         bsum[blockIdx.x] = t;
         if (o)
             atomicAdd(&C->bucket_overflow, (unsigned long long)o);
-        atomicMax(&C->max_cell_count, (unsigned long long)x);
+        raise_max(&C->max_cell_count, x);
     }
 }
 
@@ -304,12 +444,12 @@ k_scan_apply(unsigned *__restrict__ cnt, unsigned m, const unsigned *__restrict_
 }
 
 // Drop (id, source index) into the cell ranges in slot order.
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kTPB)
 k_scatter(int n, const unsigned *__restrict__ key, const unsigned *__restrict__ slot,
           const unsigned *__restrict__ start, const int *__restrict__ id_src,
           int *__restrict__ perm_src, int *__restrict__ perm_id)
 {
-    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned i = xcd_block() * kTPB + threadIdx.x;
     if (i >= (unsigned)n)
         return;
     const unsigned pos = start[key[i]] + slot[i];
@@ -321,12 +461,12 @@ k_scatter(int n, const unsigned *__restrict__ key, const unsigned *__restrict__ 
 // (cell, id) order.  The slot order within a cell is arbitrary (atomics);
 // ranking restores the reference's bucket order (ascending id).
 template <int NF>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kTPB)
 k_reorder(int n, const unsigned *__restrict__ start, const int *__restrict__ perm_src,
           const int *__restrict__ perm_id, const unsigned *__restrict__ key,
           sph_reorder_fields<NF> f, int *__restrict__ id_dst, unsigned *__restrict__ key_dst)
 {
-    const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned k = xcd_block() * kTPB + threadIdx.x;
     if (k >= (unsigned)n)
         return;
     const int i = perm_src[k];
@@ -347,40 +487,50 @@ k_reorder(int n, const unsigned *__restrict__ start, const int *__restrict__ per
 // ---------------------------------------------------------------------------
 // Density (src/hash.c:190-194, 225-229 with src/fluid.c:527-539) in gather
 // form: each particle sums (1-r/h)^2 and (1-r/h)^3 over its neighbours.
-__global__ void __launch_bounds__(256)
-k_density(const sph_dev_params *__restrict__ Pp, const float *__restrict__ X,
-          const float *__restrict__ Y, const unsigned *__restrict__ K,
-          const unsigned *__restrict__ start, float *__restrict__ RHO, float *__restrict__ RHON,
-          sph_dev_counters *__restrict__ C)
+__global__ void __launch_bounds__(kTPB)
+k_density(const sph_dev_params *__restrict__ Pp, const float2 *__restrict__ POS,
+          const unsigned *__restrict__ K, const unsigned *__restrict__ start,
+          float2 *__restrict__ RHO, sph_dev_counters *__restrict__ C)
 {
+    __shared__ TilePlan T;
+    __shared__ float2 tile[kTileCap];
     const sph_dev_params P = *Pp;
-    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned n = (unsigned)P.n;
+    const unsigned s0 = xcd_block() * kTPB;
+    const unsigned s1 = min(s0 + kTPB, n);
+    const unsigned s = s0 + threadIdx.x;
+    if (threadIdx.x == 0)
+        plan_tile(s0, s1, K, start, P, T);
+    __syncthreads();
+    if (T.ok)
+        stage_tile(T, tile, [&](unsigned g) { return POS[g]; });
     unsigned full = 0, half = 0;
-    if (s < (unsigned)P.n) {
-        const float ax = X[s], ay = Y[s];
+    if (s < s1) {
+        const float2 a = POS[s];
         float rho = 0.0f, rhon = 0.0f;
-        sph_rows R;
-        sph_row_ranges(K[s], P, start, R);
-        for (int r = 0; r < R.nr; r++) {
-            for (unsigned k = R.b0[r]; k < R.b3[r]; k++) {
-                if (k == s)
-                    continue;
-                const float dx = ax - X[k], dy = ay - Y[k];
-                const float r2 = dx * dx + dy * dy;
-                if (r2 > P.h2)
-                    continue;
-                full++;
-                half += sph_owns(k, s, R.dy[r], R.b1[r], R.b2[r]);
-                const float ratio = sqrtf(r2) * P.h_recip;
-                const float OmR2 = (1.0f - ratio) * (1.0f - ratio);
-                if (ratio < 1.0f) {
-                    rho += OmR2;
-                    rhon += OmR2 * (1.0f - ratio);
-                }
+        auto test = [&](unsigned k, float2 b) {
+            const float dx = a.x - b.x, ddy = a.y - b.y;
+            return (k != s) & !(dx * dx + ddy * ddy > P.h2);
+        };
+        auto body = [&](unsigned k, int dy, unsigned b1, unsigned b2, float2 b) {
+            half += sph_owns(k, s, dy, b1, b2);
+            const float dx = a.x - b.x, ddy = a.y - b.y;
+            const float r2 = dx * dx + ddy * ddy;
+            const float ratio = sph_sqrt(r2) * P.h_recip;
+            const float OmR2 = (1.0f - ratio) * (1.0f - ratio);
+            if (ratio < 1.0f) {
+                rho += OmR2;
+                rhon += OmR2 * (1.0f - ratio);
             }
-        }
-        RHO[s] = rho;
-        RHON[s] = rhon;
+        };
+        const unsigned key = K[s];
+        if (T.ok)
+            full = sph_walk_masked(key, P, start, lds_src(tile, T, key, P.sx), test, body);
+        else
+            full = sph_walk_masked(key, P, start,
+                                   global_src<float2>([&](unsigned g) { return POS[g]; }), test,
+                                   body);
+        RHO[s] = make_float2(rho, rhon);
     }
     report_neighbors(full, half, C);
 }
@@ -390,122 +540,129 @@ k_density(const sph_dev_params *__restrict__ Pp, const float *__restrict__ X,
 // v = (x - x_prev)/dt clamped, and the histogram of the next sort; without
 // FUSED only the relaxed positions are written.
 template <bool FUSED>
-__global__ void __launch_bounds__(256)
-k_relax(const sph_dev_params *__restrict__ Pp, const float *__restrict__ X,
-        const float *__restrict__ Y, const float *__restrict__ XP, const float *__restrict__ YP,
-        const float *__restrict__ RHO, const float *__restrict__ RHON,
+__global__ void __launch_bounds__(kTPB)
+k_relax(const sph_dev_params *__restrict__ Pp, const float2 *__restrict__ POS,
+        const float2 *__restrict__ PREV, const float2 *__restrict__ RHO,
         const unsigned *__restrict__ K, const unsigned *__restrict__ start,
-        float *__restrict__ out_x, float *__restrict__ out_y, float *__restrict__ out_vx,
-        float *__restrict__ out_vy, unsigned *__restrict__ out_key,
-        unsigned *__restrict__ out_slot, unsigned *__restrict__ cnt,
-        sph_dev_counters *__restrict__ C)
+        float2 *__restrict__ out_pos, float2 *__restrict__ out_vel,
+        unsigned *__restrict__ out_key, unsigned *__restrict__ out_slot,
+        unsigned *__restrict__ cnt, sph_dev_counters *__restrict__ C)
 {
+    __shared__ TilePlan T;
+    __shared__ float4 tile[kTileCap];
     const sph_dev_params P = *Pp;
-    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= (unsigned)P.n)
+    const unsigned n = (unsigned)P.n;
+    const unsigned s0 = xcd_block() * kTPB;
+    const unsigned s1 = min(s0 + kTPB, n);
+    const unsigned s = s0 + threadIdx.x;
+    if (threadIdx.x == 0)
+        plan_tile(s0, s1, K, start, P, T);
+    __syncthreads();
+    auto load = [&](unsigned g) {
+        const float2 p = POS[g], r = RHO[g];
+        return make_float4(p.x, p.y, r.x, r.y);
+    };
+    if (T.ok)
+        stage_tile(T, tile, load);
+    if (s >= s1)
         return;
-    const float ax = X[s], ay = Y[s];
-    const float aP = P.k * (RHO[s] - P.rest_density);
-    const float aPn = P.k_near * RHON[s];
-    float nx = ax, ny = ay;
-    sph_rows R;
-    sph_row_ranges(K[s], P, start, R);
-    for (int r = 0; r < R.nr; r++) {
-        for (unsigned k = R.b0[r]; k < R.b3[r]; k++) {
-            if (k == s)
-                continue;
-            const float bx = X[k], by = Y[k];
-            const float dx = ax - bx, dy = ay - by;
-            const float r2 = dx * dx + dy * dy;
-            if (r2 > P.h2)
-                continue;
-            const float rr = sqrtf(r2);
-            const float r_recip = 1.0f / rr;
-            const float ratio = rr * P.h_recip;
-            const float OmR = 1.0f - ratio;
-            if (rr <= 0.000001f && sph_owns(k, s, R.dy[r], R.b1[r], R.b2[r])) {
-                nx += 0.000001f;
-                ny += 0.000001f;
-            }
-            if (ratio < 1.0f && rr > 0.0f) {
-                const float bP = P.k * (RHO[k] - P.rest_density);
-                const float bPn = P.k_near * RHON[k];
-                const float D = sph_relax_D(P.dt, aP + bP, aPn + bPn, OmR, P.k_spring, P.h, rr);
-                const float Dx = D * (bx - ax) * r_recip;
-                const float Dy = D * (by - ay) * r_recip;
-                nx -= Dx;
-                ny -= Dy;
-            }
+    const float2 a = POS[s];
+    const float2 ar = RHO[s];
+    const float aP = P.k * (ar.x - P.rest_density);
+    const float aPn = P.k_near * ar.y;
+    float nx = a.x, ny = a.y;
+    auto test = [&](unsigned k, float4 b) {
+        const float dx = a.x - b.x, ddy = a.y - b.y;
+        return (k != s) & !(dx * dx + ddy * ddy > P.h2);
+    };
+    auto body = [&](unsigned k, int dy, unsigned b1, unsigned b2, float4 b) {
+        const float dx = a.x - b.x, ddy = a.y - b.y;
+        const float r2 = dx * dx + ddy * ddy;
+        const float rr = sph_sqrt(r2);
+        const float r_recip = 1.0f / rr;
+        const float ratio = rr * P.h_recip;
+        const float OmR = 1.0f - ratio;
+        if (rr <= 0.000001f && sph_owns(k, s, dy, b1, b2)) {
+            nx += 0.000001f;
+            ny += 0.000001f;
         }
-    }
+        if (ratio < 1.0f && rr > 0.0f) {
+            const float bP = P.k * (b.z - P.rest_density);
+            const float bPn = P.k_near * b.w;
+            const float D = sph_relax_D(P.dt, aP + bP, aPn + bPn, OmR, P.k_spring, P.h, rr);
+            const float Dx = D * (b.x - a.x) * r_recip;
+            const float Dy = D * (b.y - a.y) * r_recip;
+            nx -= Dx;
+            ny -= Dy;
+        }
+    };
+    const unsigned key = K[s];
+    if (T.ok)
+        sph_walk_masked(key, P, start, lds_src(tile, T, key, P.sx), test, body);
+    else
+        sph_walk_masked(key, P, start, global_src<float4>(load), test, body);
     if (FUSED) {
         sph_boundary(nx, ny, P);
-        float vx = (nx - XP[s]) / P.dt;
-        float vy = (ny - YP[s]) / P.dt;
+        const float2 q = PREV[s];
+        float vx = (nx - q.x) / P.dt;
+        float vy = (ny - q.y) / P.dt;
         sph_check_velocity(vx, vy);
-        out_vx[s] = vx;
-        out_vy[s] = vy;
+        out_vel[s] = make_float2(vx, vy);
         if (!(isfinite(nx) && isfinite(ny)))
             atomicAdd(&C->nonfinite, 1ull);
-        const unsigned key = sph_cell_key(nx, ny, P, &C->key_out_of_range);
-        out_key[s] = key;
-        out_slot[s] = atomicAdd(&cnt[key], 1u);
+        const unsigned nk = sph_cell_key(nx, ny, P, &C->key_out_of_range);
+        out_key[s] = nk;
+        out_slot[s] = atomicAdd(&cnt[nk], 1u);
     }
-    out_x[s] = nx;
-    out_y[s] = ny;
+    out_pos[s] = make_float2(nx, ny);
+}
+
+// Reference coordinate packing (src/fluid.c:356-363), written by id.
+__global__ void __launch_bounds__(kTPB)
+k_pack_coords(int n, const float2 *__restrict__ POS, const int *__restrict__ ID, float max_x,
+              float max_y, short *__restrict__ out)
+{
+    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= (unsigned)n)
+        return;
+    const int id = ID[s];
+    const float2 p = POS[s];
+    out[2 * id] = (short)((2.0f * p.x / max_x - 1.0f) * 32767.0f);
+    out[2 * id + 1] = (short)((2.0f * p.y / max_y - 1.0f) * 32767.0f);
 }
 
 // Neighbour sets of the current hash, written by id (small-N read-back for
 // tests): full set in canonical order, and the half-list length the
 // reference would hold for each particle.
-__global__ void __launch_bounds__(256)
-k_neighbor_lists(const sph_dev_params *__restrict__ Pp, const float *__restrict__ X,
-                 const float *__restrict__ Y, const unsigned *__restrict__ K,
-                 const int *__restrict__ ID, const unsigned *__restrict__ start, int stride,
-                 int *__restrict__ full_by_id, int *__restrict__ half_by_id,
-                 int *__restrict__ ids_by_id)
+__global__ void __launch_bounds__(kTPB)
+k_neighbor_lists(const sph_dev_params *__restrict__ Pp, const float2 *__restrict__ POS,
+                 const unsigned *__restrict__ K, const int *__restrict__ ID,
+                 const unsigned *__restrict__ start, int stride, int *__restrict__ full_by_id,
+                 int *__restrict__ half_by_id, int *__restrict__ ids_by_id)
 {
     const sph_dev_params P = *Pp;
     const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= (unsigned)P.n)
         return;
     const int me = ID[s];
-    const float ax = X[s], ay = Y[s];
+    const float2 a = POS[s];
     int full = 0, half = 0;
-    sph_rows R;
-    sph_row_ranges(K[s], P, start, R);
-    for (int r = 0; r < R.nr; r++)
-        for (unsigned k = R.b0[r]; k < R.b3[r]; k++) {
-            if (k == s)
-                continue;
-            const float dx = ax - X[k], dy = ay - Y[k];
-            if (dx * dx + dy * dy > P.h2)
-                continue;
-            if (full < stride)
-                ids_by_id[(size_t)me * stride + full] = ID[k];
-            full++;
-            half += sph_owns(k, s, R.dy[r], R.b1[r], R.b2[r]);
-        }
+    sph_walk(s, K[s], P, start, global_src<float2>([&](unsigned g) { return POS[g]; }),
+             [&](unsigned k, int dy, unsigned b1, unsigned b2, float2 b) {
+                 const float dx = a.x - b.x, ddy = a.y - b.y;
+                 if (k == s || dx * dx + ddy * ddy > P.h2)
+                     return;
+                 if (full < stride)
+                     ids_by_id[(size_t)me * stride + full] = ID[k];
+                 full++;
+                 half += sph_owns(k, s, dy, b1, b2);
+             });
     full_by_id[me] = full;
     half_by_id[me] = half;
 }
 
-// Reference coordinate packing (src/fluid.c:356-363), written by id.
-__global__ void __launch_bounds__(256)
-k_pack_coords(int n, const float *__restrict__ X, const float *__restrict__ Y,
-              const int *__restrict__ ID, float max_x, float max_y, short *__restrict__ out)
-{
-    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= (unsigned)n)
-        return;
-    const int id = ID[s];
-    out[2 * id] = (short)((2.0f * X[s] / max_x - 1.0f) * 32767.0f);
-    out[2 * id + 1] = (short)((2.0f * Y[s] / max_y - 1.0f) * 32767.0f);
-}
-
-// Scatter a field from one sorted order to another through ids:
-// dst[pos_of_id[id_src[k]]] = src[k] (re-canonicalises auxiliary fields).
-__global__ void __launch_bounds__(256)
+// dst[pos_of_id[id_src[k]]] = src[k]: re-canonicalise auxiliary fields.
+__global__ void __launch_bounds__(kTPB)
 k_pos_of_id(int n, const int *__restrict__ ID, int *__restrict__ pos_of_id)
 {
     const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -513,33 +670,32 @@ k_pos_of_id(int n, const int *__restrict__ ID, int *__restrict__ pos_of_id)
         pos_of_id[ID[s]] = (int)s;
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kTPB)
 k_permute_by_id(int n, const int *__restrict__ id_src, const int *__restrict__ pos_of_id,
-                sph_reorder_fields<4> f)
+                sph_reorder_fields<2> f)
 {
     const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (unsigned)n)
         return;
     const int d = pos_of_id[id_src[k]];
 #pragma unroll
-    for (int q = 0; q < 4; q++)
+    for (int q = 0; q < 2; q++)
         f.dst[q][d] = f.src[q][k];
 }
 
 // ---------------------------------------------------------------------------
-// Host-side launchers (C linkage inside the library).
+// Host-side launchers.
 
-#define GRID(n, b) dim3((unsigned)(((n) + (b)-1) / (b)))
+#define GRID(n) dim3((unsigned)(((n) + kTPB - 1) / kTPB))
 
 void sph_launch_visc_predict(hipStream_t st, int n, bool grav, bool visc, bool pred,
-                             const sph_dev_params *P, const float *X, const float *Y,
-                             const float *VX, const float *VY, const unsigned *K,
-                             const unsigned *start, float *ox, float *oy, unsigned *okey,
-                             unsigned *oslot, unsigned *cnt, sph_dev_counters *C)
+                             const sph_dev_params *P, const float2 *POS, const float2 *VEL,
+                             const unsigned *K, const unsigned *start, float2 *opos,
+                             unsigned *okey, unsigned *oslot, unsigned *cnt, sph_dev_counters *C)
 {
 #define VP(G, V, R)                                                                              \
-    k_visc_predict<G, V, R><<<GRID(n, 256), 256, 0, st>>>(P, X, Y, VX, VY, K, start, ox, oy,    \
-                                                          okey, oslot, cnt, C)
+    k_visc_predict<G, V, R><<<GRID(n), kTPB, 0, st>>>(P, POS, VEL, K, start, opos, okey, oslot,  \
+                                                      cnt, C)
     if (grav && visc && pred)
         VP(true, true, true);
     else if (grav && !visc && pred)
@@ -555,30 +711,27 @@ void sph_launch_visc_predict(hipStream_t st, int n, bool grav, bool visc, bool p
 #undef VP
 }
 
-void sph_launch_gravity(hipStream_t st, int n, const sph_dev_params *P, float *VY, float *RHO,
-                        float *RHON)
+void sph_launch_gravity(hipStream_t st, int n, const sph_dev_params *P, float2 *VEL, float2 *RHO)
 {
-    k_gravity<<<GRID(n, 256), 256, 0, st>>>(P, VY, RHO, RHON);
+    k_gravity<<<GRID(n), kTPB, 0, st>>>(P, VEL, RHO);
 }
 
-void sph_launch_predict(hipStream_t st, int n, const sph_dev_params *P, float *X, float *Y,
-                        const float *VX, const float *VY, float *XP, float *YP)
+void sph_launch_predict(hipStream_t st, int n, const sph_dev_params *P, float2 *POS,
+                        const float2 *VEL, float2 *PREV)
 {
-    k_predict<<<GRID(n, 256), 256, 0, st>>>(P, X, Y, VX, VY, XP, YP);
+    k_predict<<<GRID(n), kTPB, 0, st>>>(P, POS, VEL, PREV);
 }
 
-void sph_launch_update_velocities(hipStream_t st, int n, const sph_dev_params *P, float *X,
-                                  float *Y, float *VX, float *VY, const float *XP,
-                                  const float *YP)
+void sph_launch_update_velocities(hipStream_t st, int n, const sph_dev_params *P, float2 *POS,
+                                  float2 *VEL, const float2 *PREV)
 {
-    k_update_velocities<<<GRID(n, 256), 256, 0, st>>>(P, X, Y, VX, VY, XP, YP);
+    k_update_velocities<<<GRID(n), kTPB, 0, st>>>(P, POS, VEL, PREV);
 }
 
-void sph_launch_key_hist(hipStream_t st, int n, const sph_dev_params *P, const float *X,
-                         const float *Y, unsigned *okey, unsigned *oslot, unsigned *cnt,
-                         sph_dev_counters *C)
+void sph_launch_key_hist(hipStream_t st, int n, const sph_dev_params *P, const float2 *POS,
+                         unsigned *okey, unsigned *oslot, unsigned *cnt, sph_dev_counters *C)
 {
-    k_key_hist<<<GRID(n, 256), 256, 0, st>>>(P, X, Y, okey, oslot, cnt, C);
+    k_key_hist<<<GRID(n), kTPB, 0, st>>>(P, POS, okey, oslot, cnt, C);
 }
 
 unsigned sph_scan_blocks(unsigned m) { return (m + kScanTile - 1) / kScanTile; }
@@ -598,68 +751,64 @@ void sph_launch_scan_apply(hipStream_t st, unsigned *cnt, unsigned m, const unsi
 void sph_launch_scatter(hipStream_t st, int n, const unsigned *key, const unsigned *slot,
                         const unsigned *start, const int *id_src, int *perm_src, int *perm_id)
 {
-    k_scatter<<<GRID(n, 256), 256, 0, st>>>(n, key, slot, start, id_src, perm_src, perm_id);
+    k_scatter<<<GRID(n), kTPB, 0, st>>>(n, key, slot, start, id_src, perm_src, perm_id);
+}
+
+void sph_launch_reorder2(hipStream_t st, int n, const unsigned *start, const int *perm_src,
+                         const int *perm_id, const unsigned *key, sph_reorder_fields<2> f,
+                         int *id_dst, unsigned *key_dst)
+{
+    k_reorder<2><<<GRID(n), kTPB, 0, st>>>(n, start, perm_src, perm_id, key, f, id_dst, key_dst);
 }
 
 void sph_launch_reorder4(hipStream_t st, int n, const unsigned *start, const int *perm_src,
                          const int *perm_id, const unsigned *key, sph_reorder_fields<4> f,
                          int *id_dst, unsigned *key_dst)
 {
-    k_reorder<4><<<GRID(n, 256), 256, 0, st>>>(n, start, perm_src, perm_id, key, f, id_dst,
-                                               key_dst);
+    k_reorder<4><<<GRID(n), kTPB, 0, st>>>(n, start, perm_src, perm_id, key, f, id_dst, key_dst);
 }
 
-void sph_launch_reorder8(hipStream_t st, int n, const unsigned *start, const int *perm_src,
-                         const int *perm_id, const unsigned *key, sph_reorder_fields<8> f,
-                         int *id_dst, unsigned *key_dst)
+void sph_launch_density(hipStream_t st, int n, const sph_dev_params *P, const float2 *POS,
+                        const unsigned *K, const unsigned *start, float2 *RHO,
+                        sph_dev_counters *C)
 {
-    k_reorder<8><<<GRID(n, 256), 256, 0, st>>>(n, start, perm_src, perm_id, key, f, id_dst,
-                                               key_dst);
+    k_density<<<GRID(n), kTPB, 0, st>>>(P, POS, K, start, RHO, C);
 }
 
-void sph_launch_density(hipStream_t st, int n, const sph_dev_params *P, const float *X,
-                        const float *Y, const unsigned *K, const unsigned *start, float *RHO,
-                        float *RHON, sph_dev_counters *C)
-{
-    k_density<<<GRID(n, 256), 256, 0, st>>>(P, X, Y, K, start, RHO, RHON, C);
-}
-
-void sph_launch_relax(hipStream_t st, int n, bool fused, const sph_dev_params *P, const float *X,
-                      const float *Y, const float *XP, const float *YP, const float *RHO,
-                      const float *RHON, const unsigned *K, const unsigned *start, float *ox,
-                      float *oy, float *ovx, float *ovy, unsigned *okey, unsigned *oslot,
-                      unsigned *cnt, sph_dev_counters *C)
+void sph_launch_relax(hipStream_t st, int n, bool fused, const sph_dev_params *P,
+                      const float2 *POS, const float2 *PREV, const float2 *RHO, const unsigned *K,
+                      const unsigned *start, float2 *opos, float2 *ovel, unsigned *okey,
+                      unsigned *oslot, unsigned *cnt, sph_dev_counters *C)
 {
     if (fused)
-        k_relax<true><<<GRID(n, 256), 256, 0, st>>>(P, X, Y, XP, YP, RHO, RHON, K, start, ox, oy,
-                                                    ovx, ovy, okey, oslot, cnt, C);
+        k_relax<true><<<GRID(n), kTPB, 0, st>>>(P, POS, PREV, RHO, K, start, opos, ovel, okey,
+                                                oslot, cnt, C);
     else
-        k_relax<false><<<GRID(n, 256), 256, 0, st>>>(P, X, Y, XP, YP, RHO, RHON, K, start, ox,
-                                                     oy, ovx, ovy, okey, oslot, cnt, C);
+        k_relax<false><<<GRID(n), kTPB, 0, st>>>(P, POS, PREV, RHO, K, start, opos, ovel, okey,
+                                                 oslot, cnt, C);
 }
 
-void sph_launch_pack_coords(hipStream_t st, int n, const float *X, const float *Y, const int *ID,
-                            float max_x, float max_y, short *out)
+void sph_launch_pack_coords(hipStream_t st, int n, const float2 *POS, const int *ID, float max_x,
+                            float max_y, short *out)
 {
-    k_pack_coords<<<GRID(n, 256), 256, 0, st>>>(n, X, Y, ID, max_x, max_y, out);
+    k_pack_coords<<<GRID(n), kTPB, 0, st>>>(n, POS, ID, max_x, max_y, out);
 }
 
-void sph_launch_neighbor_lists(hipStream_t st, int n, const sph_dev_params *P, const float *X,
-                               const float *Y, const unsigned *K, const int *ID,
-                               const unsigned *start, int stride, int *full_by_id,
-                               int *half_by_id, int *ids_by_id)
+void sph_launch_neighbor_lists(hipStream_t st, int n, const sph_dev_params *P, const float2 *POS,
+                               const unsigned *K, const int *ID, const unsigned *start,
+                               int stride, int *full_by_id, int *half_by_id, int *ids_by_id)
 {
-    k_neighbor_lists<<<GRID(n, 256), 256, 0, st>>>(P, X, Y, K, ID, start, stride, full_by_id,
-                                                    half_by_id, ids_by_id);
+    k_neighbor_lists<<<GRID(n), kTPB, 0, st>>>(P, POS, K, ID, start, stride, full_by_id,
+                                               half_by_id, ids_by_id);
 }
 
 void sph_launch_pos_of_id(hipStream_t st, int n, const int *ID, int *pos_of_id)
 {
-    k_pos_of_id<<<GRID(n, 256), 256, 0, st>>>(n, ID, pos_of_id);
+    k_pos_of_id<<<GRID(n), kTPB, 0, st>>>(n, ID, pos_of_id);
 }
 
 void sph_launch_permute_by_id(hipStream_t st, int n, const int *id_src, const int *pos_of_id,
-                              sph_reorder_fields<4> f)
+                              sph_reorder_fields<2> f)
 {
-    k_permute_by_id<<<GRID(n, 256), 256, 0, st>>>(n, id_src, pos_of_id, f);
+    k_permute_by_id<<<GRID(n), kTPB, 0, st>>>(n, id_src, pos_of_id, f);
 }

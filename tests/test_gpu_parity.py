@@ -212,3 +212,24 @@ def test_full_size_c2_bitexact_and_sorted():
     assert np.array_equal(np.sort(ids), np.arange(len(parts)))
     c = s.counters()
     assert c["bucket_overflow"] == 0 and c["neighbor_overflow"] == 0 and c["nonfinite"] == 0
+
+
+def test_reference_loop_through_shims_matches_step(tmp_path):
+    """The C driver runs the reference's loop body (src/fluid.c:273-348)
+    through the reference-named entry points; its result equals the fused,
+    graph-replayed sph_step bit for bit."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(sa.LIB_PATH), "sph_main")
+    out = tmp_path / "out.bin"
+    r = subprocess.run([exe, "32", "24", "40", str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.fromfile(out, sa.PARTICLE)
+    tun = dam_break_tunables()
+    box = po.make_box(30, 15)
+    parts = sa.scene_lattice(32, 24, 0.5, 0.0, 0)
+    s = sa.Solver(sa.Tunables.from_buffer_copy(bytes(tun)), sa.AABB.from_buffer_copy(bytes(box)),
+                  len(parts))
+    s.upload(parts)
+    s.step(40)
+    assert_bitexact(got, s.download(), what="shim loop vs sph_step")
