@@ -25,6 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import sph_amd as sa  # noqa: E402
+from sph_amd import scenes as sc  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
@@ -80,6 +81,25 @@ def cpu_baseline(tun, box, parts, budget_s=15.0):
                       f"{dt:.1f} s on one host core ({os.cpu_count()} visible)"}
 
 
+def batched(steps, warmup):
+    """The batched configuration: 256 scenes (2k-64k particles each, four
+    parameter presets) stepped by the same launches through one graph."""
+    scenes = sc.batched_scenes()
+    parts, counts = sc.concat(scenes)
+    s = sa.Solver.batch([(t, b, len(p)) for t, b, p, _ in scenes])
+    s.upload_batch(parts, counts)
+    s.step(warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    el = time.perf_counter() - t0
+    n = int(counts.sum())
+    return {"value": n * steps / el, "unit": "particle-steps/s", "ms_per_step": el * 1e3 / steps,
+            "steps": steps, "warmup": warmup, "scenes": len(scenes), "particles": n,
+            "counters": s.counters()}
+
+
 def calibration_copy(mbytes):
     """A device-to-device copy of known size through the HIP runtime (a blit
     kernel), run after the timed region: its FETCH_SIZE / WRITE_SIZE counter
@@ -104,6 +124,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched workload line")
     ap.add_argument("--calib-copy-mb", type=float, default=0.0,
                     help="also run a device copy of this many MB (profiler calibration)")
     args = ap.parse_args()
@@ -125,7 +146,7 @@ def main():
     value = n * args.steps / el
     counters = s.counters()
 
-    # per-kernel HIP-event timing on the solver's stream (eager launches)
+    # per-kernel HIP-event timing on the solver's stream
     kt = s.time_kernels(10)
     sx, sy = s.cells()[2]
     c = sx * sy / n
@@ -164,6 +185,8 @@ def main():
         "kernels": kernel_bw,
         "counters": counters,
     }
+    if not args.no_batched:
+        out["batched"] = batched(max(10, args.steps // 2), max(2, args.warmup // 2))
     if args.calib_copy_mb > 0:
         calibration_copy(args.calib_copy_mb)
     if not args.no_cpu:

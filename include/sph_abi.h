@@ -194,6 +194,28 @@ typedef struct {
 int sph_create(sph_ctx **ctx, const param *params, const AABB_t *boundary, int capacity);
 void sph_destroy(sph_ctx *ctx);
 
+/* Several independent scenes stepped together by the same launches (the
+ * batched configuration).  Each scene has its own box, tunables and
+ * particle capacity; scenes never interact.  sph_create is the one-scene
+ * case.  Scenes are numbered 0..nscenes-1 in the order given. */
+typedef struct {
+    tunable_parameters tunables;
+    AABB_t boundary; /* must start at the origin */
+    int capacity;
+} sph_scene_desc;
+
+int sph_create_batch(sph_ctx **ctx, const sph_scene_desc *scenes, int nscenes);
+int sph_num_scenes(const sph_ctx *ctx);
+/* Particles of scene s, grid size of its box. */
+int sph_scene_info(const sph_ctx *ctx, int scene, int *n, unsigned *size_x, unsigned *size_y);
+/* Upload all scenes: particles holds counts[0] records of scene 0, then
+ * counts[1] of scene 1, ...; ids are local to each scene (a permutation of
+ * 0..counts[s]-1).  sph_download writes back in the same concatenated
+ * layout (scene s's particle id at offset sum(counts[0..s)) + id). */
+int sph_upload_batch(sph_ctx *ctx, const fluid_particle *particles, const int *counts);
+/* Tunables of one scene (sph_set_tunables sets every scene). */
+int sph_set_scene_tunables(sph_ctx *ctx, int scene, const tunable_parameters *tunables);
+
 /* Upload n particles (AoS, reference layout).  Particle ids must be a
  * permutation of 0..n-1; id k is the particle's pointer-array index in the
  * reference.  Marks the neighbour lists invalid, as a fresh reference run
@@ -255,10 +277,11 @@ int sph_read_sorted(sph_ctx *ctx, int field, void *out);
 int sph_read_cells(sph_ctx *ctx, unsigned int *start, unsigned int *count);
 int sph_num_cells(const sph_ctx *ctx, unsigned int *size_x, unsigned int *size_y);
 
-/* Per-kernel timing: run nsteps full steps launching kernel by kernel with
- * a HIP event pair around every launch on the context stream, and return
- * the mean duration (ms) of each kernel kind.  names/ms have room for
- * max_kinds entries; *nkinds receives the count. */
+/* Per-kernel timing: enqueue nsteps full steps kernel by kernel, with
+ * pre-created HIP events recorded on the context stream before every kernel
+ * and after each step's last one, then synchronise once and return the mean
+ * duration (ms) of each kernel kind.  The steps advance the simulation.
+ * names/ms have room for max_kinds entries; *nkinds receives the count. */
 int sph_time_kernels(sph_ctx *ctx, int nsteps, int max_kinds, const char **names,
                      float *ms, int *nkinds);
 

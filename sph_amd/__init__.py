@@ -17,11 +17,12 @@ import os
 
 import numpy as np
 
-__all__ = ["PARTICLE", "Tunables", "Param", "AABB", "Counters", "Solver", "SphError",
+__all__ = ["PARTICLE", "Tunables", "Param", "AABB", "SceneDesc", "Counters", "Solver", "SphError",
            "make_tunables", "make_box", "scene_lattice", "lib", "LIB_PATH"]
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG_DIR, "libsph_hip.so")
+# SPH_AMD_LIB may name another build of the library (e.g. a tuning variant).
+LIB_PATH = os.environ.get("SPH_AMD_LIB") or os.path.join(PKG_DIR, "libsph_hip.so")
 
 _FIELDS = ("x_prev", "y_prev", "x", "y", "v_x", "v_y", "a_x", "a_y",
            "density", "density_near", "pressure", "pressure_near")
@@ -54,6 +55,11 @@ class AABB(ctypes.Structure):
     _fields_ = [(n, ctypes.c_float) for n in ("min_x", "max_x", "min_y", "max_y", "min_z", "max_z")]
 
 
+class SceneDesc(ctypes.Structure):
+    """sph_scene_desc: one scene of a batched context."""
+    _fields_ = [("tunables", Tunables), ("boundary", AABB), ("capacity", ctypes.c_int)]
+
+
 class Counters(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in (
         "steps", "hashes", "bucket_overflow", "neighbor_overflow", "key_out_of_range",
@@ -84,6 +90,12 @@ def lib():
         sigs = {
             "sph_create": (ip, [ctypes.POINTER(vp), ctypes.POINTER(Param), ctypes.POINTER(AABB), ip]),
             "sph_destroy": (None, [vp]),
+            "sph_create_batch": (ip, [ctypes.POINTER(vp), ctypes.POINTER(SceneDesc), ip]),
+            "sph_num_scenes": (ip, [vp]),
+            "sph_scene_info": (ip, [vp, ip, ctypes.POINTER(ip), ctypes.POINTER(ctypes.c_uint),
+                                    ctypes.POINTER(ctypes.c_uint)]),
+            "sph_upload_batch": (ip, [vp, vp, vp]),
+            "sph_set_scene_tunables": (ip, [vp, ip, ctypes.POINTER(Tunables)]),
             "sph_upload": (ip, [vp, vp, ip]),
             "sph_set_tunables": (ip, [vp, ctypes.POINTER(Tunables)]),
             "sph_step": (ip, [vp, ip]),
@@ -160,6 +172,33 @@ class Solver:
         self.tunables = tunables
         _check(lib().sph_create(ctypes.byref(self._h), ctypes.byref(p), ctypes.byref(box), capacity),
                "sph_create")
+
+    @classmethod
+    def batch(cls, scenes):
+        """A batched context: scenes = [(tunables, box, capacity), ...]."""
+        self = cls.__new__(cls)
+        self._h = ctypes.c_void_p()
+        arr = (SceneDesc * len(scenes))()
+        for i, (t, b, cap) in enumerate(scenes):
+            arr[i].tunables = t
+            arr[i].boundary = b
+            arr[i].capacity = int(cap)
+        self.box = None
+        self.tunables = None
+        _check(lib().sph_create_batch(ctypes.byref(self._h), arr, len(scenes)), "sph_create_batch")
+        return self
+
+    @property
+    def nscenes(self):
+        return lib().sph_num_scenes(self._h)
+
+    def upload_batch(self, particles: np.ndarray, counts):
+        p = np.ascontiguousarray(particles, dtype=PARTICLE)
+        c = np.ascontiguousarray(counts, dtype=np.int32)
+        _check(lib().sph_upload_batch(self._h, p.ctypes.data, c.ctypes.data), "sph_upload_batch")
+
+    def set_scene_tunables(self, scene: int, t: Tunables):
+        _check(lib().sph_set_scene_tunables(self._h, scene, ctypes.byref(t)), "sph_set_scene_tunables")
 
     def close(self):
         if self._h:

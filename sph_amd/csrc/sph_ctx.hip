@@ -1,18 +1,22 @@
 // sph_ctx.hip - host side of libsph_hip.so: the device-resident context
 // behind the C ABI of include/sph_abi.h.
 //
-// One context = one scene on one GPU: a SoA particle array sorted by
-// (cell, id), the cell table of the last hash, parameter and counter
-// blocks, and a HIP stream.  sph_step replays a captured HIP graph of the
-// fused step; the phase calls run the reference's functions one by one.
+// One context = one or more independent scenes on one GPU, stepped
+// together.  All scenes share one SoA particle array sorted by
+// (global cell, id): scene s owns slots [poff_s, poff_s + n_s) (its range
+// padded to whole 256-thread workgroups) and global cells
+// [cell_base_s, cell_base_s + ncells_s) plus one padding cell.  sph_step
+// replays a captured HIP graph of the fused step; the phase calls run the
+// reference's functions one by one.
 #include <hip/hip_runtime.h>
 
+#include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
-#include <string>
+#include <new>
 #include <vector>
 
 #include "sph_kernels.h"
@@ -45,17 +49,26 @@ const char *kKindNames[NKIND] = {"visc_predict", "scan_reduce", "scan_apply", "s
 
 constexpr int kGraphSteps = 16; // steps per replay of the long graph
 
+struct Scene {
+    tunable_parameters tun{};
+    AABB_t box{};
+    int cap = 0, n = 0;
+    unsigned sx = 0, sy = 0, ncells = 0;
+    unsigned cell_base = 0, poff = 0, slots = 0, uoff = 0;
+};
+
 } // namespace
 
 struct sph_ctx {
     int device = 0;
     hipStream_t st = nullptr;
-    int cap = 0, n = 0;
-    unsigned sx = 0, sy = 0, ncells = 0;
-    tunable_parameters tun{};
-    AABB_t box{};
-    sph_dev_params hp{};
+    std::vector<Scene> sc;
+    unsigned slots = 0;       // padded slots of all scenes
+    unsigned cells = 0;       // global cells incl. one padding cell per scene
+    int n_total = 0;
+    std::vector<sph_dev_params> hp;
     sph_dev_params *dP = nullptr;
+    int *BS = nullptr; // workgroup -> scene
     sph_dev_counters *dC = nullptr;
     // state (canonical: sorted by (cell, id); ID0/K0 give id and key);
     // pairs interleaved: POS=(x,y) VEL=(v_x,v_y) PREV=(x_prev,y_prev)
@@ -72,49 +85,58 @@ struct sph_ctx {
     bool lists_valid = false; // a hash ran since upload: K0/start describe POS0
     int aux_order = 0;        // PREV, RHO ordered like ID0 (0) or ID1 (1)
     hipGraphExec_t g1 = nullptr, gK = nullptr;
-    int graph_n = -1;
     uint64_t steps = 0, hashes = 0;
-    // per-kernel timing (only while sph_time_kernels runs)
+    // per-kernel timing (only while sph_time_kernels enqueues)
     std::vector<hipEvent_t> *tev = nullptr;
     std::vector<int> *tkind = nullptr;
+    size_t tidx = 0;
 };
 
 namespace {
 
 void derive_params(sph_ctx *c)
 {
-    const tunable_parameters &t = c->tun;
-    sph_dev_params &p = c->hp;
-    p.h = t.smoothing_radius;
-    p.h_recip = 1.0f / t.smoothing_radius;
-    p.h2 = t.smoothing_radius * t.smoothing_radius;
-    p.dt = t.time_step;
-    p.g_neg = -t.g;
-    p.rest_density = t.rest_density;
-    p.k = t.k;
-    p.k_near = t.k_near;
-    p.k_spring = t.k_spring;
-    p.sigma = t.sigma;
-    p.beta = t.beta;
-    p.min_x = c->box.min_x;
-    p.max_x = c->box.max_x;
-    p.min_y = c->box.min_y;
-    p.max_y = c->box.max_y;
-    p.mover_cx = t.mover_center_x;
-    p.mover_cy = t.mover_center_y;
-    p.mover_w = t.mover_width;
-    p.mover_h = t.mover_height;
-    p.mover_type = t.mover_type;
-    p.sx = c->sx;
-    p.sy = c->sy;
-    p.ncells = c->ncells;
-    p.n = c->n;
+    c->hp.resize(c->sc.size());
+    for (size_t s = 0; s < c->sc.size(); s++) {
+        const Scene &S = c->sc[s];
+        const tunable_parameters &t = S.tun;
+        sph_dev_params &p = c->hp[s];
+        p.h = t.smoothing_radius;
+        p.h_recip = 1.0f / t.smoothing_radius;
+        p.h2 = t.smoothing_radius * t.smoothing_radius;
+        p.dt = t.time_step;
+        p.g_neg = -t.g;
+        p.rest_density = t.rest_density;
+        p.k = t.k;
+        p.k_near = t.k_near;
+        p.k_spring = t.k_spring;
+        p.sigma = t.sigma;
+        p.beta = t.beta;
+        p.min_x = S.box.min_x;
+        p.max_x = S.box.max_x;
+        p.min_y = S.box.min_y;
+        p.max_y = S.box.max_y;
+        p.mover_cx = t.mover_center_x;
+        p.mover_cy = t.mover_center_y;
+        p.mover_w = t.mover_width;
+        p.mover_h = t.mover_height;
+        p.mover_type = t.mover_type;
+        p.sx = S.sx;
+        p.sy = S.sy;
+        p.ncells = S.ncells;
+        p.cell_base = S.cell_base;
+        p.poff = S.poff;
+        p.uoff = S.uoff;
+        p.n = S.n;
+        p.pad = S.slots - (unsigned)S.n;
+    }
 }
 
 int push_params(sph_ctx *c)
 {
     derive_params(c);
-    HIPCHK(hipMemcpy(c->dP, &c->hp, sizeof c->hp, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->dP, c->hp.data(), sizeof(sph_dev_params) * c->hp.size(),
+                     hipMemcpyHostToDevice));
     return SPH_OK;
 }
 
@@ -125,39 +147,27 @@ void free_graphs(sph_ctx *c)
     if (c->gK)
         (void)hipGraphExecDestroy(c->gK);
     c->g1 = c->gK = nullptr;
-    c->graph_n = -1;
 }
 
-// Timing hooks around each launch (no-ops unless sph_time_kernels runs).
+// Timing hook before each launch (a no-op unless sph_time_kernels is
+// enqueueing): records the next boundary event, so kernel i runs between
+// boundary events i and i+1.
 struct Tick {
-    sph_ctx *c;
-    Tick(sph_ctx *c_, int kind) : c(c_)
+    Tick(sph_ctx *c, int kind)
     {
         if (c->tev) {
-            hipEvent_t e;
-            (void)hipEventCreate(&e);
-            (void)hipEventRecord(e, c->st);
-            c->tev->push_back(e);
+            (void)hipEventRecord((*c->tev)[c->tidx++], c->st);
             c->tkind->push_back(kind);
-        }
-    }
-    ~Tick()
-    {
-        if (c->tev) {
-            hipEvent_t e;
-            (void)hipEventCreate(&e);
-            (void)hipEventRecord(e, c->st);
-            c->tev->push_back(e);
         }
     }
 };
 
 void enqueue_sort_tail(sph_ctx *c, const int *id_src)
 {
-    const unsigned m = c->ncells + 1;
+    const unsigned m = c->cells + 1;
     {
         Tick t(c, KSR);
-        sph_launch_scan_reduce(c->st, c->cnt, m, c->bsum, c->dC);
+        sph_launch_scan_reduce(c->st, c->cnt, m, c->bsum);
     }
     {
         Tick t(c, KSA);
@@ -165,8 +175,8 @@ void enqueue_sort_tail(sph_ctx *c, const int *id_src)
     }
     {
         Tick t(c, KSC);
-        sph_launch_scatter(c->st, c->n, c->ukey, c->uslot, c->start, id_src, c->perm_src,
-                           c->perm_id);
+        sph_launch_scatter(c->st, c->slots, c->dP, c->BS, c->ukey, c->uslot, c->start, id_src,
+                           c->perm_src, c->perm_id);
     }
 }
 
@@ -178,34 +188,34 @@ void enqueue_sort_tail(sph_ctx *c, const int *id_src)
 //   scan, scan, scatter, reorder   -> sorted relaxed state (POS0, VEL)
 void enqueue_fused_step(sph_ctx *c, bool visc)
 {
-    const int n = c->n;
+    const unsigned S = c->slots;
     {
         Tick t(c, KV);
-        sph_launch_visc_predict(c->st, n, true, visc, true, c->dP, c->POS0, c->VEL, c->K0,
+        sph_launch_visc_predict(c->st, S, true, visc, true, c->dP, c->BS, c->POS0, c->VEL, c->K0,
                                 c->start, c->UPOS, c->ukey, c->uslot, c->cnt, c->dC);
     }
     enqueue_sort_tail(c, c->ID0);
     {
         Tick t(c, KRO);
         sph_reorder_fields<2> f{{c->UPOS, c->POS0}, {c->POS1, c->PREV}};
-        sph_launch_reorder2(c->st, n, c->start, c->perm_src, c->perm_id, c->ukey, f, c->ID1,
-                            c->K1);
+        sph_launch_reorder2(c->st, S, c->dP, c->BS, c->start, c->perm_src, c->perm_id, c->ukey,
+                            f, c->ID1, c->K1, c->dC);
     }
     {
         Tick t(c, KDE);
-        sph_launch_density(c->st, n, c->dP, c->POS1, c->K1, c->start, c->RHO, c->dC);
+        sph_launch_density(c->st, S, c->dP, c->BS, c->POS1, c->K1, c->start, c->RHO, c->dC);
     }
     {
         Tick t(c, KRE);
-        sph_launch_relax(c->st, n, true, c->dP, c->POS1, c->PREV, c->RHO, c->K1, c->start,
+        sph_launch_relax(c->st, S, true, c->dP, c->BS, c->POS1, c->PREV, c->RHO, c->K1, c->start,
                          c->UPOS, c->UVEL, c->ukey, c->uslot, c->cnt, c->dC);
     }
     enqueue_sort_tail(c, c->ID1);
     {
         Tick t(c, KRO);
         sph_reorder_fields<2> f{{c->UPOS, c->UVEL}, {c->POS0, c->VEL}};
-        sph_launch_reorder2(c->st, n, c->start, c->perm_src, c->perm_id, c->ukey, f, c->ID0,
-                            c->K0);
+        sph_launch_reorder2(c->st, S, c->dP, c->BS, c->start, c->perm_src, c->perm_id, c->ukey,
+                            f, c->ID0, c->K0, c->dC);
     }
 }
 
@@ -225,31 +235,26 @@ int build_graph(sph_ctx *c, int steps, hipGraphExec_t *out)
 
 int ensure_graphs(sph_ctx *c)
 {
-    if (c->graph_n == c->n && c->g1 && c->gK)
+    if (c->g1 && c->gK)
         return SPH_OK;
     free_graphs(c);
     int rc = build_graph(c, 1, &c->g1);
     if (rc)
         return rc;
-    rc = build_graph(c, kGraphSteps, &c->gK);
-    if (rc)
-        return rc;
-    c->graph_n = c->n;
-    return SPH_OK;
+    return build_graph(c, kGraphSteps, &c->gK);
 }
 
 // Bring PREV/RHO into ID0 order (after fused steps they follow ID1).
 int canonicalize(sph_ctx *c)
 {
-    if (c->aux_order == 0 || c->n == 0) {
+    if (c->aux_order == 0 || c->n_total == 0) {
         c->aux_order = 0;
         return SPH_OK;
     }
-    const int n = c->n;
-    sph_launch_pos_of_id(c->st, n, c->ID0, c->pos_of_id);
+    sph_launch_pos_of_id(c->st, c->slots, c->dP, c->BS, c->ID0, c->pos_of_id);
     sph_reorder_fields<2> f{{c->PREV, c->RHO}, {c->T2a, c->T2b}};
-    sph_launch_permute_by_id(c->st, n, c->ID1, c->pos_of_id, f);
-    size_t b = sizeof(float2) * (size_t)n;
+    sph_launch_permute_by_id(c->st, c->slots, c->dP, c->BS, c->ID1, c->pos_of_id, f);
+    size_t b = sizeof(float2) * (size_t)c->slots;
     HIPCHK(hipMemcpyAsync(c->PREV, c->T2a, b, hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->RHO, c->T2b, b, hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipGetLastError());
@@ -274,24 +279,34 @@ int dalloc(T **p, size_t count)
     return SPH_OK;
 }
 
+int check_scene_desc(const sph_scene_desc &d, int idx)
+{
+    const float h = d.tunables.smoothing_radius;
+    if (!(h > 0.0f))
+        return set_err(SPH_EINVAL, "scene %d: smoothing_radius must be > 0", idx);
+    if (d.boundary.min_x != 0.0f || d.boundary.min_y != 0.0f)
+        return set_err(SPH_EINVAL, "scene %d: the box must start at the origin", idx);
+    if (!(d.boundary.max_x > 0.0f && d.boundary.max_y > 0.0f))
+        return set_err(SPH_EINVAL, "scene %d: empty box", idx);
+    if (d.capacity < 0)
+        return set_err(SPH_EINVAL, "scene %d: negative capacity", idx);
+    return SPH_OK;
+}
+
 } // namespace
 
 extern "C" {
 
 const char *sph_last_error(void) { return g_err; }
 
-int sph_create(sph_ctx **out, const param *params, const AABB_t *boundary, int capacity)
+int sph_create_batch(sph_ctx **out, const sph_scene_desc *scenes, int nscenes)
 {
-    if (!out || !params || !boundary || capacity < 0)
-        return set_err(SPH_EINVAL, "sph_create: null argument or negative capacity");
+    if (!out || !scenes || nscenes <= 0)
+        return set_err(SPH_EINVAL, "sph_create_batch: null argument or no scenes");
     *out = nullptr;
-    const float h = params->tunable_params.smoothing_radius;
-    if (!(h > 0.0f))
-        return set_err(SPH_EINVAL, "sph_create: smoothing_radius must be > 0");
-    if (boundary->min_x != 0.0f || boundary->min_y != 0.0f)
-        return set_err(SPH_EINVAL, "sph_create: the box must start at the origin");
-    if (!(boundary->max_x > 0.0f && boundary->max_y > 0.0f))
-        return set_err(SPH_EINVAL, "sph_create: empty box");
+    for (int s = 0; s < nscenes; s++)
+        if (int rc = check_scene_desc(scenes[s], s))
+            return rc;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return set_err(SPH_ENODEV, "no HIP device available");
@@ -303,33 +318,52 @@ int sph_create(sph_ctx **out, const param *params, const AABB_t *boundary, int c
         return set_err(SPH_ENODEV, "device %d is %s; this library is built for gfx950", dev,
                        prop.gcnArchName);
 
-    // grid extent, src/fluid.c:214-215
-    const double gx = ceil((boundary->max_x - boundary->min_x) / h);
-    const double gy = ceil((boundary->max_y - boundary->min_y) / h);
-    if (gx * gy >= 4294967295.0)
-        return set_err(SPH_EINVAL, "sph_create: %g x %g cells exceed 32-bit keys", gx, gy);
-
     sph_ctx *c = new (std::nothrow) sph_ctx();
     if (!c)
         return set_err(SPH_ENOMEM, "host allocation failed");
     c->device = dev;
-    c->cap = capacity;
-    c->tun = params->tunable_params;
-    c->box = *boundary;
-    c->sx = (unsigned)gx;
-    c->sy = (unsigned)gy;
-    c->ncells = c->sx * c->sy;
+    c->sc.resize((size_t)nscenes);
+    unsigned long long slots = 0, cells = 0;
+    for (int s = 0; s < nscenes; s++) {
+        Scene &S = c->sc[s];
+        S.tun = scenes[s].tunables;
+        S.box = scenes[s].boundary;
+        S.cap = scenes[s].capacity;
+        const float h = S.tun.smoothing_radius;
+        // grid extent, src/fluid.c:214-215
+        const double gx = ceil((S.box.max_x - S.box.min_x) / h);
+        const double gy = ceil((S.box.max_y - S.box.min_y) / h);
+        if (gx * gy >= 4294967295.0) {
+            delete c;
+            return set_err(SPH_EINVAL, "scene %d: %g x %g cells exceed 32-bit keys", s, gx, gy);
+        }
+        S.sx = (unsigned)gx;
+        S.sy = (unsigned)gy;
+        S.ncells = S.sx * S.sy;
+        S.cell_base = (unsigned)cells;
+        S.poff = (unsigned)slots;
+        S.slots = ((unsigned)(S.cap > 0 ? S.cap : 1) + kSphTPB - 1) / kSphTPB * kSphTPB;
+        cells += (unsigned long long)S.ncells + 1; // + the scene's padding cell
+        slots += S.slots;
+    }
+    if (cells >= 4294967295ull || slots >= 2147483647ull) {
+        delete c;
+        return set_err(SPH_EINVAL, "sph_create_batch: %llu cells / %llu slots exceed 32-bit indexing",
+                       cells, slots);
+    }
+    c->cells = (unsigned)cells;
+    c->slots = (unsigned)slots;
     int rc = SPH_OK;
-    const size_t N = (size_t)capacity, M = (size_t)c->ncells + 1;
-#define A(p, cnt)                                                                                \
+    const size_t N = c->slots, M = (size_t)c->cells + 1;
+#define A(p, count)                                                                              \
     if (!rc)                                                                                     \
-    rc = dalloc(&c->p, cnt)
+    rc = dalloc(&c->p, count)
     A(POS0, N); A(VEL, N); A(PREV, N); A(RHO, N);
     A(POS1, N); A(UPOS, N); A(UVEL, N); A(T2a, N); A(T2b, N);
     A(ID0, N); A(ID1, N); A(perm_src, N); A(perm_id, N); A(pos_of_id, N);
     A(K0, N); A(K1, N); A(ukey, N); A(uslot, N);
     A(cnt, M); A(start, M); A(bsum, sph_scan_blocks((unsigned)M));
-    A(coords, 2 * N); A(dP, 1); A(dC, 1);
+    A(coords, 2 * N); A(dP, (size_t)nscenes); A(BS, N / kSphTPB); A(dC, 1);
 #undef A
     if (!rc) {
         hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
@@ -337,13 +371,21 @@ int sph_create(sph_ctx **out, const param *params, const AABB_t *boundary, int c
             rc = set_err(SPH_EHIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
     }
     if (!rc) {
-        hipError_t e = hipMemset(c->cnt, 0, sizeof(unsigned) * M);
+        std::vector<int> bs(N / kSphTPB);
+        for (int s = 0; s < nscenes; s++)
+            for (unsigned b = 0; b < c->sc[s].slots / kSphTPB; b++)
+                bs[c->sc[s].poff / kSphTPB + b] = s;
+        hipError_t e = hipMemcpy(c->BS, bs.data(), sizeof(int) * bs.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemset(c->cnt, 0, sizeof(unsigned) * M);
         if (e == hipSuccess)
             e = hipMemset(c->start, 0, sizeof(unsigned) * M);
         if (e == hipSuccess)
             e = hipMemset(c->dC, 0, sizeof(sph_dev_counters));
+        if (e == hipSuccess)
+            e = hipMemset(c->POS0, 0, sizeof(float2) * N);
         if (e != hipSuccess)
-            rc = set_err(SPH_EHIP, "hipMemset failed: %s", hipGetErrorString(e));
+            rc = set_err(SPH_EHIP, "context set-up failed: %s", hipGetErrorString(e));
     }
     if (!rc)
         rc = push_params(c);
@@ -353,6 +395,17 @@ int sph_create(sph_ctx **out, const param *params, const AABB_t *boundary, int c
     }
     *out = c;
     return SPH_OK;
+}
+
+int sph_create(sph_ctx **out, const param *params, const AABB_t *boundary, int capacity)
+{
+    if (!out || !params || !boundary || capacity < 0)
+        return set_err(SPH_EINVAL, "sph_create: null argument or negative capacity");
+    sph_scene_desc d;
+    d.tunables = params->tunable_params;
+    d.boundary = *boundary;
+    d.capacity = capacity;
+    return sph_create_batch(out, &d, 1);
 }
 
 void sph_destroy(sph_ctx *c)
@@ -365,7 +418,8 @@ void sph_destroy(sph_ctx *c)
     void *ptrs[] = {c->POS0, c->VEL,  c->PREV,     c->RHO,     c->POS1,      c->UPOS,
                     c->UVEL, c->T2a,  c->T2b,      c->ID0,     c->ID1,       c->perm_src,
                     c->perm_id, c->pos_of_id, c->K0, c->K1,    c->ukey,      c->uslot,
-                    c->cnt,  c->start, c->bsum,    c->coords,  c->dP,        c->dC};
+                    c->cnt,  c->start, c->bsum,    c->coords,  c->dP,        c->BS,
+                    c->dC};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -374,60 +428,95 @@ void sph_destroy(sph_ctx *c)
     delete c;
 }
 
-int sph_num_particles(const sph_ctx *c) { return c ? c->n : 0; }
+int sph_num_particles(const sph_ctx *c) { return c ? c->n_total : 0; }
+
+int sph_num_scenes(const sph_ctx *c) { return c ? (int)c->sc.size() : 0; }
 
 int sph_num_cells(const sph_ctx *c, unsigned *size_x, unsigned *size_y)
 {
     if (!c)
         return set_err(SPH_EINVAL, "null context");
     if (size_x)
-        *size_x = c->sx;
+        *size_x = c->sc[0].sx;
     if (size_y)
-        *size_y = c->sy;
+        *size_y = c->sc[0].sy;
     return SPH_OK;
 }
 
+int sph_scene_info(const sph_ctx *c, int scene, int *n, unsigned *size_x, unsigned *size_y)
+{
+    if (!c || scene < 0 || scene >= (int)c->sc.size())
+        return set_err(SPH_EINVAL, "sph_scene_info: bad context or scene");
+    if (n)
+        *n = c->sc[scene].n;
+    if (size_x)
+        *size_x = c->sc[scene].sx;
+    if (size_y)
+        *size_y = c->sc[scene].sy;
+    return SPH_OK;
+}
+
+// particles: scenes back to back, counts[s] records each, ids local to the
+// scene; pp (pointer array) is the one-scene form.
 static int upload_impl(sph_ctx *c, const fluid_particle *const *pp, const fluid_particle *arr,
-                       int n)
+                       const int *counts)
 {
     if (int rc = check_ctx(c))
         return rc;
-    if (n < 0 || n > c->cap)
-        return set_err(SPH_EINVAL, "sph_upload: n=%d outside [0, capacity=%d]", n, c->cap);
-    if (n > 0 && !pp && !arr)
+    const int ns = (int)c->sc.size();
+    long total = 0;
+    for (int s = 0; s < ns; s++) {
+        if (counts[s] < 0 || counts[s] > c->sc[s].cap)
+            return set_err(SPH_EINVAL, "sph_upload: scene %d count %d outside [0, capacity=%d]", s,
+                           counts[s], c->sc[s].cap);
+        total += counts[s];
+    }
+    if (total > 0 && !pp && !arr)
         return set_err(SPH_EINVAL, "sph_upload: null particles");
+    const size_t N = c->slots;
     std::vector<float2> f[4];
     for (auto &v : f)
-        v.resize((size_t)n);
-    std::vector<int> ids((size_t)n);
-    std::vector<char> seen((size_t)n, 0);
-    c->ax_by_id.assign((size_t)n, 0.0f);
-    c->ay_by_id.assign((size_t)n, 0.0f);
-    for (int i = 0; i < n; i++) {
-        const fluid_particle *p = pp ? pp[i] : &arr[i];
-        if (!p)
-            return set_err(SPH_EINVAL, "sph_upload: null particle pointer at %d", i);
-        if (p->id < 0 || p->id >= n || seen[p->id])
-            return set_err(SPH_EINVAL, "sph_upload: ids must be a permutation of 0..%d (bad id %d at %d)",
-                           n - 1, p->id, i);
-        seen[p->id] = 1;
-        ids[i] = p->id;
-        f[0][i] = make_float2(p->x, p->y);
-        f[1][i] = make_float2(p->v_x, p->v_y);
-        f[2][i] = make_float2(p->x_prev, p->y_prev);
-        f[3][i] = make_float2(p->density, p->density_near);
-        c->ax_by_id[p->id] = p->a_x;
-        c->ay_by_id[p->id] = p->a_y;
+        v.assign(N, make_float2(0.0f, 0.0f));
+    std::vector<int> ids(N, 0);
+    std::vector<float> ax((size_t)total, 0.0f), ay((size_t)total, 0.0f);
+    std::vector<unsigned> uoff((size_t)ns);
+    long u = 0;
+    for (int s = 0; s < ns; s++) {
+        const int n = counts[s];
+        std::vector<char> seen((size_t)n, 0);
+        for (int i = 0; i < n; i++) {
+            const fluid_particle *p = pp ? pp[u + i] : &arr[u + i];
+            if (!p)
+                return set_err(SPH_EINVAL, "sph_upload: null particle pointer at %ld", u + i);
+            if (p->id < 0 || p->id >= n || seen[p->id])
+                return set_err(SPH_EINVAL,
+                               "sph_upload: scene %d ids must be a permutation of 0..%d (bad id %d at %d)",
+                               s, n - 1, p->id, i);
+            seen[p->id] = 1;
+            const size_t slot = c->sc[s].poff + (size_t)i;
+            ids[slot] = p->id;
+            f[0][slot] = make_float2(p->x, p->y);
+            f[1][slot] = make_float2(p->v_x, p->v_y);
+            f[2][slot] = make_float2(p->x_prev, p->y_prev);
+            f[3][slot] = make_float2(p->density, p->density_near);
+            ax[u + p->id] = p->a_x;
+            ay[u + p->id] = p->a_y;
+        }
+        uoff[s] = (unsigned)u;
+        u += n;
     }
     HIPCHK(hipStreamSynchronize(c->st));
     float2 *dst[4] = {c->POS0, c->VEL, c->PREV, c->RHO};
-    const size_t b = sizeof(float2) * (size_t)n;
-    if (n > 0) {
-        for (int k = 0; k < 4; k++)
-            HIPCHK(hipMemcpy(dst[k], f[k].data(), b, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c->ID0, ids.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice));
+    for (int k = 0; k < 4; k++)
+        HIPCHK(hipMemcpy(dst[k], f[k].data(), sizeof(float2) * N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->ID0, ids.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+    for (int s = 0; s < ns; s++) {
+        c->sc[s].n = counts[s];
+        c->sc[s].uoff = uoff[s];
     }
-    c->n = n;
+    c->ax_by_id.swap(ax);
+    c->ay_by_id.swap(ay);
+    c->n_total = (int)total;
     c->lists_valid = false;
     c->aux_order = 0;
     return push_params(c);
@@ -435,27 +524,72 @@ static int upload_impl(sph_ctx *c, const fluid_particle *const *pp, const fluid_
 
 int sph_upload(sph_ctx *c, const fluid_particle *particles, int n)
 {
-    return upload_impl(c, nullptr, particles, n);
+    if (int rc = check_ctx(c))
+        return rc;
+    if (c->sc.size() != 1)
+        return set_err(SPH_EINVAL, "sph_upload: context holds %zu scenes; use sph_upload_batch",
+                       c->sc.size());
+    return upload_impl(c, nullptr, particles, &n);
 }
 
 int sph_upload_pointers(sph_ctx *c, fluid_particle *const *fluid_particle_pointers, int n)
 {
-    return upload_impl(c, (const fluid_particle *const *)fluid_particle_pointers, nullptr, n);
+    if (int rc = check_ctx(c))
+        return rc;
+    if (c->sc.size() != 1)
+        return set_err(SPH_EINVAL, "sph_upload_pointers: one-scene contexts only");
+    return upload_impl(c, (const fluid_particle *const *)fluid_particle_pointers, nullptr, &n);
+}
+
+int sph_upload_batch(sph_ctx *c, const fluid_particle *particles, const int *counts)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (!counts)
+        return set_err(SPH_EINVAL, "sph_upload_batch: null counts");
+    return upload_impl(c, nullptr, particles, counts);
+}
+
+static int set_tunables_impl(sph_ctx *c, int scene, const tunable_parameters *t)
+{
+    if (!t)
+        return set_err(SPH_EINVAL, "sph_set_tunables: null");
+    bool changed = false;
+    for (int s = 0; s < (int)c->sc.size(); s++) {
+        if (scene >= 0 && s != scene)
+            continue;
+        if (t->smoothing_radius != c->sc[s].tun.smoothing_radius)
+            return set_err(SPH_EINVAL,
+                           "sph_set_tunables: smoothing_radius fixes the grid and cannot change");
+    }
+    for (int s = 0; s < (int)c->sc.size(); s++) {
+        if (scene >= 0 && s != scene)
+            continue;
+        if (memcmp(t, &c->sc[s].tun, sizeof *t) != 0) {
+            c->sc[s].tun = *t;
+            changed = true;
+        }
+    }
+    if (!changed)
+        return SPH_OK;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return push_params(c);
 }
 
 int sph_set_tunables(sph_ctx *c, const tunable_parameters *t)
 {
     if (int rc = check_ctx(c))
         return rc;
-    if (!t)
-        return set_err(SPH_EINVAL, "sph_set_tunables: null");
-    if (t->smoothing_radius != c->tun.smoothing_radius)
-        return set_err(SPH_EINVAL, "sph_set_tunables: smoothing_radius fixes the grid and cannot change");
-    if (memcmp(t, &c->tun, sizeof *t) == 0)
-        return SPH_OK;
-    c->tun = *t;
-    HIPCHK(hipStreamSynchronize(c->st));
-    return push_params(c);
+    return set_tunables_impl(c, -1, t);
+}
+
+int sph_set_scene_tunables(sph_ctx *c, int scene, const tunable_parameters *t)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (scene < 0 || scene >= (int)c->sc.size())
+        return set_err(SPH_EINVAL, "sph_set_scene_tunables: scene %d out of range", scene);
+    return set_tunables_impl(c, scene, t);
 }
 
 int sph_step_eager(sph_ctx *c, int nsteps)
@@ -464,7 +598,7 @@ int sph_step_eager(sph_ctx *c, int nsteps)
         return rc;
     if (nsteps < 0)
         return set_err(SPH_EINVAL, "nsteps < 0");
-    if (c->n == 0)
+    if (c->n_total == 0)
         return SPH_OK;
     for (int i = 0; i < nsteps; i++) {
         enqueue_fused_step(c, c->lists_valid);
@@ -483,7 +617,7 @@ int sph_step(sph_ctx *c, int nsteps)
         return rc;
     if (nsteps < 0)
         return set_err(SPH_EINVAL, "nsteps < 0");
-    if (c->n == 0 || nsteps == 0)
+    if (c->n_total == 0 || nsteps == 0)
         return SPH_OK;
     if (!c->lists_valid) { // first step after upload: empty lists, no viscosity
         int rc = sph_step_eager(c, 1);
@@ -526,8 +660,8 @@ int sph_phase_gravity(sph_ctx *c)
         return rc;
     if (int rc = canonicalize(c))
         return rc;
-    if (c->n)
-        sph_launch_gravity(c->st, c->n, c->dP, c->VEL, c->RHO);
+    if (c->n_total)
+        sph_launch_gravity(c->st, c->slots, c->dP, c->BS, c->VEL, c->RHO);
     HIPCHK(hipGetLastError());
     return SPH_OK;
 }
@@ -538,11 +672,11 @@ int sph_phase_viscosity(sph_ctx *c)
         return rc;
     if (int rc = canonicalize(c))
         return rc;
-    if (!c->lists_valid || c->n == 0)
+    if (!c->lists_valid || c->n_total == 0)
         return SPH_OK; // empty lists (src/fluid.c:202)
-    sph_launch_visc_predict(c->st, c->n, false, true, false, c->dP, c->POS0, c->VEL, c->K0,
-                            c->start, c->UVEL, nullptr, nullptr, c->cnt, c->dC);
-    HIPCHK(hipMemcpyAsync(c->VEL, c->UVEL, sizeof(float2) * (size_t)c->n,
+    sph_launch_visc_predict(c->st, c->slots, false, true, false, c->dP, c->BS, c->POS0, c->VEL,
+                            c->K0, c->start, c->UVEL, nullptr, nullptr, c->cnt, c->dC);
+    HIPCHK(hipMemcpyAsync(c->VEL, c->UVEL, sizeof(float2) * (size_t)c->slots,
                           hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipGetLastError());
     return SPH_OK;
@@ -554,8 +688,8 @@ int sph_phase_predict(sph_ctx *c)
         return rc;
     if (int rc = canonicalize(c))
         return rc;
-    if (c->n)
-        sph_launch_predict(c->st, c->n, c->dP, c->POS0, c->VEL, c->PREV);
+    if (c->n_total)
+        sph_launch_predict(c->st, c->slots, c->dP, c->BS, c->POS0, c->VEL, c->PREV);
     HIPCHK(hipGetLastError());
     return SPH_OK;
 }
@@ -566,22 +700,23 @@ int sph_phase_hash(sph_ctx *c, int compute_density)
         return rc;
     if (int rc = canonicalize(c))
         return rc;
-    const int n = c->n;
-    if (n == 0)
+    if (c->n_total == 0)
         return SPH_OK;
-    sph_launch_key_hist(c->st, n, c->dP, c->POS0, c->ukey, c->uslot, c->cnt, c->dC);
+    const unsigned S = c->slots;
+    sph_launch_key_hist(c->st, S, c->dP, c->BS, c->POS0, c->ukey, c->uslot, c->cnt, c->dC);
     enqueue_sort_tail(c, c->ID0);
     sph_reorder_fields<4> f{{c->POS0, c->VEL, c->PREV, c->RHO},
                             {c->POS1, c->UVEL, c->T2a, c->T2b}};
-    sph_launch_reorder4(c->st, n, c->start, c->perm_src, c->perm_id, c->ukey, f, c->ID1, c->K1);
-    const size_t b = sizeof(float2) * (size_t)n;
+    sph_launch_reorder4(c->st, S, c->dP, c->BS, c->start, c->perm_src, c->perm_id, c->ukey, f,
+                        c->ID1, c->K1, c->dC);
+    const size_t b = sizeof(float2) * (size_t)S;
     for (int k = 0; k < 4; k++)
         HIPCHK(hipMemcpyAsync((void *)f.src[k], f.dst[k], b, hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(c->ID0, c->ID1, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(c->K0, c->K1, sizeof(unsigned) * (size_t)n, hipMemcpyDeviceToDevice,
+    HIPCHK(hipMemcpyAsync(c->ID0, c->ID1, sizeof(int) * (size_t)S, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->K0, c->K1, sizeof(unsigned) * (size_t)S, hipMemcpyDeviceToDevice,
                           c->st));
     if (compute_density)
-        sph_launch_density(c->st, n, c->dP, c->POS0, c->K0, c->start, c->RHO, c->dC);
+        sph_launch_density(c->st, S, c->dP, c->BS, c->POS0, c->K0, c->start, c->RHO, c->dC);
     HIPCHK(hipGetLastError());
     c->lists_valid = true;
     c->hashes++;
@@ -594,13 +729,13 @@ int sph_phase_relax(sph_ctx *c)
         return rc;
     if (int rc = canonicalize(c))
         return rc;
-    if (c->n == 0)
+    if (c->n_total == 0)
         return SPH_OK;
     if (!c->lists_valid)
         return set_err(SPH_ESTATE, "sph_phase_relax: no hash since upload");
-    sph_launch_relax(c->st, c->n, false, c->dP, c->POS0, c->PREV, c->RHO, c->K0, c->start,
-                     c->UPOS, nullptr, nullptr, nullptr, nullptr, c->dC);
-    HIPCHK(hipMemcpyAsync(c->POS0, c->UPOS, sizeof(float2) * (size_t)c->n,
+    sph_launch_relax(c->st, c->slots, false, c->dP, c->BS, c->POS0, c->PREV, c->RHO, c->K0,
+                     c->start, c->UPOS, nullptr, nullptr, nullptr, nullptr, c->dC);
+    HIPCHK(hipMemcpyAsync(c->POS0, c->UPOS, sizeof(float2) * (size_t)c->slots,
                           hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipGetLastError());
     return SPH_OK;
@@ -612,8 +747,8 @@ int sph_phase_update_velocities(sph_ctx *c)
         return rc;
     if (int rc = canonicalize(c))
         return rc;
-    if (c->n)
-        sph_launch_update_velocities(c->st, c->n, c->dP, c->POS0, c->VEL, c->PREV);
+    if (c->n_total)
+        sph_launch_update_velocities(c->st, c->slots, c->dP, c->BS, c->POS0, c->VEL, c->PREV);
     HIPCHK(hipGetLastError());
     return SPH_OK;
 }
@@ -624,43 +759,47 @@ int sph_download(sph_ctx *c, fluid_particle *out)
 {
     if (int rc = check_ctx(c))
         return rc;
-    const int n = c->n;
-    if (n > 0 && !out)
+    if (c->n_total > 0 && !out)
         return set_err(SPH_EINVAL, "sph_download: null output");
     HIPCHK(hipStreamSynchronize(c->st));
-    if (n == 0)
+    if (c->n_total == 0)
         return SPH_OK;
+    const size_t N = c->slots;
     std::vector<float2> f[4];
     const float2 *src[4] = {c->POS0, c->VEL, c->PREV, c->RHO};
-    const size_t b = sizeof(float2) * (size_t)n;
     for (int k = 0; k < 4; k++) {
-        f[k].resize((size_t)n);
-        HIPCHK(hipMemcpy(f[k].data(), src[k], b, hipMemcpyDeviceToHost));
+        f[k].resize(N);
+        HIPCHK(hipMemcpy(f[k].data(), src[k], sizeof(float2) * N, hipMemcpyDeviceToHost));
     }
-    std::vector<int> id0((size_t)n), idaux((size_t)n);
-    HIPCHK(hipMemcpy(id0.data(), c->ID0, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(idaux.data(), c->aux_order ? c->ID1 : c->ID0, sizeof(int) * (size_t)n,
+    std::vector<int> id0(N), idaux(N);
+    HIPCHK(hipMemcpy(id0.data(), c->ID0, sizeof(int) * N, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(idaux.data(), c->aux_order ? c->ID1 : c->ID0, sizeof(int) * N,
                      hipMemcpyDeviceToHost));
-    const tunable_parameters &t = c->tun;
-    for (int s = 0; s < n; s++) {
-        fluid_particle *p = &out[id0[s]];
-        p->x = f[0][s].x;
-        p->y = f[0][s].y;
-        p->v_x = f[1][s].x;
-        p->v_y = f[1][s].y;
-        p->id = id0[s];
-        p->a_x = c->ax_by_id[id0[s]];
-        p->a_y = c->ay_by_id[id0[s]];
-    }
-    for (int s = 0; s < n; s++) {
-        fluid_particle *p = &out[idaux[s]];
-        p->x_prev = f[2][s].x;
-        p->y_prev = f[2][s].y;
-        p->density = f[3][s].x;
-        p->density_near = f[3][s].y;
-        // src/fluid.c:563-564
-        p->pressure = t.k * (f[3][s].x - t.rest_density);
-        p->pressure_near = t.k_near * f[3][s].y;
+    for (const Scene &S : c->sc) {
+        const tunable_parameters &t = S.tun;
+        for (int i = 0; i < S.n; i++) {
+            const size_t slot = S.poff + (size_t)i;
+            const size_t g = S.uoff + (size_t)id0[slot];
+            fluid_particle *p = &out[g];
+            p->x = f[0][slot].x;
+            p->y = f[0][slot].y;
+            p->v_x = f[1][slot].x;
+            p->v_y = f[1][slot].y;
+            p->id = id0[slot];
+            p->a_x = c->ax_by_id[g];
+            p->a_y = c->ay_by_id[g];
+        }
+        for (int i = 0; i < S.n; i++) {
+            const size_t slot = S.poff + (size_t)i;
+            fluid_particle *p = &out[S.uoff + (size_t)idaux[slot]];
+            p->x_prev = f[2][slot].x;
+            p->y_prev = f[2][slot].y;
+            p->density = f[3][slot].x;
+            p->density_near = f[3][slot].y;
+            // src/fluid.c:563-564
+            p->pressure = t.k * (f[3][slot].x - t.rest_density);
+            p->pressure_near = t.k_near * f[3][slot].y;
+        }
     }
     return SPH_OK;
 }
@@ -669,13 +808,13 @@ int sph_pack_coords_i16(sph_ctx *c, short *coords)
 {
     if (int rc = check_ctx(c))
         return rc;
-    if (c->n > 0 && !coords)
+    if (c->n_total > 0 && !coords)
         return set_err(SPH_EINVAL, "sph_pack_coords_i16: null output");
-    if (c->n == 0)
+    if (c->n_total == 0)
         return SPH_OK;
-    sph_launch_pack_coords(c->st, c->n, c->POS0, c->ID0, c->box.max_x, c->box.max_y, c->coords);
+    sph_launch_pack_coords(c->st, c->slots, c->dP, c->BS, c->POS0, c->ID0, c->coords);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(coords, c->coords, sizeof(short) * 2 * (size_t)c->n,
+    HIPCHK(hipMemcpyAsync(coords, c->coords, sizeof(short) * 2 * (size_t)c->n_total,
                           hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     return SPH_OK;
@@ -685,7 +824,7 @@ int sph_read_sorted(sph_ctx *c, int field, void *out)
 {
     if (int rc = check_ctx(c))
         return rc;
-    if (!out && c->n)
+    if (!out && c->n_total)
         return set_err(SPH_EINVAL, "sph_read_sorted: null output");
     if (int rc = canonicalize(c))
         return rc;
@@ -705,17 +844,19 @@ int sph_read_sorted(sph_ctx *c, int field, void *out)
     default: return set_err(SPH_EINVAL, "sph_read_sorted: unknown field %d", field);
     }
     HIPCHK(hipStreamSynchronize(c->st));
-    if (c->n == 0)
+    if (c->n_total == 0)
         return SPH_OK;
-    if (comp < 0) {
-        HIPCHK(hipMemcpy(out, src, 4 * (size_t)c->n, hipMemcpyDeviceToHost));
-    } else {
-        std::vector<float2> tmp((size_t)c->n);
-        HIPCHK(hipMemcpy(tmp.data(), src, sizeof(float2) * (size_t)c->n, hipMemcpyDeviceToHost));
-        float *o = (float *)out;
-        for (int i = 0; i < c->n; i++)
-            o[i] = comp ? tmp[i].y : tmp[i].x;
-    }
+    const size_t N = c->slots;
+    std::vector<float2> tmp(N);
+    HIPCHK(hipMemcpy(tmp.data(), src, (comp < 0 ? 4 : 8) * N, hipMemcpyDeviceToHost));
+    const uint32_t *w = (const uint32_t *)tmp.data();
+    uint32_t *o = (uint32_t *)out;
+    size_t k = 0;
+    for (const Scene &S : c->sc)
+        for (int i = 0; i < S.n; i++) {
+            const size_t slot = S.poff + (size_t)i;
+            o[k++] = comp < 0 ? w[slot] : w[2 * slot + (size_t)comp];
+        }
     return SPH_OK;
 }
 
@@ -723,12 +864,14 @@ int sph_read_cells(sph_ctx *c, unsigned *start, unsigned *count)
 {
     if (int rc = check_ctx(c))
         return rc;
-    std::vector<unsigned> s((size_t)c->ncells + 1);
+    const Scene &S = c->sc[0];
+    std::vector<unsigned> s((size_t)S.ncells + 1);
     HIPCHK(hipStreamSynchronize(c->st));
-    HIPCHK(hipMemcpy(s.data(), c->start, sizeof(unsigned) * s.size(), hipMemcpyDeviceToHost));
-    for (unsigned i = 0; i < c->ncells; i++) {
+    HIPCHK(hipMemcpy(s.data(), c->start + S.cell_base, sizeof(unsigned) * s.size(),
+                     hipMemcpyDeviceToHost));
+    for (unsigned i = 0; i < S.ncells; i++) {
         if (start)
-            start[i] = s[i];
+            start[i] = s[i] - S.poff;
         if (count)
             count[i] = s[i + 1] - s[i];
     }
@@ -743,7 +886,7 @@ int sph_read_neighbors(sph_ctx *c, int stride, int *full_by_id, int *half_by_id,
         return set_err(SPH_EINVAL, "sph_read_neighbors: bad arguments");
     if (!c->lists_valid)
         return set_err(SPH_ESTATE, "sph_read_neighbors: no hash since upload");
-    const int n = c->n;
+    const int n = c->n_total;
     if (n == 0)
         return SPH_OK;
     int *dfull = nullptr, *dhalf = nullptr, *dids = nullptr;
@@ -753,8 +896,8 @@ int sph_read_neighbors(sph_ctx *c, int stride, int *full_by_id, int *half_by_id,
     if (!rc)
         rc = dalloc(&dids, (size_t)n * (size_t)(stride ? stride : 1));
     if (!rc) {
-        sph_launch_neighbor_lists(c->st, n, c->dP, c->POS0, c->K0, c->ID0, c->start, stride,
-                                  dfull, dhalf, dids);
+        sph_launch_neighbor_lists(c->st, c->slots, c->dP, c->BS, c->POS0, c->K0, c->ID0, c->start,
+                                  stride, dfull, dhalf, dids);
         hipError_t e = hipStreamSynchronize(c->st);
         if (e == hipSuccess)
             e = hipMemcpy(full_by_id, dfull, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost);
@@ -812,31 +955,58 @@ int sph_time_kernels(sph_ctx *c, int nsteps, int max_kinds, const char **names, 
     if (nsteps <= 0 || max_kinds < NKIND || !names || !ms || !nkinds)
         return set_err(SPH_EINVAL, "sph_time_kernels: need nsteps > 0 and room for %d kinds",
                        (int)NKIND);
-    if (c->n == 0)
+    if (c->n_total == 0)
         return set_err(SPH_ESTATE, "sph_time_kernels: no particles");
-    std::vector<hipEvent_t> ev;
+    if (!c->lists_valid) {
+        int rc = sph_step_eager(c, 1);
+        if (rc)
+            return rc;
+    }
+    // Boundary events are created up front and recorded before every kernel
+    // and after the last one of each step; all nsteps steps are enqueued
+    // without synchronising, so the host runs ahead of the device and the
+    // events bracket the kernels without launch gaps.
+    constexpr int kLaunches = 11;
+    const size_t nev = (size_t)nsteps * (kLaunches + 1);
+    std::vector<hipEvent_t> ev(nev, nullptr);
     std::vector<int> kind;
-    c->tev = &ev;
-    c->tkind = &kind;
-    int rc = sph_step_eager(c, nsteps);
-    c->tev = nullptr;
-    c->tkind = nullptr;
-    hipError_t se = hipStreamSynchronize(c->st);
+    hipError_t e = hipSuccess;
+    for (auto &x : ev)
+        if (e == hipSuccess)
+            e = hipEventCreate(&x);
+    if (e == hipSuccess) {
+        c->tev = &ev;
+        c->tkind = &kind;
+        c->tidx = 0;
+        for (int it = 0; it < nsteps; it++) {
+            enqueue_fused_step(c, true);
+            (void)hipEventRecord(ev[c->tidx++], c->st);
+            kind.push_back(-1); // step boundary
+            c->steps++;
+            c->hashes += 2;
+        }
+        c->tev = nullptr;
+        c->tkind = nullptr;
+        c->aux_order = 1;
+        e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(c->st);
+    }
     double tot[NKIND] = {0};
     int cntk[NKIND] = {0};
-    for (size_t i = 0; i < kind.size(); i++) {
+    for (size_t i = 0; e == hipSuccess && i + 1 < c->tidx; i++) {
+        if (kind[i] < 0)
+            continue;
         float t = 0.0f;
-        if (hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]) == hipSuccess) {
-            tot[kind[i]] += t;
-            cntk[kind[i]]++;
-        }
+        e = hipEventElapsedTime(&t, ev[i], ev[i + 1]);
+        tot[kind[i]] += t;
+        cntk[kind[i]]++;
     }
-    for (hipEvent_t e : ev)
-        (void)hipEventDestroy(e);
-    if (rc)
-        return rc;
-    if (se != hipSuccess)
-        return set_err(SPH_EHIP, "hipStreamSynchronize failed: %s", hipGetErrorString(se));
+    for (auto &x : ev)
+        if (x)
+            (void)hipEventDestroy(x);
+    if (e != hipSuccess)
+        return set_err(SPH_EHIP, "sph_time_kernels: %s", hipGetErrorString(e));
     for (int k = 0; k < NKIND; k++) {
         names[k] = kKindNames[k];
         ms[k] = cntk[k] ? (float)(tot[k] / cntk[k]) : 0.0f;

@@ -10,18 +10,32 @@
 
 #include "../../include/sph_abi.h"
 
-// Step parameters as the kernels read them, derived on the host from
-// tunable_parameters exactly as the reference derives them
+// Accepted neighbour pairs evaluated per iteration of a particle's walk
+// (independent dependency chains for latency hiding).
+#ifndef SPH_PAIR_ILP
+#define SPH_PAIR_ILP 1
+#endif
+
+// Step parameters of one scene as the kernels read them, derived on the
+// host from tunable_parameters exactly as the reference derives them
 // (h_recip = 1.0f/h: src/fluid.c:427, 555; g = -g: src/fluid.c:403;
-// h2 = h*h: src/hash.c:133).
+// h2 = h*h: src/hash.c:133), plus the scene's place in the shared arrays:
+// its particles occupy slots [poff, poff + n) of the sorted arrays (the
+// range is padded to a multiple of 256 slots, so every workgroup belongs to
+// one scene) and its cells are global cells [cell_base, cell_base + ncells),
+// followed by one padding cell that absorbs the pad slots in the sort.
 struct sph_dev_params {
     float h, h_recip, h2, dt, g_neg;
     float rest_density, k, k_near, k_spring, sigma, beta;
     float min_x, max_x, min_y, max_y;
     float mover_cx, mover_cy, mover_w, mover_h;
     int mover_type;
-    unsigned sx, sy, ncells;
-    int n;
+    unsigned sx, sy, ncells; // local grid: ncells = sx * sy
+    unsigned cell_base;      // first global cell of the scene
+    unsigned poff;           // first slot of the scene in the sorted arrays
+    unsigned uoff;           // first index of the scene in caller (unpadded) order
+    int n;                   // particles of the scene
+    unsigned pad;            // padding slots after the scene's particles
 };
 
 // Device health counters (mirrors sph_counters_t from the key counters on).
@@ -36,22 +50,23 @@ struct sph_dev_counters {
 
 // src/hash.c:35-47.  key = floor(y/h)*size_x + floor(x/h); keys the
 // reference could not index (>= cells, negative, NaN: undefined behaviour
-// there) are counted and clamped to the last cell.
+// there) are counted and clamped to the scene's last cell.  Returns the
+// global cell (cell_base + local key).
 __device__ __forceinline__ unsigned sph_cell_key(float x, float y, const sph_dev_params &P,
                                                  unsigned long long *oor)
 {
     float qx = x / P.h, qy = y / P.h;
     if (!(qx >= 0.0f && qy >= 0.0f && qx < 4294967296.0f && qy < 4294967296.0f)) {
         atomicAdd(oor, 1ull);
-        return P.ncells - 1;
+        return P.cell_base + P.ncells - 1;
     }
     unsigned long long k = (unsigned long long)(unsigned)floorf(qy) * P.sx +
                            (unsigned long long)(unsigned)floorf(qx);
     if (k >= P.ncells) {
         atomicAdd(oor, 1ull);
-        return P.ncells - 1;
+        return P.cell_base + P.ncells - 1;
     }
-    return (unsigned)k;
+    return P.cell_base + (unsigned)k;
 }
 
 // Correctly rounded square root.  For 2^-96 <= x < inf, v_sqrt_f32 is
@@ -187,8 +202,8 @@ __device__ __forceinline__ bool sph_owns(unsigned k, unsigned s, int dy, unsigne
     return dy == 0 && k > s;
 }
 
-// Walk the 3x3 neighbourhood of the particle at sorted position s (cell
-// `key`) in canonical order: rows j-1, j, j+1; in each row the contiguous
+// Walk the 3x3 neighbourhood of the particle at sorted position s (global
+// cell `key`) in canonical order: rows j-1, j, j+1; in each row the contiguous
 // sorted range of cells i-1..i+1 (clipped to the grid), ascending.  Self
 // (k == s) is visited too; every body rejects it together with its distance
 // test, so the wave takes one branch per candidate.  `src.row(dy)` prepares
@@ -200,7 +215,8 @@ __device__ __forceinline__ void sph_walk(unsigned s, unsigned key, const sph_dev
                                          Body &&body)
 {
     const unsigned sx = P.sx, sy = P.sy;
-    const unsigned i = key % sx, j = key / sx;
+    const unsigned local = key - P.cell_base;
+    const unsigned i = local % sx, j = local / sx;
     const unsigned i0 = i > 0 ? i - 1 : 0;
     const unsigned i1 = i + 1 < sx ? i + 1 : sx - 1;
 #pragma unroll
@@ -208,7 +224,7 @@ __device__ __forceinline__ void sph_walk(unsigned s, unsigned key, const sph_dev
         const long r = (long)j + dy;
         if (r < 0 || r >= (long)sy)
             continue;
-        const unsigned base = (unsigned)r * sx;
+        const unsigned base = P.cell_base + (unsigned)r * sx;
         const unsigned b0 = start[base + i0], b1 = start[base + i];
         const unsigned b2 = start[base + i + 1], b3 = start[base + i1 + 1];
         const int row = src.row(dy);
@@ -223,14 +239,16 @@ __device__ __forceinline__ void sph_walk(unsigned s, unsigned key, const sph_dev
 // the set bits only, lowest first.  The visiting order of accepted
 // candidates is exactly sph_walk's, so sums are bit-identical, but a wave no
 // longer executes the pair work for every candidate that any one of its
-// lanes accepts.  Returns the number of accepted candidates.
+// lanes accepts.  body(k, dy, b1, b2, rec, use) applies its updates only
+// when `use` is true.  Returns the number of accepted candidates.
 template <class Src, class Test, class Body>
 __device__ __forceinline__ unsigned sph_walk_masked(unsigned key, const sph_dev_params &P,
                                                     const unsigned *__restrict__ start,
                                                     const Src &src, Test &&test, Body &&body)
 {
     const unsigned sx = P.sx, sy = P.sy;
-    const unsigned i = key % sx, j = key / sx;
+    const unsigned local = key - P.cell_base;
+    const unsigned i = local % sx, j = local / sx;
     const unsigned i0 = i > 0 ? i - 1 : 0;
     const unsigned i1 = i + 1 < sx ? i + 1 : sx - 1;
     unsigned accepted = 0;
@@ -239,7 +257,7 @@ __device__ __forceinline__ unsigned sph_walk_masked(unsigned key, const sph_dev_
         const long r = (long)j + dy;
         if (r < 0 || r >= (long)sy)
             continue;
-        const unsigned base = (unsigned)r * sx;
+        const unsigned base = P.cell_base + (unsigned)r * sx;
         const unsigned b0 = start[base + i0], b1 = start[base + i];
         const unsigned b2 = start[base + i + 1], b3 = start[base + i1 + 1];
         const int row = src.row(dy);
@@ -249,11 +267,28 @@ __device__ __forceinline__ unsigned sph_walk_masked(unsigned key, const sph_dev_
             for (unsigned t = 0; t < m; t++)
                 mask |= (unsigned long long)test(c + t, src.at(c + t, row)) << t;
             accepted += (unsigned)__popcll(mask);
+#if SPH_PAIR_ILP >= 2
+            // two accepted candidates per iteration, as independent chains;
+            // the second is applied only if it exists (use = false turns
+            // its updates into no-ops), so the order of updates is unchanged
+            while (mask) {
+                const unsigned t0 = (unsigned)__builtin_ctzll(mask);
+                mask &= mask - 1;
+                const bool two = mask != 0;
+                const unsigned t1 = two ? (unsigned)__builtin_ctzll(mask) : t0;
+                mask &= two ? mask - 1 : mask;
+                const auto r0 = src.at(c + t0, row);
+                const auto r1 = src.at(c + t1, row);
+                body(c + t0, dy, b1, b2, r0, true);
+                body(c + t1, dy, b1, b2, r1, two);
+            }
+#else
             while (mask) {
                 const unsigned t = (unsigned)__builtin_ctzll(mask);
                 mask &= mask - 1;
-                body(c + t, dy, b1, b2, src.at(c + t, row));
+                body(c + t, dy, b1, b2, src.at(c + t, row), true);
             }
+#endif
         }
     }
     return accepted;

@@ -11,42 +11,53 @@ struct sph_reorder_fields {
     float2 *dst[NF];
 };
 
-void sph_launch_visc_predict(hipStream_t st, int n, bool grav, bool visc, bool pred,
-                             const sph_dev_params *P, const float2 *POS, const float2 *VEL,
-                             const unsigned *K, const unsigned *start, float2 *opos,
-                             unsigned *okey, unsigned *oslot, unsigned *cnt, sph_dev_counters *C);
-void sph_launch_gravity(hipStream_t st, int n, const sph_dev_params *P, float2 *VEL, float2 *RHO);
-void sph_launch_predict(hipStream_t st, int n, const sph_dev_params *P, float2 *POS,
-                        const float2 *VEL, float2 *PREV);
-void sph_launch_update_velocities(hipStream_t st, int n, const sph_dev_params *P, float2 *POS,
-                                  float2 *VEL, const float2 *PREV);
-void sph_launch_key_hist(hipStream_t st, int n, const sph_dev_params *P, const float2 *POS,
-                         unsigned *okey, unsigned *oslot, unsigned *cnt, sph_dev_counters *C);
+// Workgroup size of every particle kernel; scenes are padded to multiples.
+constexpr unsigned kSphTPB = 256;
+
+void sph_launch_visc_predict(hipStream_t st, unsigned slots, bool grav, bool visc, bool pred,
+                             const sph_dev_params *P, const int *BS, const float2 *POS,
+                             const float2 *VEL, const unsigned *K, const unsigned *start,
+                             float2 *opos, unsigned *okey, unsigned *oslot, unsigned *cnt,
+                             sph_dev_counters *C);
+void sph_launch_gravity(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                        float2 *VEL, float2 *RHO);
+void sph_launch_predict(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                        float2 *POS, const float2 *VEL, float2 *PREV);
+void sph_launch_update_velocities(hipStream_t st, unsigned slots, const sph_dev_params *P,
+                                  const int *BS, float2 *POS, float2 *VEL, const float2 *PREV);
+void sph_launch_key_hist(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                         const float2 *POS, unsigned *okey, unsigned *oslot, unsigned *cnt,
+                         sph_dev_counters *C);
 unsigned sph_scan_blocks(unsigned m);
-void sph_launch_scan_reduce(hipStream_t st, const unsigned *cnt, unsigned m, unsigned *bsum,
-                            sph_dev_counters *C);
+void sph_launch_scan_reduce(hipStream_t st, const unsigned *cnt, unsigned m, unsigned *bsum);
 void sph_launch_scan_apply(hipStream_t st, unsigned *cnt, unsigned m, const unsigned *bsum,
                            unsigned *start);
-void sph_launch_scatter(hipStream_t st, int n, const unsigned *key, const unsigned *slot,
-                        const unsigned *start, const int *id_src, int *perm_src, int *perm_id);
-void sph_launch_reorder2(hipStream_t st, int n, const unsigned *start, const int *perm_src,
-                         const int *perm_id, const unsigned *key, sph_reorder_fields<2> f,
-                         int *id_dst, unsigned *key_dst);
-void sph_launch_reorder4(hipStream_t st, int n, const unsigned *start, const int *perm_src,
-                         const int *perm_id, const unsigned *key, sph_reorder_fields<4> f,
-                         int *id_dst, unsigned *key_dst);
-void sph_launch_density(hipStream_t st, int n, const sph_dev_params *P, const float2 *POS,
-                        const unsigned *K, const unsigned *start, float2 *RHO,
+void sph_launch_scatter(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                        const unsigned *key, const unsigned *slot, const unsigned *start,
+                        const int *id_src, int *perm_src, int *perm_id);
+void sph_launch_reorder2(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                         const unsigned *start, const int *perm_src, const int *perm_id,
+                         const unsigned *key, sph_reorder_fields<2> f, int *id_dst,
+                         unsigned *key_dst, sph_dev_counters *C);
+void sph_launch_reorder4(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                         const unsigned *start, const int *perm_src, const int *perm_id,
+                         const unsigned *key, sph_reorder_fields<4> f, int *id_dst,
+                         unsigned *key_dst, sph_dev_counters *C);
+void sph_launch_density(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                        const float2 *POS, const unsigned *K, const unsigned *start, float2 *RHO,
                         sph_dev_counters *C);
-void sph_launch_relax(hipStream_t st, int n, bool fused, const sph_dev_params *P,
-                      const float2 *POS, const float2 *PREV, const float2 *RHO, const unsigned *K,
-                      const unsigned *start, float2 *opos, float2 *ovel, unsigned *okey,
-                      unsigned *oslot, unsigned *cnt, sph_dev_counters *C);
-void sph_launch_pack_coords(hipStream_t st, int n, const float2 *POS, const int *ID, float max_x,
-                            float max_y, short *out);
-void sph_launch_neighbor_lists(hipStream_t st, int n, const sph_dev_params *P, const float2 *POS,
-                               const unsigned *K, const int *ID, const unsigned *start,
-                               int stride, int *full_by_id, int *half_by_id, int *ids_by_id);
-void sph_launch_pos_of_id(hipStream_t st, int n, const int *ID, int *pos_of_id);
-void sph_launch_permute_by_id(hipStream_t st, int n, const int *id_src, const int *pos_of_id,
+void sph_launch_relax(hipStream_t st, unsigned slots, bool fused, const sph_dev_params *P,
+                      const int *BS, const float2 *POS, const float2 *PREV, const float2 *RHO,
+                      const unsigned *K, const unsigned *start, float2 *opos, float2 *ovel,
+                      unsigned *okey, unsigned *oslot, unsigned *cnt, sph_dev_counters *C);
+void sph_launch_pack_coords(hipStream_t st, unsigned slots, const sph_dev_params *P,
+                            const int *BS, const float2 *POS, const int *ID, short *out);
+void sph_launch_neighbor_lists(hipStream_t st, unsigned slots, const sph_dev_params *P,
+                               const int *BS, const float2 *POS, const unsigned *K, const int *ID,
+                               const unsigned *start, int stride, int *full_by_id,
+                               int *half_by_id, int *ids_by_id);
+void sph_launch_pos_of_id(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                          const int *ID, int *pos_of_id);
+void sph_launch_permute_by_id(hipStream_t st, unsigned slots, const sph_dev_params *P,
+                              const int *BS, const int *id_src, const int *pos_of_id,
                               sph_reorder_fields<2> f);

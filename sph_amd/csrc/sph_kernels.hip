@@ -24,6 +24,12 @@
 // gathers its fields there - the canonical (cell, id) order of
 // src/hash.c:153-163.
 //
+// Several independent scenes can share one launch: each scene owns a
+// 256-aligned range of slots of the sorted arrays and a range of global
+// cells (plus a padding cell that takes the slack of its slot range in the
+// sort), so a workgroup always belongs to exactly one scene and reads that
+// scene's parameters as uniform values (BS maps workgroup -> scene).
+//
 // Workgroups are numbered XCD-aware: the hardware deals workgroups to the
 // 8 XCDs round-robin, so logical tile b is remapped such that each XCD
 // processes one contiguous eighth of the sorted array and its neighbours
@@ -34,7 +40,10 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kTPB = 256;
-constexpr int kTileCap = 1536; // staged candidates per workgroup
+#ifndef SPH_TILE_CAP
+#define SPH_TILE_CAP 1536
+#endif
+constexpr int kTileCap = SPH_TILE_CAP; // staged candidates per workgroup
 constexpr unsigned kXcds = 8;
 
 __device__ __forceinline__ unsigned wave_max_u32(unsigned v)
@@ -86,6 +95,15 @@ __device__ __forceinline__ void report_neighbors(unsigned full, unsigned half,
         raise_max(&C->max_neighbors, m);
 }
 
+// A scene's slot range is padded to whole workgroups; the padding slots are
+// counted into the scene's padding cell by its first workgroup, so the
+// scan places every scene's particles at its fixed slot offset.
+__device__ __forceinline__ void add_pad(const sph_dev_params &P, unsigned s0, unsigned *cnt)
+{
+    if (threadIdx.x == 0 && s0 == P.poff && P.pad)
+        atomicAdd(&cnt[P.cell_base + P.ncells], P.pad);
+}
+
 // ---------------------------------------------------------------------------
 // Tiles: the candidate ranges of one workgroup of sorted particles.
 struct TilePlan {
@@ -102,7 +120,7 @@ __device__ void plan_tile(unsigned s0, unsigned s1, const unsigned *__restrict__
                           TilePlan &T)
 {
     const unsigned sx = P.sx, sy = P.sy;
-    const unsigned kf = K[s0], kl = K[s1 - 1];
+    const unsigned kf = K[s0] - P.cell_base, kl = K[s1 - 1] - P.cell_base;
     const unsigned jA = kf / sx, jB = kl / sx;
     T.jA = jA;
     T.ok = 0;
@@ -125,8 +143,8 @@ __device__ void plan_tile(unsigned s0, unsigned s1, const unsigned *__restrict__
             if (r < 0 || r >= (long)sy)
                 continue;
             const int q = (int)rho * 3 + dy + 1;
-            const unsigned a = start[(unsigned)r * sx + c0];
-            const unsigned b = start[(unsigned)r * sx + c1 + 1];
+            const unsigned a = start[P.cell_base + (unsigned)r * sx + c0];
+            const unsigned b = start[P.cell_base + (unsigned)r * sx + c1 + 1];
             T.g0[q] = a;
             T.len[q] = b - a;
             T.off[q] = (int)total - (int)a;
@@ -162,7 +180,7 @@ struct LdsSrc {
 
 template <class Rec>
 __device__ __forceinline__ LdsSrc<Rec> lds_src(const Rec *tile, const TilePlan &T, unsigned key,
-                                              unsigned sx)
+                                              const sph_dev_params &P)
 {
     LdsSrc<Rec> r;
     r.tile = tile;
@@ -170,7 +188,7 @@ __device__ __forceinline__ LdsSrc<Rec> lds_src(const Rec *tile, const TilePlan &
         r.offA[q] = T.off[q];
         r.offB[q] = T.off[3 + q];
     }
-    r.rowB = key / sx != T.jA;
+    r.rowB = (key - P.cell_base) / P.sx != T.jA;
     return r;
 }
 
@@ -203,20 +221,25 @@ __device__ __forceinline__ GlobalSrc<Rec, Load> global_src(Load l)
 //       post-viscosity velocity is written to out_pos.
 template <bool GRAV, bool VISC, bool PRED>
 __global__ void __launch_bounds__(kTPB)
-k_visc_predict(const sph_dev_params *__restrict__ Pp, const float2 *__restrict__ POS,
-               const float2 *__restrict__ VEL, const unsigned *__restrict__ K,
-               const unsigned *__restrict__ start, float2 *__restrict__ out_pos,
-               unsigned *__restrict__ out_key, unsigned *__restrict__ out_slot,
-               unsigned *__restrict__ cnt, sph_dev_counters *__restrict__ C)
+k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+               const float2 *__restrict__ POS, const float2 *__restrict__ VEL,
+               const unsigned *__restrict__ K, const unsigned *__restrict__ start,
+               float2 *__restrict__ out_pos, unsigned *__restrict__ out_key,
+               unsigned *__restrict__ out_slot, unsigned *__restrict__ cnt,
+               sph_dev_counters *__restrict__ C)
 {
     __shared__ TilePlan T;
     __shared__ float4 tile[VISC ? kTileCap : 1];
-    const sph_dev_params P = *Pp;
-    const unsigned n = (unsigned)P.n;
-    const unsigned s0 = xcd_block() * kTPB;
-    const unsigned s1 = min(s0 + kTPB, n);
+    const unsigned blk = xcd_block();
+    const sph_dev_params P = Pp[BS[blk]];
+    const unsigned s0 = blk * kTPB;
+    const unsigned s1 = min(s0 + kTPB, P.poff + (unsigned)P.n);
     const unsigned s = s0 + threadIdx.x;
     const float gdt = P.g_neg * P.dt;
+    if (PRED)
+        add_pad(P, s0, cnt);
+    if (s0 >= s1)
+        return; // a padding workgroup
     if (VISC) {
         if (threadIdx.x == 0)
             plan_tile(s0, s1, K, start, P, T);
@@ -239,18 +262,18 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const float2 *__restrict__
                 const float dx = b.x - a.x, ddy = b.y - a.y;
                 return (k != s) & !(dx * dx + ddy * ddy > P.h2);
             };
-            auto body = [&](unsigned k, int dy, unsigned b1, unsigned b2, float4 b) {
-                half += sph_owns(k, s, dy, b1, b2);
+            auto body = [&](unsigned k, int dy, unsigned b1, unsigned b2, float4 b, bool use) {
+                half += use && sph_owns(k, s, dy, b1, b2);
                 const float bvy = GRAV ? b.w + gdt : b.w;
                 float ix, iy;
-                if (sph_visc_impulse(a.x, a.y, avx, avy, b.x, b.y, b.z, bvy, P, ix, iy)) {
+                if (sph_visc_impulse(a.x, a.y, avx, avy, b.x, b.y, b.z, bvy, P, ix, iy) && use) {
                     nvx -= ix * 0.5f;
                     nvy -= iy * 0.5f;
                 }
             };
             const unsigned key = K[s];
             if (T.ok)
-                full = sph_walk_masked(key, P, start, lds_src(tile, T, key, P.sx), test, body);
+                full = sph_walk_masked(key, P, start, lds_src(tile, T, key, P), test, body);
             else
                 full = sph_walk_masked(key, P, start, global_src<float4>([&](unsigned g) {
                                            const float2 p = POS[g], v = VEL[g];
@@ -275,24 +298,27 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const float2 *__restrict__
 
 // src/fluid.c:398-413 as a separate pass (phase API).
 __global__ void __launch_bounds__(kTPB)
-k_gravity(const sph_dev_params *__restrict__ Pp, float2 *__restrict__ VEL,
-          float2 *__restrict__ RHO)
+k_gravity(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+          float2 *__restrict__ VEL, float2 *__restrict__ RHO)
 {
-    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= (unsigned)Pp->n)
+    const unsigned blk = xcd_block();
+    const sph_dev_params P = Pp[BS[blk]];
+    const unsigned s = blk * kTPB + threadIdx.x;
+    if (s >= P.poff + (unsigned)P.n)
         return;
-    VEL[s].y += Pp->g_neg * Pp->dt;
+    VEL[s].y += P.g_neg * P.dt;
     RHO[s] = make_float2(0.0f, 0.0f);
 }
 
 // src/fluid.c:507-523 as a separate, in-place pass (phase API).
 __global__ void __launch_bounds__(kTPB)
-k_predict(const sph_dev_params *__restrict__ Pp, float2 *__restrict__ POS,
-          const float2 *__restrict__ VEL, float2 *__restrict__ PREV)
+k_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+          float2 *__restrict__ POS, const float2 *__restrict__ VEL, float2 *__restrict__ PREV)
 {
-    const sph_dev_params P = *Pp;
-    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= (unsigned)P.n)
+    const unsigned blk = xcd_block();
+    const sph_dev_params P = Pp[BS[blk]];
+    const unsigned s = blk * kTPB + threadIdx.x;
+    if (s >= P.poff + (unsigned)P.n)
         return;
     const float2 p = POS[s], v = VEL[s];
     PREV[s] = p;
@@ -304,12 +330,14 @@ k_predict(const sph_dev_params *__restrict__ Pp, float2 *__restrict__ POS,
 
 // src/fluid.c:642-653 as a separate, in-place pass (phase API).
 __global__ void __launch_bounds__(kTPB)
-k_update_velocities(const sph_dev_params *__restrict__ Pp, float2 *__restrict__ POS,
-                    float2 *__restrict__ VEL, const float2 *__restrict__ PREV)
+k_update_velocities(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+                    float2 *__restrict__ POS, float2 *__restrict__ VEL,
+                    const float2 *__restrict__ PREV)
 {
-    const sph_dev_params P = *Pp;
-    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= (unsigned)P.n)
+    const unsigned blk = xcd_block();
+    const sph_dev_params P = Pp[BS[blk]];
+    const unsigned s = blk * kTPB + threadIdx.x;
+    if (s >= P.poff + (unsigned)P.n)
         return;
     float2 p = POS[s];
     const float2 q = PREV[s];
@@ -324,13 +352,16 @@ k_update_velocities(const sph_dev_params *__restrict__ Pp, float2 *__restrict__ 
 // Keys and slots of the current positions (histogram pass of a standalone
 // hash, phase API; the fused step folds this into its producers).
 __global__ void __launch_bounds__(kTPB)
-k_key_hist(const sph_dev_params *__restrict__ Pp, const float2 *__restrict__ POS,
-           unsigned *__restrict__ out_key, unsigned *__restrict__ out_slot,
-           unsigned *__restrict__ cnt, sph_dev_counters *__restrict__ C)
+k_key_hist(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+           const float2 *__restrict__ POS, unsigned *__restrict__ out_key,
+           unsigned *__restrict__ out_slot, unsigned *__restrict__ cnt,
+           sph_dev_counters *__restrict__ C)
 {
-    const sph_dev_params P = *Pp;
-    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= (unsigned)P.n)
+    const unsigned blk = xcd_block();
+    const sph_dev_params P = Pp[BS[blk]];
+    const unsigned s = blk * kTPB + threadIdx.x;
+    add_pad(P, blk * kTPB, cnt);
+    if (s >= P.poff + (unsigned)P.n)
         return;
     const float2 p = POS[s];
     const unsigned key = sph_cell_key(p.x, p.y, P, &C->key_out_of_range);
@@ -348,40 +379,23 @@ constexpr int kScanPerThread = 8;
 constexpr int kScanTile = kScanThreads * kScanPerThread;
 
 __global__ void __launch_bounds__(kScanThreads)
-k_scan_reduce(const unsigned *__restrict__ cnt, unsigned m, unsigned *__restrict__ bsum,
-              sph_dev_counters *__restrict__ C)
+k_scan_reduce(const unsigned *__restrict__ cnt, unsigned m, unsigned *__restrict__ bsum)
 {
-    __shared__ unsigned red[kScanThreads / kWave][3];
+    __shared__ unsigned red[kScanThreads / kWave];
     const unsigned base = blockIdx.x * kScanTile + threadIdx.x * kScanPerThread;
-    unsigned sum = 0, over = 0, mx = 0;
+    unsigned sum = 0;
 #pragma unroll
-    for (int e = 0; e < kScanPerThread; e++) {
-        unsigned c = base + e < m ? cnt[base + e] : 0u;
-        sum += c;
-        over += c > SPH_REF_MAX_BUCKET;
-        mx = max(mx, c);
-    }
+    for (int e = 0; e < kScanPerThread; e++)
+        sum += base + e < m ? cnt[base + e] : 0u;
     sum = wave_sum_u32(sum);
-    over = wave_sum_u32(over);
-    mx = wave_max_u32(mx);
-    const int w = threadIdx.x / kWave;
-    if (lane0()) {
-        red[w][0] = sum;
-        red[w][1] = over;
-        red[w][2] = mx;
-    }
+    if (lane0())
+        red[threadIdx.x / kWave] = sum;
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned t = 0, o = 0, x = 0;
-        for (int i = 0; i < kScanThreads / kWave; i++) {
-            t += red[i][0];
-            o += red[i][1];
-            x = max(x, red[i][2]);
-        }
+        unsigned t = 0;
+        for (int i = 0; i < kScanThreads / kWave; i++)
+            t += red[i];
         bsum[blockIdx.x] = t;
-        if (o)
-            atomicAdd(&C->bucket_overflow, (unsigned long long)o);
-        raise_max(&C->max_cell_count, x);
     }
 }
 
@@ -445,12 +459,15 @@ k_scan_apply(unsigned *__restrict__ cnt, unsigned m, const unsigned *__restrict_
 
 // Drop (id, source index) into the cell ranges in slot order.
 __global__ void __launch_bounds__(kTPB)
-k_scatter(int n, const unsigned *__restrict__ key, const unsigned *__restrict__ slot,
+k_scatter(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+          const unsigned *__restrict__ key, const unsigned *__restrict__ slot,
           const unsigned *__restrict__ start, const int *__restrict__ id_src,
           int *__restrict__ perm_src, int *__restrict__ perm_id)
 {
-    const unsigned i = xcd_block() * kTPB + threadIdx.x;
-    if (i >= (unsigned)n)
+    const unsigned blk = xcd_block();
+    const sph_dev_params &P = Pp[BS[blk]];
+    const unsigned i = blk * kTPB + threadIdx.x;
+    if (i >= P.poff + (unsigned)P.n)
         return;
     const unsigned pos = start[key[i]] + slot[i];
     perm_src[pos] = (int)i;
@@ -459,46 +476,62 @@ k_scatter(int n, const unsigned *__restrict__ key, const unsigned *__restrict__ 
 
 // Rank each particle by id inside its cell and gather its fields to
 // (cell, id) order.  The slot order within a cell is arbitrary (atomics);
-// ranking restores the reference's bucket order (ascending id).
+// ranking restores the reference's bucket order (ascending id).  The rank
+// also feeds the occupancy counters: a particle of rank 100 means its cell
+// holds more than the reference's 100 bucket slots (src/fluid.c:174).
 template <int NF>
 __global__ void __launch_bounds__(kTPB)
-k_reorder(int n, const unsigned *__restrict__ start, const int *__restrict__ perm_src,
+k_reorder(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+          const unsigned *__restrict__ start, const int *__restrict__ perm_src,
           const int *__restrict__ perm_id, const unsigned *__restrict__ key,
-          sph_reorder_fields<NF> f, int *__restrict__ id_dst, unsigned *__restrict__ key_dst)
+          sph_reorder_fields<NF> f, int *__restrict__ id_dst, unsigned *__restrict__ key_dst,
+          sph_dev_counters *__restrict__ C)
 {
-    const unsigned k = xcd_block() * kTPB + threadIdx.x;
-    if (k >= (unsigned)n)
-        return;
-    const int i = perm_src[k];
-    const int myid = perm_id[k];
-    const unsigned c = key[i];
-    const unsigned b = start[c], e = start[c + 1];
-    unsigned rank = 0;
-    for (unsigned m = b; m < e; m++)
-        rank += perm_id[m] < myid;
-    const unsigned d = b + rank;
+    const unsigned blk = xcd_block();
+    const sph_dev_params &P = Pp[BS[blk]];
+    const unsigned k = blk * kTPB + threadIdx.x;
+    unsigned occupancy = 0;
+    if (k < P.poff + (unsigned)P.n) {
+        const int i = perm_src[k];
+        const int myid = perm_id[k];
+        const unsigned c = key[i];
+        const unsigned b = start[c], e = start[c + 1];
+        unsigned rank = 0;
+        for (unsigned m = b; m < e; m++)
+            rank += perm_id[m] < myid;
+        const unsigned d = b + rank;
 #pragma unroll
-    for (int q = 0; q < NF; q++)
-        f.dst[q][d] = f.src[q][i];
-    id_dst[d] = myid;
-    key_dst[d] = c;
+        for (int q = 0; q < NF; q++)
+            f.dst[q][d] = f.src[q][i];
+        id_dst[d] = myid;
+        key_dst[d] = c;
+        if (rank == SPH_REF_MAX_BUCKET)
+            atomicAdd(&C->bucket_overflow, 1ull);
+        occupancy = rank + 1;
+    }
+    const unsigned m = wave_max_u32(occupancy);
+    if (lane0())
+        raise_max(&C->max_cell_count, m);
 }
 
 // ---------------------------------------------------------------------------
 // Density (src/hash.c:190-194, 225-229 with src/fluid.c:527-539) in gather
 // form: each particle sums (1-r/h)^2 and (1-r/h)^3 over its neighbours.
 __global__ void __launch_bounds__(kTPB)
-k_density(const sph_dev_params *__restrict__ Pp, const float2 *__restrict__ POS,
-          const unsigned *__restrict__ K, const unsigned *__restrict__ start,
-          float2 *__restrict__ RHO, sph_dev_counters *__restrict__ C)
+k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+          const float2 *__restrict__ POS, const unsigned *__restrict__ K,
+          const unsigned *__restrict__ start, float2 *__restrict__ RHO,
+          sph_dev_counters *__restrict__ C)
 {
     __shared__ TilePlan T;
     __shared__ float2 tile[kTileCap];
-    const sph_dev_params P = *Pp;
-    const unsigned n = (unsigned)P.n;
-    const unsigned s0 = xcd_block() * kTPB;
-    const unsigned s1 = min(s0 + kTPB, n);
+    const unsigned blk = xcd_block();
+    const sph_dev_params P = Pp[BS[blk]];
+    const unsigned s0 = blk * kTPB;
+    const unsigned s1 = min(s0 + kTPB, P.poff + (unsigned)P.n);
     const unsigned s = s0 + threadIdx.x;
+    if (s0 >= s1)
+        return; // a padding workgroup
     if (threadIdx.x == 0)
         plan_tile(s0, s1, K, start, P, T);
     __syncthreads();
@@ -512,20 +545,20 @@ k_density(const sph_dev_params *__restrict__ Pp, const float2 *__restrict__ POS,
             const float dx = a.x - b.x, ddy = a.y - b.y;
             return (k != s) & !(dx * dx + ddy * ddy > P.h2);
         };
-        auto body = [&](unsigned k, int dy, unsigned b1, unsigned b2, float2 b) {
-            half += sph_owns(k, s, dy, b1, b2);
+        auto body = [&](unsigned k, int dy, unsigned b1, unsigned b2, float2 b, bool use) {
+            half += use && sph_owns(k, s, dy, b1, b2);
             const float dx = a.x - b.x, ddy = a.y - b.y;
             const float r2 = dx * dx + ddy * ddy;
             const float ratio = sph_sqrt(r2) * P.h_recip;
             const float OmR2 = (1.0f - ratio) * (1.0f - ratio);
-            if (ratio < 1.0f) {
+            if (ratio < 1.0f && use) {
                 rho += OmR2;
                 rhon += OmR2 * (1.0f - ratio);
             }
         };
         const unsigned key = K[s];
         if (T.ok)
-            full = sph_walk_masked(key, P, start, lds_src(tile, T, key, P.sx), test, body);
+            full = sph_walk_masked(key, P, start, lds_src(tile, T, key, P), test, body);
         else
             full = sph_walk_masked(key, P, start,
                                    global_src<float2>([&](unsigned g) { return POS[g]; }), test,
@@ -541,20 +574,25 @@ k_density(const sph_dev_params *__restrict__ Pp, const float2 *__restrict__ POS,
 // FUSED only the relaxed positions are written.
 template <bool FUSED>
 __global__ void __launch_bounds__(kTPB)
-k_relax(const sph_dev_params *__restrict__ Pp, const float2 *__restrict__ POS,
-        const float2 *__restrict__ PREV, const float2 *__restrict__ RHO,
-        const unsigned *__restrict__ K, const unsigned *__restrict__ start,
-        float2 *__restrict__ out_pos, float2 *__restrict__ out_vel,
-        unsigned *__restrict__ out_key, unsigned *__restrict__ out_slot,
-        unsigned *__restrict__ cnt, sph_dev_counters *__restrict__ C)
+k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+        const float2 *__restrict__ POS, const float2 *__restrict__ PREV,
+        const float2 *__restrict__ RHO, const unsigned *__restrict__ K,
+        const unsigned *__restrict__ start, float2 *__restrict__ out_pos,
+        float2 *__restrict__ out_vel, unsigned *__restrict__ out_key,
+        unsigned *__restrict__ out_slot, unsigned *__restrict__ cnt,
+        sph_dev_counters *__restrict__ C)
 {
     __shared__ TilePlan T;
     __shared__ float4 tile[kTileCap];
-    const sph_dev_params P = *Pp;
-    const unsigned n = (unsigned)P.n;
-    const unsigned s0 = xcd_block() * kTPB;
-    const unsigned s1 = min(s0 + kTPB, n);
+    const unsigned blk = xcd_block();
+    const sph_dev_params P = Pp[BS[blk]];
+    const unsigned s0 = blk * kTPB;
+    const unsigned s1 = min(s0 + kTPB, P.poff + (unsigned)P.n);
     const unsigned s = s0 + threadIdx.x;
+    if (FUSED)
+        add_pad(P, s0, cnt);
+    if (s0 >= s1)
+        return; // a padding workgroup
     if (threadIdx.x == 0)
         plan_tile(s0, s1, K, start, P, T);
     __syncthreads();
@@ -575,18 +613,18 @@ k_relax(const sph_dev_params *__restrict__ Pp, const float2 *__restrict__ POS,
         const float dx = a.x - b.x, ddy = a.y - b.y;
         return (k != s) & !(dx * dx + ddy * ddy > P.h2);
     };
-    auto body = [&](unsigned k, int dy, unsigned b1, unsigned b2, float4 b) {
+    auto body = [&](unsigned k, int dy, unsigned b1, unsigned b2, float4 b, bool use) {
         const float dx = a.x - b.x, ddy = a.y - b.y;
         const float r2 = dx * dx + ddy * ddy;
         const float rr = sph_sqrt(r2);
         const float r_recip = 1.0f / rr;
         const float ratio = rr * P.h_recip;
         const float OmR = 1.0f - ratio;
-        if (rr <= 0.000001f && sph_owns(k, s, dy, b1, b2)) {
+        if (use && rr <= 0.000001f && sph_owns(k, s, dy, b1, b2)) {
             nx += 0.000001f;
             ny += 0.000001f;
         }
-        if (ratio < 1.0f && rr > 0.0f) {
+        if (use && ratio < 1.0f && rr > 0.0f) {
             const float bP = P.k * (b.z - P.rest_density);
             const float bPn = P.k_near * b.w;
             const float D = sph_relax_D(P.dt, aP + bP, aPn + bPn, OmR, P.k_spring, P.h, rr);
@@ -598,7 +636,7 @@ k_relax(const sph_dev_params *__restrict__ Pp, const float2 *__restrict__ POS,
     };
     const unsigned key = K[s];
     if (T.ok)
-        sph_walk_masked(key, P, start, lds_src(tile, T, key, P.sx), test, body);
+        sph_walk_masked(key, P, start, lds_src(tile, T, key, P), test, body);
     else
         sph_walk_masked(key, P, start, global_src<float4>(load), test, body);
     if (FUSED) {
@@ -619,32 +657,37 @@ k_relax(const sph_dev_params *__restrict__ Pp, const float2 *__restrict__ POS,
 
 // Reference coordinate packing (src/fluid.c:356-363), written by id.
 __global__ void __launch_bounds__(kTPB)
-k_pack_coords(int n, const float2 *__restrict__ POS, const int *__restrict__ ID, float max_x,
-              float max_y, short *__restrict__ out)
+k_pack_coords(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+              const float2 *__restrict__ POS, const int *__restrict__ ID,
+              short *__restrict__ out)
 {
-    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= (unsigned)n)
+    const unsigned blk = xcd_block();
+    const sph_dev_params &P = Pp[BS[blk]];
+    const unsigned s = blk * kTPB + threadIdx.x;
+    if (s >= P.poff + (unsigned)P.n)
         return;
-    const int id = ID[s];
+    const size_t id = P.uoff + (unsigned)ID[s];
     const float2 p = POS[s];
-    out[2 * id] = (short)((2.0f * p.x / max_x - 1.0f) * 32767.0f);
-    out[2 * id + 1] = (short)((2.0f * p.y / max_y - 1.0f) * 32767.0f);
+    out[2 * id] = (short)((2.0f * p.x / P.max_x - 1.0f) * 32767.0f);
+    out[2 * id + 1] = (short)((2.0f * p.y / P.max_y - 1.0f) * 32767.0f);
 }
 
 // Neighbour sets of the current hash, written by id (small-N read-back for
 // tests): full set in canonical order, and the half-list length the
 // reference would hold for each particle.
 __global__ void __launch_bounds__(kTPB)
-k_neighbor_lists(const sph_dev_params *__restrict__ Pp, const float2 *__restrict__ POS,
-                 const unsigned *__restrict__ K, const int *__restrict__ ID,
-                 const unsigned *__restrict__ start, int stride, int *__restrict__ full_by_id,
-                 int *__restrict__ half_by_id, int *__restrict__ ids_by_id)
+k_neighbor_lists(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+                 const float2 *__restrict__ POS, const unsigned *__restrict__ K,
+                 const int *__restrict__ ID, const unsigned *__restrict__ start, int stride,
+                 int *__restrict__ full_by_id, int *__restrict__ half_by_id,
+                 int *__restrict__ ids_by_id)
 {
-    const sph_dev_params P = *Pp;
-    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= (unsigned)P.n)
+    const unsigned blk = xcd_block();
+    const sph_dev_params P = Pp[BS[blk]];
+    const unsigned s = blk * kTPB + threadIdx.x;
+    if (s >= P.poff + (unsigned)P.n)
         return;
-    const int me = ID[s];
+    const int me = (int)P.uoff + ID[s];
     const float2 a = POS[s];
     int full = 0, half = 0;
     sph_walk(s, K[s], P, start, global_src<float2>([&](unsigned g) { return POS[g]; }),
@@ -663,39 +706,47 @@ k_neighbor_lists(const sph_dev_params *__restrict__ Pp, const float2 *__restrict
 
 // dst[pos_of_id[id_src[k]]] = src[k]: re-canonicalise auxiliary fields.
 __global__ void __launch_bounds__(kTPB)
-k_pos_of_id(int n, const int *__restrict__ ID, int *__restrict__ pos_of_id)
+k_pos_of_id(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+            const int *__restrict__ ID, int *__restrict__ pos_of_id)
 {
-    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < (unsigned)n)
-        pos_of_id[ID[s]] = (int)s;
+    const unsigned blk = xcd_block();
+    const sph_dev_params &P = Pp[BS[blk]];
+    const unsigned s = blk * kTPB + threadIdx.x;
+    if (s < P.poff + (unsigned)P.n)
+        pos_of_id[P.uoff + (unsigned)ID[s]] = (int)s;
 }
 
 __global__ void __launch_bounds__(kTPB)
-k_permute_by_id(int n, const int *__restrict__ id_src, const int *__restrict__ pos_of_id,
+k_permute_by_id(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+                const int *__restrict__ id_src, const int *__restrict__ pos_of_id,
                 sph_reorder_fields<2> f)
 {
-    const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (unsigned)n)
+    const unsigned blk = xcd_block();
+    const sph_dev_params &P = Pp[BS[blk]];
+    const unsigned k = blk * kTPB + threadIdx.x;
+    if (k >= P.poff + (unsigned)P.n)
         return;
-    const int d = pos_of_id[id_src[k]];
+    const int d = pos_of_id[P.uoff + (unsigned)id_src[k]];
 #pragma unroll
     for (int q = 0; q < 2; q++)
         f.dst[q][d] = f.src[q][k];
 }
 
 // ---------------------------------------------------------------------------
-// Host-side launchers.
+// Host-side launchers.  `slots` is the padded slot count of all scenes
+// (a multiple of kTPB); BS maps each logical workgroup to its scene.
 
-#define GRID(n) dim3((unsigned)(((n) + kTPB - 1) / kTPB))
+#define GRID(slots) dim3((unsigned)((slots) / kTPB))
 
-void sph_launch_visc_predict(hipStream_t st, int n, bool grav, bool visc, bool pred,
-                             const sph_dev_params *P, const float2 *POS, const float2 *VEL,
-                             const unsigned *K, const unsigned *start, float2 *opos,
-                             unsigned *okey, unsigned *oslot, unsigned *cnt, sph_dev_counters *C)
+void sph_launch_visc_predict(hipStream_t st, unsigned slots, bool grav, bool visc, bool pred,
+                             const sph_dev_params *P, const int *BS, const float2 *POS,
+                             const float2 *VEL, const unsigned *K, const unsigned *start,
+                             float2 *opos, unsigned *okey, unsigned *oslot, unsigned *cnt,
+                             sph_dev_counters *C)
 {
 #define VP(G, V, R)                                                                              \
-    k_visc_predict<G, V, R><<<GRID(n), kTPB, 0, st>>>(P, POS, VEL, K, start, opos, okey, oslot,  \
-                                                      cnt, C)
+    k_visc_predict<G, V, R><<<GRID(slots), kTPB, 0, st>>>(P, BS, POS, VEL, K, start, opos, okey, \
+                                                          oslot, cnt, C)
     if (grav && visc && pred)
         VP(true, true, true);
     else if (grav && !visc && pred)
@@ -711,35 +762,36 @@ void sph_launch_visc_predict(hipStream_t st, int n, bool grav, bool visc, bool p
 #undef VP
 }
 
-void sph_launch_gravity(hipStream_t st, int n, const sph_dev_params *P, float2 *VEL, float2 *RHO)
+void sph_launch_gravity(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                        float2 *VEL, float2 *RHO)
 {
-    k_gravity<<<GRID(n), kTPB, 0, st>>>(P, VEL, RHO);
+    k_gravity<<<GRID(slots), kTPB, 0, st>>>(P, BS, VEL, RHO);
 }
 
-void sph_launch_predict(hipStream_t st, int n, const sph_dev_params *P, float2 *POS,
-                        const float2 *VEL, float2 *PREV)
+void sph_launch_predict(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                        float2 *POS, const float2 *VEL, float2 *PREV)
 {
-    k_predict<<<GRID(n), kTPB, 0, st>>>(P, POS, VEL, PREV);
+    k_predict<<<GRID(slots), kTPB, 0, st>>>(P, BS, POS, VEL, PREV);
 }
 
-void sph_launch_update_velocities(hipStream_t st, int n, const sph_dev_params *P, float2 *POS,
-                                  float2 *VEL, const float2 *PREV)
+void sph_launch_update_velocities(hipStream_t st, unsigned slots, const sph_dev_params *P,
+                                  const int *BS, float2 *POS, float2 *VEL, const float2 *PREV)
 {
-    k_update_velocities<<<GRID(n), kTPB, 0, st>>>(P, POS, VEL, PREV);
+    k_update_velocities<<<GRID(slots), kTPB, 0, st>>>(P, BS, POS, VEL, PREV);
 }
 
-void sph_launch_key_hist(hipStream_t st, int n, const sph_dev_params *P, const float2 *POS,
-                         unsigned *okey, unsigned *oslot, unsigned *cnt, sph_dev_counters *C)
+void sph_launch_key_hist(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                         const float2 *POS, unsigned *okey, unsigned *oslot, unsigned *cnt,
+                         sph_dev_counters *C)
 {
-    k_key_hist<<<GRID(n), kTPB, 0, st>>>(P, POS, okey, oslot, cnt, C);
+    k_key_hist<<<GRID(slots), kTPB, 0, st>>>(P, BS, POS, okey, oslot, cnt, C);
 }
 
 unsigned sph_scan_blocks(unsigned m) { return (m + kScanTile - 1) / kScanTile; }
 
-void sph_launch_scan_reduce(hipStream_t st, const unsigned *cnt, unsigned m, unsigned *bsum,
-                            sph_dev_counters *C)
+void sph_launch_scan_reduce(hipStream_t st, const unsigned *cnt, unsigned m, unsigned *bsum)
 {
-    k_scan_reduce<<<dim3(sph_scan_blocks(m)), kScanThreads, 0, st>>>(cnt, m, bsum, C);
+    k_scan_reduce<<<dim3(sph_scan_blocks(m)), kScanThreads, 0, st>>>(cnt, m, bsum);
 }
 
 void sph_launch_scan_apply(hipStream_t st, unsigned *cnt, unsigned m, const unsigned *bsum,
@@ -748,67 +800,75 @@ void sph_launch_scan_apply(hipStream_t st, unsigned *cnt, unsigned m, const unsi
     k_scan_apply<<<dim3(sph_scan_blocks(m)), kScanThreads, 0, st>>>(cnt, m, bsum, start);
 }
 
-void sph_launch_scatter(hipStream_t st, int n, const unsigned *key, const unsigned *slot,
-                        const unsigned *start, const int *id_src, int *perm_src, int *perm_id)
+void sph_launch_scatter(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                        const unsigned *key, const unsigned *slot, const unsigned *start,
+                        const int *id_src, int *perm_src, int *perm_id)
 {
-    k_scatter<<<GRID(n), kTPB, 0, st>>>(n, key, slot, start, id_src, perm_src, perm_id);
+    k_scatter<<<GRID(slots), kTPB, 0, st>>>(P, BS, key, slot, start, id_src, perm_src, perm_id);
 }
 
-void sph_launch_reorder2(hipStream_t st, int n, const unsigned *start, const int *perm_src,
-                         const int *perm_id, const unsigned *key, sph_reorder_fields<2> f,
-                         int *id_dst, unsigned *key_dst)
+void sph_launch_reorder2(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                         const unsigned *start, const int *perm_src, const int *perm_id,
+                         const unsigned *key, sph_reorder_fields<2> f, int *id_dst,
+                         unsigned *key_dst, sph_dev_counters *C)
 {
-    k_reorder<2><<<GRID(n), kTPB, 0, st>>>(n, start, perm_src, perm_id, key, f, id_dst, key_dst);
+    k_reorder<2><<<GRID(slots), kTPB, 0, st>>>(P, BS, start, perm_src, perm_id, key, f, id_dst,
+                                               key_dst, C);
 }
 
-void sph_launch_reorder4(hipStream_t st, int n, const unsigned *start, const int *perm_src,
-                         const int *perm_id, const unsigned *key, sph_reorder_fields<4> f,
-                         int *id_dst, unsigned *key_dst)
+void sph_launch_reorder4(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                         const unsigned *start, const int *perm_src, const int *perm_id,
+                         const unsigned *key, sph_reorder_fields<4> f, int *id_dst,
+                         unsigned *key_dst, sph_dev_counters *C)
 {
-    k_reorder<4><<<GRID(n), kTPB, 0, st>>>(n, start, perm_src, perm_id, key, f, id_dst, key_dst);
+    k_reorder<4><<<GRID(slots), kTPB, 0, st>>>(P, BS, start, perm_src, perm_id, key, f, id_dst,
+                                               key_dst, C);
 }
 
-void sph_launch_density(hipStream_t st, int n, const sph_dev_params *P, const float2 *POS,
-                        const unsigned *K, const unsigned *start, float2 *RHO,
+void sph_launch_density(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                        const float2 *POS, const unsigned *K, const unsigned *start, float2 *RHO,
                         sph_dev_counters *C)
 {
-    k_density<<<GRID(n), kTPB, 0, st>>>(P, POS, K, start, RHO, C);
+    k_density<<<GRID(slots), kTPB, 0, st>>>(P, BS, POS, K, start, RHO, C);
 }
 
-void sph_launch_relax(hipStream_t st, int n, bool fused, const sph_dev_params *P,
-                      const float2 *POS, const float2 *PREV, const float2 *RHO, const unsigned *K,
-                      const unsigned *start, float2 *opos, float2 *ovel, unsigned *okey,
-                      unsigned *oslot, unsigned *cnt, sph_dev_counters *C)
+void sph_launch_relax(hipStream_t st, unsigned slots, bool fused, const sph_dev_params *P,
+                      const int *BS, const float2 *POS, const float2 *PREV, const float2 *RHO,
+                      const unsigned *K, const unsigned *start, float2 *opos, float2 *ovel,
+                      unsigned *okey, unsigned *oslot, unsigned *cnt, sph_dev_counters *C)
 {
     if (fused)
-        k_relax<true><<<GRID(n), kTPB, 0, st>>>(P, POS, PREV, RHO, K, start, opos, ovel, okey,
-                                                oslot, cnt, C);
+        k_relax<true><<<GRID(slots), kTPB, 0, st>>>(P, BS, POS, PREV, RHO, K, start, opos, ovel,
+                                                    okey, oslot, cnt, C);
     else
-        k_relax<false><<<GRID(n), kTPB, 0, st>>>(P, POS, PREV, RHO, K, start, opos, ovel, okey,
-                                                 oslot, cnt, C);
+        k_relax<false><<<GRID(slots), kTPB, 0, st>>>(P, BS, POS, PREV, RHO, K, start, opos, ovel,
+                                                     okey, oslot, cnt, C);
 }
 
-void sph_launch_pack_coords(hipStream_t st, int n, const float2 *POS, const int *ID, float max_x,
-                            float max_y, short *out)
+void sph_launch_pack_coords(hipStream_t st, unsigned slots, const sph_dev_params *P,
+                            const int *BS, const float2 *POS, const int *ID, short *out)
 {
-    k_pack_coords<<<GRID(n), kTPB, 0, st>>>(n, POS, ID, max_x, max_y, out);
+    k_pack_coords<<<GRID(slots), kTPB, 0, st>>>(P, BS, POS, ID, out);
 }
 
-void sph_launch_neighbor_lists(hipStream_t st, int n, const sph_dev_params *P, const float2 *POS,
-                               const unsigned *K, const int *ID, const unsigned *start,
-                               int stride, int *full_by_id, int *half_by_id, int *ids_by_id)
+void sph_launch_neighbor_lists(hipStream_t st, unsigned slots, const sph_dev_params *P,
+                               const int *BS, const float2 *POS, const unsigned *K, const int *ID,
+                               const unsigned *start, int stride, int *full_by_id,
+                               int *half_by_id, int *ids_by_id)
 {
-    k_neighbor_lists<<<GRID(n), kTPB, 0, st>>>(P, POS, K, ID, start, stride, full_by_id,
-                                               half_by_id, ids_by_id);
+    k_neighbor_lists<<<GRID(slots), kTPB, 0, st>>>(P, BS, POS, K, ID, start, stride, full_by_id,
+                                                   half_by_id, ids_by_id);
 }
 
-void sph_launch_pos_of_id(hipStream_t st, int n, const int *ID, int *pos_of_id)
+void sph_launch_pos_of_id(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                          const int *ID, int *pos_of_id)
 {
-    k_pos_of_id<<<GRID(n), kTPB, 0, st>>>(n, ID, pos_of_id);
+    k_pos_of_id<<<GRID(slots), kTPB, 0, st>>>(P, BS, ID, pos_of_id);
 }
 
-void sph_launch_permute_by_id(hipStream_t st, int n, const int *id_src, const int *pos_of_id,
+void sph_launch_permute_by_id(hipStream_t st, unsigned slots, const sph_dev_params *P,
+                              const int *BS, const int *id_src, const int *pos_of_id,
                               sph_reorder_fields<2> f)
 {
-    k_permute_by_id<<<GRID(n), kTPB, 0, st>>>(n, id_src, pos_of_id, f);
+    k_permute_by_id<<<GRID(slots), kTPB, 0, st>>>(P, BS, id_src, pos_of_id, f);
 }

@@ -1,0 +1,70 @@
+"""Seeded scene generators of the benchmark configurations.
+
+* ``dam_break(nx, ny, box, seed)`` - the plain dam break: an nx x ny lattice
+  at spacing 0.5h in the lower-left of the box (src/geometry.c:29-67 order,
+  id = index), +-0.01h seeded jitter, v = 0.
+* ``batched_scenes(n, seed)`` - the batched configuration: particle counts
+  log-uniform in [2048, 65536], box aspect U[1, 2], fluid fraction
+  U[0.2, 0.5], one of four parameter presets (configs/presets.json) each;
+  box area = N (0.5h)^2 / fraction, W / H = aspect, the fluid a block of the
+  box's aspect in its lower-left corner.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+
+import numpy as np
+
+from . import PARTICLE, make_box, make_tunables, scene_lattice
+
+_CFG = os.path.join(os.path.dirname(os.path.abspath(__file__)), "configs", "presets.json")
+
+
+def presets():
+    with open(_CFG) as f:
+        cfg = json.load(f)
+    out = []
+    for p in cfg["presets"]:
+        kw = dict(cfg["base"])
+        kw.update(p)
+        out.append(kw)
+    return out
+
+
+def preset_tunables(i):
+    return make_tunables(**presets()[i])
+
+
+def dam_break(nx=1024, ny=1024, box=(1200.0, 800.0), seed=12345, spacing=0.5, jitter=0.01):
+    """Tunables, box and particles of the plain dam break (preset 0)."""
+    return preset_tunables(0), make_box(*box), scene_lattice(nx, ny, spacing, jitter, seed=seed)
+
+
+def batched_scenes(nscenes=256, seed=2026, nmin=2048, nmax=65536, spacing=0.5, jitter=0.01):
+    """List of (tunables, box, particles, preset) for the batched config."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(nscenes):
+        n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
+        aspect = float(rng.uniform(1.0, 2.0))
+        frac = float(rng.uniform(0.2, 0.5))
+        preset = int(rng.integers(0, 4))
+        area = n * spacing * spacing / frac
+        w = math.sqrt(area * aspect)
+        h = w / aspect
+        nx = int(math.ceil(math.sqrt(n * aspect)))
+        ny = int(math.ceil(n / nx))
+        parts = scene_lattice(nx, ny, spacing, jitter, seed=seed * 100003 + s,
+                              x0=spacing / 2, y0=spacing / 2)[:n].copy()
+        assert parts["id"][-1] == n - 1
+        out.append((preset_tunables(preset), make_box(np.float32(w), np.float32(h)), parts, preset))
+    return out
+
+
+def concat(scenes):
+    """Concatenated particle array and counts for sph_upload_batch."""
+    parts = np.concatenate([p for _, _, p, _ in scenes]) if scenes else np.zeros(0, PARTICLE)
+    counts = np.array([len(p) for _, _, p, _ in scenes], np.int32)
+    return parts, counts
