@@ -100,6 +100,61 @@ def batched(steps, warmup):
             "counters": s.counters()}
 
 
+def surrogate_leg(step_eq, nscenes=64, n=65536, tol=0.01, seed=11):
+    """Batched inference plus fallback: nscenes sloshing dam breaks of n
+    particles.  The same scenes run step_eq plain steps (batched graph), and
+    step_eq / 4 judged surrogate macro-steps (the plain solver re-takes any
+    scene whose mean density moves by more than tol in a macro-step).  The
+    density error is the per-scene mean density of the final states against
+    the plain run's."""
+    from sph_amd import surrogate as su
+    if not os.path.exists(su.WEIGHTS):
+        return {"skipped": f"no trained weights at {os.path.relpath(su.WEIGHTS, ROOT)}"}
+    scenes = sc.sloshing_dam_breaks(nscenes, n=n, box=(256.0, 192.0), seed=seed)
+    total = nscenes * n
+
+    def final_density(s):
+        s.phase("hash_density")
+        return s.scene_density()[0].astype(np.float64)
+
+    plain = su.make_batch(scenes)
+    plain.step(1)
+    plain.sync()
+    t0 = time.perf_counter()
+    plain.step(step_eq)
+    plain.sync()
+    t_plain = time.perf_counter() - t0
+    rho_plain = final_density(plain)
+    c_plain = plain.counters()
+    plain.close()
+
+    sur = su.make_batch(scenes)
+    sur.step(1)
+    m = sa.Surrogate(seed=0)
+    su.load_weights(m)
+    m.macro_step(sur, tol)  # first macro-step: sets up the judge's buffers
+    n_macro = step_eq // su.MACRO
+    t_sur, flags = su.run_judged(m, sur, n_macro - 1, tol)
+    # time of n_macro - 1 macro-steps; rate per plain-step equivalent
+    rho_sur = final_density(sur)
+    err = np.abs(rho_sur - rho_plain) / rho_plain
+    v_plain = total * step_eq / t_plain
+    v_sur = total * (n_macro - 1) * su.MACRO / t_sur
+    out = {"scenes": nscenes, "particles": total, "plain_step_equivalents": step_eq,
+           "plain": {"value": v_plain, "unit": "particle-steps/s", "seconds": t_plain},
+           "surrogate": {"value": v_sur, "unit": "particle-steps/s", "seconds": t_sur,
+                         "macro_steps": int(n_macro - 1), "tol": tol},
+           "speedup": v_sur / v_plain,
+           "fallback_rate": float(flags.mean()) if flags.size else 0.0,
+           "density_error": {"max": float(err.max()), "mean": float(err.mean()),
+                             "definition": "per-scene mean density of the final state vs the "
+                                           "plain run of the same scenes"},
+           "dropped_edges": m.dropped_edges(), "plain_counters": c_plain}
+    sur.close()
+    m.close()
+    return out
+
+
 def calibration_copy(mbytes):
     """A device-to-device copy of known size through the HIP runtime (a blit
     kernel), run after the timed region: its FETCH_SIZE / WRITE_SIZE counter
@@ -125,6 +180,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched workload line")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate line")
+    ap.add_argument("--surrogate-steps", type=int, default=2000,
+                    help="plain-step equivalents of the surrogate line")
     ap.add_argument("--calib-copy-mb", type=float, default=0.0,
                     help="also run a device copy of this many MB (profiler calibration)")
     args = ap.parse_args()
@@ -187,6 +245,8 @@ def main():
     }
     if not args.no_batched:
         out["batched"] = batched(max(10, args.steps // 2), max(2, args.warmup // 2))
+    if not args.no_surrogate:
+        out["batched_surrogate"] = surrogate_leg(args.surrogate_steps)
     if args.calib_copy_mb > 0:
         calibration_copy(args.calib_copy_mb)
     if not args.no_cpu:

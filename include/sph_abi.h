@@ -42,6 +42,13 @@ extern "C" {
  * 1. Reference records (layout-identical)
  * ---------------------------------------------------------------------- */
 
+/* Included after the reference's own src/fluid.h (include guard
+ * fluid_fluid_h), the records below come from there and are only checked;
+ * otherwise they are defined here with the same layout. */
+#ifdef fluid_fluid_h
+#define SPH_ABI_HAVE_REFERENCE_TYPES 1
+#else
+
 #define SPHERE_MOVER 0    /* src/fluid.h:48 */
 #define RECTANGLE_MOVER 1 /* src/fluid.h:49 */
 
@@ -138,6 +145,8 @@ struct NEIGHBOR_GRID_T {
     unsigned int max_bucket_size;
 };
 
+#endif /* fluid_fluid_h */
+
 SPH_STATIC_ASSERT(sizeof(fluid_particle) == 52, "fluid_particle must be 52 bytes");
 SPH_STATIC_ASSERT(offsetof(fluid_particle, x) == 8, "fluid_particle.x offset");
 SPH_STATIC_ASSERT(offsetof(fluid_particle, v_x) == 16, "fluid_particle.v_x offset");
@@ -215,6 +224,34 @@ int sph_scene_info(const sph_ctx *ctx, int scene, int *n, unsigned *size_x, unsi
 int sph_upload_batch(sph_ctx *ctx, const fluid_particle *particles, const int *counts);
 /* Tunables of one scene (sph_set_tunables sets every scene). */
 int sph_set_scene_tunables(sph_ctx *ctx, int scene, const tunable_parameters *tunables);
+
+/* Extensions with no counterpart in the reference (off by default):
+ * sloshing - a horizontal box acceleration A sin(2 pi t / period), applied
+ * like gravity (v_x += -A sin(2 pi t / period) dt) at the start of every
+ * fused step, t = steps since the last upload times dt;
+ * frozen scenes - flags[s] != 0 makes sph_step leave scene s unchanged
+ * (used to re-take single scenes of a batch). */
+int sph_set_scene_forcing(sph_ctx *ctx, int scene, float amplitude, float period);
+int sph_set_scene_frozen(sph_ctx *ctx, const int *flags);
+/* Sloshing clock offset of one scene: its forcing is evaluated at
+ * t = (steps since the last upload or load + steps) * dt, so a scene loaded
+ * mid-trajectory keeps its phase. */
+int sph_set_scene_time_offset(sph_ctx *ctx, int scene, unsigned steps);
+
+/* State snapshots (no counterpart in the reference; used to re-take
+ * macro-steps).  save copies the whole device state and the step clock;
+ * restore copies back the scenes whose flag is set, and the clock.  Both
+ * need a hashed state (after sph_step or sph_phase_hash). */
+int sph_snapshot_save(sph_ctx *ctx);
+int sph_snapshot_restore(sph_ctx *ctx, const int *scene_flags);
+
+/* Device-resident state exchange (no counterpart in the reference): xv is a
+ * DEVICE pointer to n_total float4 records (x, y, v_x, v_y) in caller order
+ * (scene offset + id), 16-byte aligned.  load replaces positions and
+ * velocities (x_prev = x, ids renumbered in storage order, clock reset, the
+ * state unhashed: call sph_phase_hash before stepping to keep viscosity on). */
+int sph_gather_state_dev(sph_ctx *ctx, float *xv_by_id);
+int sph_load_state_dev(sph_ctx *ctx, const float *xv_by_id);
 
 /* Upload n particles (AoS, reference layout).  Particle ids must be a
  * permutation of 0..n-1; id k is the particle's pointer-array index in the
@@ -334,6 +371,57 @@ int sph_shim_sync(void);                   /* copy device state to the caller no
 int sph_shim_mark_host_dirty(void);        /* re-upload from the caller on next call */
 void sph_shim_release(void);               /* destroy the process-global context */
 sph_ctx *sph_shim_context(void);
+
+/* ------------------------------------------------------------------------
+ * 4. The learned macro-step (no counterpart in the reference).
+ *
+ * A particle-interaction network predicts each particle's displacement over
+ * four plain steps from its neighbourhood (all particles within h):
+ *   edge features f = (dx/h, dy/h, dv_x/5, dv_y/5, r/h)
+ *   edge MLP  5 -> 64 -> 64 (SiLU after the first layer), sum over edges,
+ *   node MLP  [sum, v/5, a/10] (68) -> 64 -> 2 (SiLU),
+ *   d = 4 dt v + 10 dt^2 a + 0.05 out          (a: sloshing and gravity).
+ * BF16 weights and activations on MFMA with FP32 accumulation; FP32 master
+ * weights and Adam.  The parameter vector (sph_surrogate_num_params floats)
+ * is laid out as W1[64][16] (columns 0-4 features, 5 bias, rest zero),
+ * W2[64][64], b2[64], W3[64][80] (columns 0-63 sum, 64-67 node features,
+ * 68 bias, rest zero), W4[2][64], b4[2].
+ * Arrays "by id" are in caller order (scene offset + id), 2 floats each.
+ * ---------------------------------------------------------------------- */
+typedef struct sph_surrogate sph_surrogate;
+
+const char *sph_surrogate_last_error(void);
+int sph_surrogate_num_params(void);
+int sph_surrogate_create(sph_surrogate **m, uint64_t seed);
+void sph_surrogate_destroy(sph_surrogate *m);
+int sph_surrogate_get_params(sph_surrogate *m, float *out);
+int sph_surrogate_set_params(sph_surrogate *m, const float *in);
+int sph_surrogate_get_grads(sph_surrogate *m, float *out);
+int sph_surrogate_zero_grads(sph_surrogate *m);
+/* Forward on the context's hashed state; disp_by_id (host, optional)
+ * receives the predicted displacements. */
+int sph_surrogate_predict(sph_surrogate *m, sph_ctx *ctx, float *disp_by_id);
+/* Forward + loss + backward on the hashed state; target_by_id is a DEVICE
+ * pointer (caller order, 2 floats per particle).  Gradients accumulate;
+ * loss (host, optional) = mean |d - target|^2 / 0.05^2. */
+int sph_surrogate_backward(sph_surrogate *m, sph_ctx *ctx, const float *target_by_id,
+                           float *loss);
+/* Adam step on the accumulated gradients (which it then clears). */
+int sph_surrogate_adam(sph_surrogate *m, float lr, float beta1, float beta2, float eps);
+/* Move every unfrozen scene by the last prediction (x += d with the box
+ * clamp, v = d / (4 dt) + 1.5 a dt with the velocity clamp); the state is
+ * unhashed afterwards and the clock advances by four steps. */
+int sph_surrogate_apply(sph_surrogate *m, sph_ctx *ctx);
+/* One judged macro-step of every scene (see sph_mlp.hip); flags (optional,
+ * nscenes ints) marks the scenes re-taken by four plain steps. */
+int sph_surrogate_macro_step(sph_surrogate *m, sph_ctx *ctx, float tol, int *flags,
+                             int *nfallback);
+unsigned sph_surrogate_dropped_edges(sph_surrogate *m);
+/* D = A B of one 16x16x16 BF16 MFMA with fixed small-integer operands,
+ * row-major (checks the lane layout the kernels assume). */
+int sph_surrogate_mfma_selftest(float *out256);
+/* Per-scene mean and maximum density of the last density pass. */
+int sph_scene_density(sph_ctx *ctx, float *mean, float *maxv);
 
 #ifdef __cplusplus
 }

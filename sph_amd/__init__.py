@@ -18,7 +18,8 @@ import os
 import numpy as np
 
 __all__ = ["PARTICLE", "Tunables", "Param", "AABB", "SceneDesc", "Counters", "Solver", "SphError",
-           "make_tunables", "make_box", "scene_lattice", "lib", "LIB_PATH"]
+           "Surrogate", "mfma_selftest", "make_tunables", "make_box", "scene_lattice", "lib",
+           "LIB_PATH"]
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 # SPH_AMD_LIB may name another build of the library (e.g. a tuning variant).
@@ -96,6 +97,8 @@ def lib():
                                     ctypes.POINTER(ctypes.c_uint)]),
             "sph_upload_batch": (ip, [vp, vp, vp]),
             "sph_set_scene_tunables": (ip, [vp, ip, ctypes.POINTER(Tunables)]),
+            "sph_set_scene_forcing": (ip, [vp, ip, ctypes.c_float, ctypes.c_float]),
+            "sph_set_scene_frozen": (ip, [vp, vp]),
             "sph_upload": (ip, [vp, vp, ip]),
             "sph_set_tunables": (ip, [vp, ctypes.POINTER(Tunables)]),
             "sph_step": (ip, [vp, ip]),
@@ -121,6 +124,28 @@ def lib():
                                       ctypes.POINTER(ip)]),
             "sph_scene_lattice": (ip, [vp, ip, ip, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                        ctypes.c_float, ctypes.c_uint64]),
+            "sph_set_scene_time_offset": (ip, [vp, ip, ctypes.c_uint]),
+            "sph_snapshot_save": (ip, [vp]),
+            "sph_snapshot_restore": (ip, [vp, vp]),
+            "sph_gather_state_dev": (ip, [vp, vp]),
+            "sph_load_state_dev": (ip, [vp, vp]),
+            "sph_scene_density": (ip, [vp, vp, vp]),
+            "sph_surrogate_last_error": (cp, []),
+            "sph_surrogate_num_params": (ip, []),
+            "sph_surrogate_create": (ip, [ctypes.POINTER(vp), ctypes.c_uint64]),
+            "sph_surrogate_destroy": (None, [vp]),
+            "sph_surrogate_get_params": (ip, [vp, vp]),
+            "sph_surrogate_set_params": (ip, [vp, vp]),
+            "sph_surrogate_get_grads": (ip, [vp, vp]),
+            "sph_surrogate_zero_grads": (ip, [vp]),
+            "sph_surrogate_predict": (ip, [vp, vp, vp]),
+            "sph_surrogate_backward": (ip, [vp, vp, vp, ctypes.POINTER(ctypes.c_float)]),
+            "sph_surrogate_adam": (ip, [vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                        ctypes.c_float]),
+            "sph_surrogate_apply": (ip, [vp, vp]),
+            "sph_surrogate_macro_step": (ip, [vp, vp, ctypes.c_float, vp, ctypes.POINTER(ip)]),
+            "sph_surrogate_dropped_edges": (ctypes.c_uint, [vp]),
+            "sph_surrogate_mfma_selftest": (ip, [vp]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -134,6 +159,25 @@ def _check(rc, what):
     if rc != 0:
         msg = lib().sph_last_error().decode(errors="replace")
         raise SphError(f"{what} failed ({rc}): {msg}")
+
+
+def _check_m(rc, what):
+    if rc != 0:
+        msg = lib().sph_surrogate_last_error().decode(errors="replace")
+        raise SphError(f"{what} failed ({rc}): {msg}")
+
+
+def _dev_ptr(t, nfloat4=None):
+    """Device address of a contiguous float32 CUDA tensor (torch is used as the
+    device allocator only)."""
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32
+            and t.is_contiguous()):
+        raise SphError("expected a contiguous float32 tensor on the GPU")
+    if nfloat4 is not None and t.numel() < 4 * nfloat4:
+        raise SphError(f"device buffer holds {t.numel()} floats, {4 * nfloat4} needed")
+    torch.cuda.synchronize(t.device)  # the library works on its own stream
+    return t.data_ptr()
 
 
 def make_tunables(**kw) -> Tunables:
@@ -196,6 +240,41 @@ class Solver:
         p = np.ascontiguousarray(particles, dtype=PARTICLE)
         c = np.ascontiguousarray(counts, dtype=np.int32)
         _check(lib().sph_upload_batch(self._h, p.ctypes.data, c.ctypes.data), "sph_upload_batch")
+
+    def set_scene_forcing(self, scene: int, amplitude: float, period: float):
+        _check(lib().sph_set_scene_forcing(self._h, scene, amplitude, period), "sph_set_scene_forcing")
+
+    def set_scene_time_offset(self, scene: int, steps: int):
+        _check(lib().sph_set_scene_time_offset(self._h, scene, int(steps)),
+               "sph_set_scene_time_offset")
+
+    def snapshot_save(self):
+        _check(lib().sph_snapshot_save(self._h), "sph_snapshot_save")
+
+    def snapshot_restore(self, flags):
+        f = np.ascontiguousarray(flags, dtype=np.int32)
+        _check(lib().sph_snapshot_restore(self._h, f.ctypes.data), "sph_snapshot_restore")
+
+    def gather_state(self, out):
+        """(x, y, v_x, v_y) per particle in caller order into a float32 CUDA
+        tensor of n x 4."""
+        _check(lib().sph_gather_state_dev(self._h, _dev_ptr(out, self.n)), "sph_gather_state_dev")
+
+    def load_state(self, xv):
+        """Replace positions/velocities from a float32 CUDA tensor of n x 4."""
+        _check(lib().sph_load_state_dev(self._h, _dev_ptr(xv, self.n)), "sph_load_state_dev")
+
+    def scene_density(self):
+        ns = self.nscenes
+        mean = np.zeros(ns, np.float32)
+        mx = np.zeros(ns, np.float32)
+        _check_m(lib().sph_scene_density(self._h, mean.ctypes.data, mx.ctypes.data),
+                 "sph_scene_density")
+        return mean, mx
+
+    def set_frozen(self, flags):
+        f = np.ascontiguousarray(flags, dtype=np.int32)
+        _check(lib().sph_set_scene_frozen(self._h, f.ctypes.data), "sph_set_scene_frozen")
 
     def set_scene_tunables(self, scene: int, t: Tunables):
         _check(lib().sph_set_scene_tunables(self._h, scene, ctypes.byref(t)), "sph_set_scene_tunables")
@@ -301,3 +380,77 @@ class Solver:
         _check(lib().sph_time_kernels(self._h, nsteps, K, names, ms, ctypes.byref(nk)),
                "sph_time_kernels")
         return {names[i].decode(): float(ms[i]) for i in range(nk.value)}
+
+
+class Surrogate:
+    """The learned 4-step macro-step (sph_surrogate_*)."""
+
+    def __init__(self, seed: int = 0):
+        self._h = ctypes.c_void_p()
+        _check_m(lib().sph_surrogate_create(ctypes.byref(self._h), seed), "sph_surrogate_create")
+        self.nparams = lib().sph_surrogate_num_params()
+
+    def close(self):
+        if self._h:
+            lib().sph_surrogate_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.nparams, np.float32)
+        _check_m(lib().sph_surrogate_get_params(self._h, out.ctypes.data), "get_params")
+        return out
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparams
+        _check_m(lib().sph_surrogate_set_params(self._h, p.ctypes.data), "set_params")
+
+    def grads(self) -> np.ndarray:
+        out = np.zeros(self.nparams, np.float32)
+        _check_m(lib().sph_surrogate_get_grads(self._h, out.ctypes.data), "get_grads")
+        return out
+
+    def zero_grads(self):
+        _check_m(lib().sph_surrogate_zero_grads(self._h), "zero_grads")
+
+    def predict(self, solver: Solver) -> np.ndarray:
+        out = np.zeros((solver.n, 2), np.float32)
+        _check_m(lib().sph_surrogate_predict(self._h, solver._h, out.ctypes.data), "predict")
+        return out
+
+    def backward(self, solver: Solver, target) -> float:
+        """target: float32 CUDA tensor n x 2 (caller order)."""
+        loss = ctypes.c_float()
+        _check_m(lib().sph_surrogate_backward(self._h, solver._h, _dev_ptr(target),
+                                              ctypes.byref(loss)), "backward")
+        return float(loss.value)
+
+    def adam(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        _check_m(lib().sph_surrogate_adam(self._h, lr, beta1, beta2, eps), "adam")
+
+    def apply(self, solver: Solver):
+        _check_m(lib().sph_surrogate_apply(self._h, solver._h), "apply")
+
+    def macro_step(self, solver: Solver, tol=0.01):
+        flags = np.zeros(solver.nscenes, np.int32)
+        nf = ctypes.c_int()
+        _check_m(lib().sph_surrogate_macro_step(self._h, solver._h, tol, flags.ctypes.data,
+                                                ctypes.byref(nf)), "macro_step")
+        return flags
+
+    def dropped_edges(self) -> int:
+        return int(lib().sph_surrogate_dropped_edges(self._h))
+
+
+def mfma_selftest() -> np.ndarray:
+    out = np.zeros(256, np.float32)
+    _check_m(lib().sph_surrogate_mfma_selftest(out.ctypes.data), "mfma_selftest")
+    return out.reshape(16, 16)

@@ -35,7 +35,7 @@ HIP_FLAGS = [
 ]
 C_FLAGS = ["-O2", "-fPIC", "-std=c11", "-ffp-contract=off", "-fno-fast-math", "-Wall"]
 
-HIP_SRCS = ["sph_kernels.hip", "sph_ctx.hip"]
+HIP_SRCS = ["sph_kernels.hip", "sph_ctx.hip", "sph_mlp.hip"]
 C_SRCS = ["scene.c", "sph_shim.c"]
 
 

@@ -19,7 +19,7 @@
 #include <new>
 #include <vector>
 
-#include "sph_kernels.h"
+#include "sph_ctx_internal.h"
 
 namespace {
 
@@ -49,48 +49,9 @@ const char *kKindNames[NKIND] = {"visc_predict", "scan_reduce", "scan_apply", "s
 
 constexpr int kGraphSteps = 16; // steps per replay of the long graph
 
-struct Scene {
-    tunable_parameters tun{};
-    AABB_t box{};
-    int cap = 0, n = 0;
-    unsigned sx = 0, sy = 0, ncells = 0;
-    unsigned cell_base = 0, poff = 0, slots = 0, uoff = 0;
-};
-
 } // namespace
 
-struct sph_ctx {
-    int device = 0;
-    hipStream_t st = nullptr;
-    std::vector<Scene> sc;
-    unsigned slots = 0;       // padded slots of all scenes
-    unsigned cells = 0;       // global cells incl. one padding cell per scene
-    int n_total = 0;
-    std::vector<sph_dev_params> hp;
-    sph_dev_params *dP = nullptr;
-    int *BS = nullptr; // workgroup -> scene
-    sph_dev_counters *dC = nullptr;
-    // state (canonical: sorted by (cell, id); ID0/K0 give id and key);
-    // pairs interleaved: POS=(x,y) VEL=(v_x,v_y) PREV=(x_prev,y_prev)
-    // RHO=(density,density_near)
-    float2 *POS0 = nullptr, *VEL = nullptr, *PREV = nullptr, *RHO = nullptr;
-    // work buffers
-    float2 *POS1 = nullptr, *UPOS = nullptr, *UVEL = nullptr, *T2a = nullptr, *T2b = nullptr;
-    int *ID0 = nullptr, *ID1 = nullptr, *perm_src = nullptr, *perm_id = nullptr,
-        *pos_of_id = nullptr;
-    unsigned *K0 = nullptr, *K1 = nullptr, *ukey = nullptr, *uslot = nullptr, *cnt = nullptr,
-             *start = nullptr, *bsum = nullptr;
-    short *coords = nullptr;
-    std::vector<float> ax_by_id, ay_by_id; // a_x/a_y: carried, never changed (src/fluid.c)
-    bool lists_valid = false; // a hash ran since upload: K0/start describe POS0
-    int aux_order = 0;        // PREV, RHO ordered like ID0 (0) or ID1 (1)
-    hipGraphExec_t g1 = nullptr, gK = nullptr;
-    uint64_t steps = 0, hashes = 0;
-    // per-kernel timing (only while sph_time_kernels enqueues)
-    std::vector<hipEvent_t> *tev = nullptr;
-    std::vector<int> *tkind = nullptr;
-    size_t tidx = 0;
-};
+using sph_internal::Scene;
 
 namespace {
 
@@ -129,6 +90,10 @@ void derive_params(sph_ctx *c)
         p.uoff = S.uoff;
         p.n = S.n;
         p.pad = S.slots - (unsigned)S.n;
+        p.slosh_amp = S.slosh_amp;
+        p.slosh_omega = S.slosh_omega;
+        p.slosh_t0 = S.slosh_t0;
+        p.frozen = S.frozen;
     }
 }
 
@@ -192,7 +157,8 @@ void enqueue_fused_step(sph_ctx *c, bool visc)
     {
         Tick t(c, KV);
         sph_launch_visc_predict(c->st, S, true, visc, true, c->dP, c->BS, c->POS0, c->VEL, c->K0,
-                                c->start, c->UPOS, c->ukey, c->uslot, c->cnt, c->dC);
+                                c->start, c->UPOS, c->ukey, c->uslot, c->cnt, c->dC, c->dclock,
+                                c->UVEL);
     }
     enqueue_sort_tail(c, c->ID0);
     {
@@ -208,7 +174,7 @@ void enqueue_fused_step(sph_ctx *c, bool visc)
     {
         Tick t(c, KRE);
         sph_launch_relax(c->st, S, true, c->dP, c->BS, c->POS1, c->PREV, c->RHO, c->K1, c->start,
-                         c->UPOS, c->UVEL, c->ukey, c->uslot, c->cnt, c->dC);
+                         c->UPOS, c->UVEL, c->ukey, c->uslot, c->cnt, c->dC, c->dclock);
     }
     enqueue_sort_tail(c, c->ID1);
     {
@@ -363,7 +329,7 @@ int sph_create_batch(sph_ctx **out, const sph_scene_desc *scenes, int nscenes)
     A(ID0, N); A(ID1, N); A(perm_src, N); A(perm_id, N); A(pos_of_id, N);
     A(K0, N); A(K1, N); A(ukey, N); A(uslot, N);
     A(cnt, M); A(start, M); A(bsum, sph_scan_blocks((unsigned)M));
-    A(coords, 2 * N); A(dP, (size_t)nscenes); A(BS, N / kSphTPB); A(dC, 1);
+    A(coords, 2 * N); A(dP, (size_t)nscenes); A(BS, N / kSphTPB); A(dC, 1); A(dclock, 1);
 #undef A
     if (!rc) {
         hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
@@ -382,6 +348,8 @@ int sph_create_batch(sph_ctx **out, const sph_scene_desc *scenes, int nscenes)
             e = hipMemset(c->start, 0, sizeof(unsigned) * M);
         if (e == hipSuccess)
             e = hipMemset(c->dC, 0, sizeof(sph_dev_counters));
+        if (e == hipSuccess)
+            e = hipMemset(c->dclock, 0, sizeof(unsigned long long));
         if (e == hipSuccess)
             e = hipMemset(c->POS0, 0, sizeof(float2) * N);
         if (e != hipSuccess)
@@ -419,7 +387,8 @@ void sph_destroy(sph_ctx *c)
                     c->UVEL, c->T2a,  c->T2b,      c->ID0,     c->ID1,       c->perm_src,
                     c->perm_id, c->pos_of_id, c->K0, c->K1,    c->ukey,      c->uslot,
                     c->cnt,  c->start, c->bsum,    c->coords,  c->dP,        c->BS,
-                    c->dC};
+                    c->dC,   c->dclock, c->SPOS,    c->SVEL,    c->SPREV,     c->SRHO,
+                    c->SID,  c->SK,    c->Sstart,  c->Sclock,  c->dflags};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -510,6 +479,7 @@ static int upload_impl(sph_ctx *c, const fluid_particle *const *pp, const fluid_
     for (int k = 0; k < 4; k++)
         HIPCHK(hipMemcpy(dst[k], f[k].data(), sizeof(float2) * N, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->ID0, ids.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(c->dclock, 0, sizeof(unsigned long long)));
     for (int s = 0; s < ns; s++) {
         c->sc[s].n = counts[s];
         c->sc[s].uoff = uoff[s];
@@ -519,6 +489,7 @@ static int upload_impl(sph_ctx *c, const fluid_particle *const *pp, const fluid_
     c->n_total = (int)total;
     c->lists_valid = false;
     c->aux_order = 0;
+    c->snap_valid = false;
     return push_params(c);
 }
 
@@ -590,6 +561,41 @@ int sph_set_scene_tunables(sph_ctx *c, int scene, const tunable_parameters *t)
     if (scene < 0 || scene >= (int)c->sc.size())
         return set_err(SPH_EINVAL, "sph_set_scene_tunables: scene %d out of range", scene);
     return set_tunables_impl(c, scene, t);
+}
+
+int sph_set_scene_forcing(sph_ctx *c, int scene, float amplitude, float period)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (scene < 0 || scene >= (int)c->sc.size() || !(period > 0.0f) || !(amplitude >= 0.0f))
+        return set_err(SPH_EINVAL, "sph_set_scene_forcing: bad scene, amplitude or period");
+    c->sc[scene].slosh_amp = amplitude;
+    c->sc[scene].slosh_omega = (float)(2.0 * 3.14159265358979323846 / period);
+    HIPCHK(hipStreamSynchronize(c->st));
+    return push_params(c);
+}
+
+int sph_set_scene_time_offset(sph_ctx *c, int scene, unsigned steps)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (scene < 0 || scene >= (int)c->sc.size())
+        return set_err(SPH_EINVAL, "sph_set_scene_time_offset: scene %d out of range", scene);
+    c->sc[scene].slosh_t0 = steps;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return push_params(c);
+}
+
+int sph_set_scene_frozen(sph_ctx *c, const int *frozen)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (!frozen)
+        return set_err(SPH_EINVAL, "sph_set_scene_frozen: null flags");
+    for (size_t s = 0; s < c->sc.size(); s++)
+        c->sc[s].frozen = frozen[s] ? 1 : 0;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return push_params(c);
 }
 
 int sph_step_eager(sph_ctx *c, int nsteps)
@@ -675,7 +681,8 @@ int sph_phase_viscosity(sph_ctx *c)
     if (!c->lists_valid || c->n_total == 0)
         return SPH_OK; // empty lists (src/fluid.c:202)
     sph_launch_visc_predict(c->st, c->slots, false, true, false, c->dP, c->BS, c->POS0, c->VEL,
-                            c->K0, c->start, c->UVEL, nullptr, nullptr, c->cnt, c->dC);
+                            c->K0, c->start, c->UVEL, nullptr, nullptr, c->cnt, c->dC, c->dclock,
+                            nullptr);
     HIPCHK(hipMemcpyAsync(c->VEL, c->UVEL, sizeof(float2) * (size_t)c->slots,
                           hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipGetLastError());
@@ -734,7 +741,7 @@ int sph_phase_relax(sph_ctx *c)
     if (!c->lists_valid)
         return set_err(SPH_ESTATE, "sph_phase_relax: no hash since upload");
     sph_launch_relax(c->st, c->slots, false, c->dP, c->BS, c->POS0, c->PREV, c->RHO, c->K0,
-                     c->start, c->UPOS, nullptr, nullptr, nullptr, nullptr, c->dC);
+                     c->start, c->UPOS, nullptr, nullptr, nullptr, nullptr, c->dC, c->dclock);
     HIPCHK(hipMemcpyAsync(c->POS0, c->UPOS, sizeof(float2) * (size_t)c->slots,
                           hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipGetLastError());
@@ -750,6 +757,110 @@ int sph_phase_update_velocities(sph_ctx *c)
     if (c->n_total)
         sph_launch_update_velocities(c->st, c->slots, c->dP, c->BS, c->POS0, c->VEL, c->PREV);
     HIPCHK(hipGetLastError());
+    return SPH_OK;
+}
+
+// ---- state snapshots (the surrogate's fallback) ---------------------------
+
+int sph_snapshot_save(sph_ctx *c)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (int rc = canonicalize(c))
+        return rc;
+    const size_t N = c->slots, M = (size_t)c->cells + 1;
+    if (!c->SPOS) {
+        int rc = SPH_OK;
+#define A(p, count)                                                                              \
+    if (!rc)                                                                                     \
+    rc = dalloc(&c->p, count)
+        A(SPOS, N); A(SVEL, N); A(SPREV, N); A(SRHO, N); A(SID, N); A(SK, N); A(Sstart, M);
+        A(Sclock, 1); A(dflags, c->sc.size());
+#undef A
+        if (rc)
+            return rc;
+    }
+    const size_t b2 = sizeof(float2) * N, b1 = sizeof(unsigned) * N;
+    HIPCHK(hipMemcpyAsync(c->SPOS, c->POS0, b2, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->SVEL, c->VEL, b2, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->SPREV, c->PREV, b2, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->SRHO, c->RHO, b2, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->SID, c->ID0, b1, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->SK, c->K0, b1, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->Sstart, c->start, sizeof(unsigned) * M, hipMemcpyDeviceToDevice,
+                          c->st));
+    HIPCHK(hipMemcpyAsync(c->Sclock, c->dclock, sizeof(unsigned long long),
+                          hipMemcpyDeviceToDevice, c->st));
+    c->snap_valid = true;
+    c->snap_lists = c->lists_valid;
+    return SPH_OK;
+}
+
+int sph_snapshot_restore(sph_ctx *c, const int *scene_flags)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (!scene_flags)
+        return set_err(SPH_EINVAL, "sph_snapshot_restore: null flags");
+    if (!c->snap_valid)
+        return set_err(SPH_ESTATE, "sph_snapshot_restore: no snapshot");
+    if (int rc = canonicalize(c))
+        return rc;
+    const size_t ns = c->sc.size();
+    bool any = false;
+    for (size_t s = 0; s < ns; s++)
+        any |= scene_flags[s] != 0;
+    if (!any)
+        return SPH_OK;
+    if (!c->snap_lists || !c->lists_valid)
+        return set_err(SPH_ESTATE, "sph_snapshot_restore: snapshot or state not hashed");
+    HIPCHK(hipMemcpyAsync(c->dflags, scene_flags, sizeof(int) * ns, hipMemcpyHostToDevice, c->st));
+    sph_restore_args a{{c->SPOS, c->SVEL, c->SPREV, c->SRHO},
+                       {c->POS0, c->VEL, c->PREV, c->RHO},
+                       c->SID, c->ID0, c->SK, c->Sstart, c->K0, c->start};
+    sph_launch_restore(c->st, c->slots, c->dP, c->BS, c->dflags, a);
+    HIPCHK(hipMemcpyAsync(c->dclock, c->Sclock, sizeof(unsigned long long),
+                          hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipGetLastError());
+    // the host copy of the flags must outlive the asynchronous upload
+    HIPCHK(hipStreamSynchronize(c->st));
+    return SPH_OK;
+}
+
+// ---- device-resident state exchange ----------------------------------------
+
+int sph_gather_state_dev(sph_ctx *c, float *xv_by_id)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (c->n_total > 0 && !xv_by_id)
+        return set_err(SPH_EINVAL, "sph_gather_state_dev: null output");
+    if (c->n_total)
+        sph_launch_gather_state(c->st, c->slots, c->dP, c->BS, c->POS0, c->VEL, c->ID0,
+                                (float4 *)xv_by_id);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st));
+    return SPH_OK;
+}
+
+int sph_load_state_dev(sph_ctx *c, const float *xv_by_id)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (c->n_total > 0 && !xv_by_id)
+        return set_err(SPH_EINVAL, "sph_load_state_dev: null input");
+    if (((uintptr_t)xv_by_id & 15u) != 0)
+        return set_err(SPH_EINVAL, "sph_load_state_dev: input not 16-byte aligned");
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (c->n_total)
+        sph_launch_load_state(c->st, c->slots, c->dP, c->BS, (const float4 *)xv_by_id, c->POS0,
+                              c->VEL, c->PREV, c->RHO, c->ID0);
+    HIPCHK(hipMemsetAsync(c->dclock, 0, sizeof(unsigned long long), c->st));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->lists_valid = false;
+    c->aux_order = 0;
+    c->snap_valid = false;
     return SPH_OK;
 }
 
@@ -1016,3 +1127,5 @@ int sph_time_kernels(sph_ctx *c, int nsteps, int max_kinds, const char **names, 
 }
 
 } // extern "C"
+
+int sph_internal_canonicalize(sph_ctx *c) { return canonicalize(c); }

@@ -36,7 +36,21 @@ struct sph_dev_params {
     unsigned uoff;           // first index of the scene in caller (unpadded) order
     int n;                   // particles of the scene
     unsigned pad;            // padding slots after the scene's particles
+    // Extensions beyond the reference (both off by default):
+    float slosh_amp;   // horizontal box acceleration A sin(omega t) ("sloshing")
+    float slosh_omega; //   applied, like gravity, as v_x += -A sin(omega t) dt
+    unsigned slosh_t0; //   with t = (steps since upload + slosh_t0) * dt
+    int frozen;        // 1: the fused step leaves this scene's state unchanged
 };
+
+// Horizontal box acceleration of a sloshing scene at the step counted by
+// `clock` (0 when the scene does not slosh).
+__device__ __forceinline__ float sph_slosh_accel(const sph_dev_params &P, unsigned long long clock)
+{
+    if (P.slosh_amp == 0.0f)
+        return 0.0f;
+    return -P.slosh_amp * sinf(P.slosh_omega * (float)(clock + P.slosh_t0) * P.dt);
+}
 
 // Device health counters (mirrors sph_counters_t from the key counters on).
 struct sph_dev_counters {

@@ -18,7 +18,8 @@ void sph_launch_visc_predict(hipStream_t st, unsigned slots, bool grav, bool vis
                              const sph_dev_params *P, const int *BS, const float2 *POS,
                              const float2 *VEL, const unsigned *K, const unsigned *start,
                              float2 *opos, unsigned *okey, unsigned *oslot, unsigned *cnt,
-                             sph_dev_counters *C);
+                             sph_dev_counters *C, const unsigned long long *clock,
+                             float2 *held_vel);
 void sph_launch_gravity(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
                         float2 *VEL, float2 *RHO);
 void sph_launch_predict(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
@@ -49,7 +50,8 @@ void sph_launch_density(hipStream_t st, unsigned slots, const sph_dev_params *P,
 void sph_launch_relax(hipStream_t st, unsigned slots, bool fused, const sph_dev_params *P,
                       const int *BS, const float2 *POS, const float2 *PREV, const float2 *RHO,
                       const unsigned *K, const unsigned *start, float2 *opos, float2 *ovel,
-                      unsigned *okey, unsigned *oslot, unsigned *cnt, sph_dev_counters *C);
+                      unsigned *okey, unsigned *oslot, unsigned *cnt, sph_dev_counters *C,
+                      unsigned long long *clock);
 void sph_launch_pack_coords(hipStream_t st, unsigned slots, const sph_dev_params *P,
                             const int *BS, const float2 *POS, const int *ID, short *out);
 void sph_launch_neighbor_lists(hipStream_t st, unsigned slots, const sph_dev_params *P,
@@ -61,3 +63,27 @@ void sph_launch_pos_of_id(hipStream_t st, unsigned slots, const sph_dev_params *
 void sph_launch_permute_by_id(hipStream_t st, unsigned slots, const sph_dev_params *P,
                               const int *BS, const int *id_src, const int *pos_of_id,
                               sph_reorder_fields<2> f);
+
+// Per-scene copy of a state snapshot back into the live arrays: the slot
+// range of every scene whose flag is set, and its cell starts (incl. the
+// scene's padding cell).  F[0..3] = POS, VEL, PREV, RHO.
+struct sph_restore_args {
+    const float2 *src[4];
+    float2 *dst[4];
+    const int *id_src;
+    int *id_dst;
+    const unsigned *key_src, *start_src;
+    unsigned *key_dst, *start_dst;
+};
+void sph_launch_restore(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                        const int *flags, sph_restore_args a);
+
+// Caller-order state exchange with device buffers of float4 (x, y, v_x, v_y)
+// per particle at uoff + id.  Load sets ids to the slot order (id = slot -
+// poff), PREV = POS and RHO = 0.
+void sph_launch_gather_state(hipStream_t st, unsigned slots, const sph_dev_params *P,
+                             const int *BS, const float2 *POS, const float2 *VEL, const int *ID,
+                             float4 *out);
+void sph_launch_load_state(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                           const float4 *in, float2 *POS, float2 *VEL, float2 *PREV, float2 *RHO,
+                           int *ID);

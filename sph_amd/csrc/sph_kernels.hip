@@ -226,7 +226,8 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
                const unsigned *__restrict__ K, const unsigned *__restrict__ start,
                float2 *__restrict__ out_pos, unsigned *__restrict__ out_key,
                unsigned *__restrict__ out_slot, unsigned *__restrict__ cnt,
-               sph_dev_counters *__restrict__ C)
+               sph_dev_counters *__restrict__ C, const unsigned long long *__restrict__ clock,
+               float2 *__restrict__ held_vel)
 {
     __shared__ TilePlan T;
     __shared__ float4 tile[VISC ? kTileCap : 1];
@@ -240,6 +241,23 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
         add_pad(P, s0, cnt);
     if (s0 >= s1)
         return; // a padding workgroup
+    if (PRED && P.frozen) {
+        // frozen scene: same position, same cell; its velocity waits in
+        // held_vel for the relaxation kernel (the sort of a frozen scene is
+        // the identity)
+        if (s < s1) {
+            const float2 a = POS[s];
+            out_pos[s] = a;
+            held_vel[s] = VEL[s];
+            const unsigned key = sph_cell_key(a.x, a.y, P, &C->key_out_of_range);
+            out_key[s] = key;
+            out_slot[s] = atomicAdd(&cnt[key], 1u);
+        }
+        return;
+    }
+    // sloshing: horizontal box acceleration, applied like gravity
+    const bool slosh = GRAV && P.slosh_amp != 0.0f;
+    const float sdt = slosh ? sph_slosh_accel(P, *clock) * P.dt : 0.0f;
     if (VISC) {
         if (threadIdx.x == 0)
             plan_tile(s0, s1, K, start, P, T);
@@ -254,7 +272,7 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
     if (s < s1) {
         const float2 a = POS[s];
         const float2 av = VEL[s];
-        const float avx = av.x;
+        const float avx = slosh ? av.x + sdt : av.x;
         const float avy = GRAV ? av.y + gdt : av.y;
         float nvx = avx, nvy = avy;
         if (VISC) {
@@ -264,9 +282,10 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
             };
             auto body = [&](unsigned k, int dy, unsigned b1, unsigned b2, float4 b, bool use) {
                 half += use && sph_owns(k, s, dy, b1, b2);
+                const float bvx = slosh ? b.z + sdt : b.z;
                 const float bvy = GRAV ? b.w + gdt : b.w;
                 float ix, iy;
-                if (sph_visc_impulse(a.x, a.y, avx, avy, b.x, b.y, b.z, bvy, P, ix, iy) && use) {
+                if (sph_visc_impulse(a.x, a.y, avx, avy, b.x, b.y, bvx, bvy, P, ix, iy) && use) {
                     nvx -= ix * 0.5f;
                     nvy -= iy * 0.5f;
                 }
@@ -530,8 +549,8 @@ k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     const unsigned s0 = blk * kTPB;
     const unsigned s1 = min(s0 + kTPB, P.poff + (unsigned)P.n);
     const unsigned s = s0 + threadIdx.x;
-    if (s0 >= s1)
-        return; // a padding workgroup
+    if (s0 >= s1 || P.frozen)
+        return; // a padding workgroup, or a frozen scene
     if (threadIdx.x == 0)
         plan_tile(s0, s1, K, start, P, T);
     __syncthreads();
@@ -580,7 +599,7 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
         const unsigned *__restrict__ start, float2 *__restrict__ out_pos,
         float2 *__restrict__ out_vel, unsigned *__restrict__ out_key,
         unsigned *__restrict__ out_slot, unsigned *__restrict__ cnt,
-        sph_dev_counters *__restrict__ C)
+        sph_dev_counters *__restrict__ C, unsigned long long *__restrict__ clock)
 {
     __shared__ TilePlan T;
     __shared__ float4 tile[kTileCap];
@@ -589,10 +608,25 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     const unsigned s0 = blk * kTPB;
     const unsigned s1 = min(s0 + kTPB, P.poff + (unsigned)P.n);
     const unsigned s = s0 + threadIdx.x;
-    if (FUSED)
+    if (FUSED) {
         add_pad(P, s0, cnt);
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            atomicAdd(clock, 1ull); // one step of simulated time
+    }
     if (s0 >= s1)
         return; // a padding workgroup
+    if (FUSED && P.frozen) {
+        // frozen scene: keep the position; out_vel already holds the
+        // velocity parked by the prediction kernel at this same slot
+        if (s < s1) {
+            const float2 a = POS[s];
+            out_pos[s] = a;
+            const unsigned nk = sph_cell_key(a.x, a.y, P, &C->key_out_of_range);
+            out_key[s] = nk;
+            out_slot[s] = atomicAdd(&cnt[nk], 1u);
+        }
+        return;
+    }
     if (threadIdx.x == 0)
         plan_tile(s0, s1, K, start, P, T);
     __syncthreads();
@@ -738,15 +772,69 @@ k_permute_by_id(const sph_dev_params *__restrict__ Pp, const int *__restrict__ B
 
 #define GRID(slots) dim3((unsigned)((slots) / kTPB))
 
+__global__ void __launch_bounds__(kTPB)
+k_restore(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+          const int *__restrict__ flags, sph_restore_args a)
+{
+    const unsigned blk = blockIdx.x;
+    const int scene = BS[blk];
+    if (!flags[scene])
+        return;
+    const sph_dev_params &P = Pp[scene];
+    const unsigned s = blk * kTPB + threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        a.dst[q][s] = a.src[q][s];
+    a.id_dst[s] = a.id_src[s];
+    a.key_dst[s] = a.key_src[s];
+    if (blk * kTPB == P.poff)
+        for (unsigned c = threadIdx.x; c <= P.ncells; c += kTPB)
+            a.start_dst[P.cell_base + c] = a.start_src[P.cell_base + c];
+}
+
+__global__ void __launch_bounds__(kTPB)
+k_gather_state(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+               const float2 *__restrict__ POS, const float2 *__restrict__ VEL,
+               const int *__restrict__ ID, float4 *__restrict__ out)
+{
+    const unsigned blk = xcd_block();
+    const sph_dev_params &P = Pp[BS[blk]];
+    const unsigned s = blk * kTPB + threadIdx.x;
+    if (s >= P.poff + (unsigned)P.n)
+        return;
+    const float2 p = POS[s], v = VEL[s];
+    out[P.uoff + (unsigned)ID[s]] = make_float4(p.x, p.y, v.x, v.y);
+}
+
+__global__ void __launch_bounds__(kTPB)
+k_load_state(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+             const float4 *__restrict__ in, float2 *__restrict__ POS, float2 *__restrict__ VEL,
+             float2 *__restrict__ PREV, float2 *__restrict__ RHO, int *__restrict__ ID)
+{
+    const unsigned blk = xcd_block();
+    const sph_dev_params &P = Pp[BS[blk]];
+    const unsigned s = blk * kTPB + threadIdx.x;
+    if (s >= P.poff + (unsigned)P.n)
+        return;
+    const unsigned id = s - P.poff;
+    const float4 q = in[P.uoff + id];
+    POS[s] = make_float2(q.x, q.y);
+    PREV[s] = make_float2(q.x, q.y);
+    VEL[s] = make_float2(q.z, q.w);
+    RHO[s] = make_float2(0.0f, 0.0f);
+    ID[s] = (int)id;
+}
+
 void sph_launch_visc_predict(hipStream_t st, unsigned slots, bool grav, bool visc, bool pred,
                              const sph_dev_params *P, const int *BS, const float2 *POS,
                              const float2 *VEL, const unsigned *K, const unsigned *start,
                              float2 *opos, unsigned *okey, unsigned *oslot, unsigned *cnt,
-                             sph_dev_counters *C)
+                             sph_dev_counters *C, const unsigned long long *clock,
+                             float2 *held_vel)
 {
 #define VP(G, V, R)                                                                              \
     k_visc_predict<G, V, R><<<GRID(slots), kTPB, 0, st>>>(P, BS, POS, VEL, K, start, opos, okey, \
-                                                          oslot, cnt, C)
+                                                          oslot, cnt, C, clock, held_vel)
     if (grav && visc && pred)
         VP(true, true, true);
     else if (grav && !visc && pred)
@@ -835,14 +923,15 @@ void sph_launch_density(hipStream_t st, unsigned slots, const sph_dev_params *P,
 void sph_launch_relax(hipStream_t st, unsigned slots, bool fused, const sph_dev_params *P,
                       const int *BS, const float2 *POS, const float2 *PREV, const float2 *RHO,
                       const unsigned *K, const unsigned *start, float2 *opos, float2 *ovel,
-                      unsigned *okey, unsigned *oslot, unsigned *cnt, sph_dev_counters *C)
+                      unsigned *okey, unsigned *oslot, unsigned *cnt, sph_dev_counters *C,
+                      unsigned long long *clock)
 {
     if (fused)
         k_relax<true><<<GRID(slots), kTPB, 0, st>>>(P, BS, POS, PREV, RHO, K, start, opos, ovel,
-                                                    okey, oslot, cnt, C);
+                                                    okey, oslot, cnt, C, clock);
     else
         k_relax<false><<<GRID(slots), kTPB, 0, st>>>(P, BS, POS, PREV, RHO, K, start, opos, ovel,
-                                                     okey, oslot, cnt, C);
+                                                     okey, oslot, cnt, C, clock);
 }
 
 void sph_launch_pack_coords(hipStream_t st, unsigned slots, const sph_dev_params *P,
@@ -871,4 +960,24 @@ void sph_launch_permute_by_id(hipStream_t st, unsigned slots, const sph_dev_para
                               sph_reorder_fields<2> f)
 {
     k_permute_by_id<<<GRID(slots), kTPB, 0, st>>>(P, BS, id_src, pos_of_id, f);
+}
+
+void sph_launch_restore(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                        const int *flags, sph_restore_args a)
+{
+    k_restore<<<dim3(slots / kTPB), kTPB, 0, st>>>(P, BS, flags, a);
+}
+
+void sph_launch_gather_state(hipStream_t st, unsigned slots, const sph_dev_params *P,
+                             const int *BS, const float2 *POS, const float2 *VEL, const int *ID,
+                             float4 *out)
+{
+    k_gather_state<<<GRID(slots), kTPB, 0, st>>>(P, BS, POS, VEL, ID, out);
+}
+
+void sph_launch_load_state(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                           const float4 *in, float2 *POS, float2 *VEL, float2 *PREV, float2 *RHO,
+                           int *ID)
+{
+    k_load_state<<<GRID(slots), kTPB, 0, st>>>(P, BS, in, POS, VEL, PREV, RHO, ID);
 }

@@ -1,0 +1,1316 @@
+// sph_mlp.hip - the learned macro-step ("surrogate") of libsph_hip.so.
+//
+// A particle-interaction network predicts each particle's displacement over
+// four plain steps from its neighbourhood at the start of the macro-step:
+//
+//   edge features  f_ij = (dx/h, dy/h, dvx/5, dvy/5, r/h, 1)   (j within h)
+//   edge MLP       e_ij = W2 SiLU(W1 f_ij)      (5 -> 64 -> 64, b1 folded
+//                                                into W1 via the constant 1)
+//   aggregation    m_i  = sum_j e_ij = W2 S_i + deg_i b2,  S_i = sum_j SiLU(W1 f_ij)
+//   node MLP       o_i  = W4 SiLU(W3 [m_i, v_i/5, a_i/10, 1]) + b4   (68 -> 64 -> 2)
+//   displacement   d_i  = 4 dt v_i + 10 dt^2 a_i + 0.05 o_i
+//
+// a_i is the external acceleration (sloshing, -g).  The edge MLP's second
+// layer is linear, so it is applied once per particle after the sum: the
+// per-edge work is W1 f_ij on MFMA plus one SiLU per hidden unit.
+//
+// Precision: weights and activations bf16, MFMA 16x16x16 bf16 with fp32
+// accumulation, SiLU / sums / losses / Adam in fp32 (fp32 master weights).
+//
+// Work layout: one wavefront per particle for the edge passes (its
+// neighbours are compacted into LDS, 16 edges per MFMA tile), one wavefront
+// per 16 consecutive particles for the node passes, split-K MFMA reductions
+// for the weight gradients.  Particles are the solver context's sorted slots.
+#include "sph_ctx_internal.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+#include <vector>
+
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+typedef __attribute__((ext_vector_type(4))) short bf16x4;
+
+namespace {
+
+constexpr int kH = 64;      // hidden width
+constexpr int kKE = 16;     // edge input width (6 used)
+constexpr int kKN = 80;     // node input width (64 + 4 + bias, padded)
+constexpr int kEdgeCap = 192; // edges kept per particle (more are counted, dropped)
+constexpr float kVScale = 0.2f, kAScale = 0.1f, kDScale = 0.05f;
+
+// parameter block (fp32 master copy)
+constexpr int oW1 = 0;                  // [64][16]
+constexpr int oW2 = oW1 + kH * kKE;     // [64][64]
+constexpr int ob2 = oW2 + kH * kH;      // [64]
+constexpr int oW3 = ob2 + kH;           // [64][80]
+constexpr int oW4 = oW3 + kH * kKN;     // [2][64]
+constexpr int ob4 = oW4 + 2 * kH;       // [2]
+constexpr int kNParams = ob4 + 2;
+
+__device__ __forceinline__ unsigned short f2bf(float f)
+{
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) // inf / nan: truncate, keep a nan a nan
+        return (unsigned short)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u); // round to nearest even
+    return (unsigned short)(u >> 16);
+}
+
+__device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
+
+__device__ __forceinline__ float silu(float x) { return x / (1.0f + __expf(-x)); }
+
+__device__ __forceinline__ float silu_grad(float x)
+{
+    const float s = 1.0f / (1.0f + __expf(-x));
+    return s * (1.0f + x * (1.0f - s));
+}
+
+__device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d)
+{
+    bf16x4 r;
+    r[0] = (short)f2bf(a);
+    r[1] = (short)f2bf(b);
+    r[2] = (short)f2bf(c);
+    r[3] = (short)f2bf(d);
+    return r;
+}
+
+__device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ unsigned lane() { return threadIdx.x & 63u; }
+
+// Particle-level parameters of a scene needed by the network.
+struct NodeCtx {
+    float h_recip, dt, g_neg, fx;
+};
+
+__device__ __forceinline__ NodeCtx node_ctx(const sph_dev_params &P, unsigned long long clock)
+{
+    NodeCtx c;
+    c.h_recip = P.h_recip;
+    c.dt = P.dt;
+    c.g_neg = P.g_neg;
+    c.fx = sph_slosh_accel(P, clock);
+    return c;
+}
+
+// Compact the neighbours of particle `node` (one wavefront) into the LDS
+// edge buffer, features as bf16 rows of 16.  Returns the number of edges
+// kept (<= kEdgeCap); `dropped` counts the rest.
+__device__ unsigned gather_edges(unsigned node, const sph_dev_params &P, const float2 *POS,
+                                 const float2 *VEL, const unsigned *K, const unsigned *start,
+                                 unsigned short (*ebuf)[kKE], unsigned *dropped)
+{
+    const float2 a = POS[node], av = VEL[node];
+    const unsigned key = K[node] - P.cell_base;
+    const unsigned sx = P.sx, sy = P.sy;
+    const unsigned i = key % sx, j = key / sx;
+    const unsigned i0 = i > 0 ? i - 1 : 0, i1 = i + 1 < sx ? i + 1 : sx - 1;
+    unsigned count = 0;
+    for (int dy = -1; dy <= 1; dy++) {
+        const long r = (long)j + dy;
+        if (r < 0 || r >= (long)sy)
+            continue;
+        const unsigned base = P.cell_base + (unsigned)r * sx;
+        const unsigned b0 = start[base + i0], b3 = start[base + i1 + 1];
+        for (unsigned c = b0; c < b3; c += 64) {
+            const unsigned k = c + lane();
+            bool acc = false;
+            float dx = 0, dyy = 0, r2 = 0;
+            if (k < b3 && k != node) {
+                const float2 q = POS[k];
+                dx = q.x - a.x;
+                dyy = q.y - a.y;
+                r2 = dx * dx + dyy * dyy;
+                acc = !(r2 > P.h2);
+            }
+            const unsigned long long m = __ballot(acc);
+            const unsigned before = (unsigned)__popcll(m & ((1ull << lane()) - 1ull));
+            if (acc) {
+                const unsigned pos = count + before;
+                if (pos < (unsigned)kEdgeCap) {
+                    const float2 qv = VEL[k];
+                    const float f[6] = {dx * P.h_recip, dyy * P.h_recip, (qv.x - av.x) * kVScale,
+                                        (qv.y - av.y) * kVScale, sqrtf(r2) * P.h_recip, 1.0f};
+                    for (int q = 0; q < kKE; q++)
+                        ebuf[pos][q] = q < 6 ? f2bf(f[q]) : (unsigned short)0;
+                }
+            }
+            count += (unsigned)__popcll(m);
+        }
+    }
+    if (count > (unsigned)kEdgeCap) {
+        if (lane() == 0)
+            atomicAdd(dropped, count - (unsigned)kEdgeCap);
+        count = kEdgeCap;
+    }
+    // zero the rows that pad the last tile
+    const unsigned padded = (count + 15u) & ~15u;
+    for (unsigned e = count + lane() / 16u; e < padded; e += 4)
+        ebuf[e][lane() % 16u] = 0;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    return count;
+}
+
+// A-operand of an edge tile: lane l holds row (edge) e0 + l%16, features
+// 4*(l/16) .. +3.
+__device__ __forceinline__ bf16x4 edge_tile_a(const unsigned short (*ebuf)[kKE], unsigned e0)
+{
+    const unsigned e = e0 + lane() % 16u, k = 4u * (lane() / 16u);
+    bf16x4 r;
+    for (int q = 0; q < 4; q++)
+        r[q] = (short)ebuf[e][k + q];
+    return r;
+}
+
+// B-operand of W1^T for hidden-unit tile t: B[k][c] = W1[16t + c][k].
+__device__ __forceinline__ bf16x4 w1_b(const unsigned short *W1b, int t)
+{
+    const unsigned c = 16u * t + lane() % 16u, k = 4u * (lane() / 16u);
+    bf16x4 r;
+    for (int q = 0; q < 4; q++)
+        r[q] = (short)W1b[c * kKE + k + q];
+    return r;
+}
+
+} // namespace
+
+struct sph_surrogate {
+    size_t slots = 0;
+    float *params = nullptr, *grads = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+    unsigned short *W1b = nullptr, *W2b = nullptr, *W3b = nullptr;
+    float *S = nullptr, *deg = nullptr, *dS = nullptr, *disp = nullptr, *tgt = nullptr;
+    unsigned short *U = nullptr, *dP3 = nullptr, *dM = nullptr; // tile-major bf16
+    float *loss = nullptr;
+    unsigned *dropped = nullptr;
+    float2 *tmp2 = nullptr;
+    int adam_t = 0;
+    bool have_pred = false;
+    // judge state (per scene)
+    size_t ns = 0;
+    float *ref = nullptr, *jsum = nullptr;
+    unsigned *jmax = nullptr;
+    int *jflags = nullptr;
+    const sph_ctx *ref_ctx = nullptr;
+    bool ref_ok = false;
+};
+
+// ---------------------------------------------------------------------------
+// kernels
+
+// MFMA lane-layout self-test: D = A B for fixed small-integer A, B
+// (exact in bf16), written row-major as the layout assumed above reads it.
+__global__ void k_mfma_selftest(float *D)
+{
+    const unsigned l = lane();
+    bf16x4 a, b;
+    for (int q = 0; q < 4; q++) {
+        const unsigned row = l % 16u, k = 4u * (l / 16u) + q;
+        a[q] = (short)f2bf((float)((int)((row * 16u + k) % 7u) - 3));
+        const unsigned kk = 4u * (l / 16u) + q, col = l % 16u;
+        b[q] = (short)f2bf((float)((int)((kk * 3u + col) % 5u) - 2));
+    }
+    f32x4 d = {0, 0, 0, 0};
+    d = mfma(a, b, d);
+    for (int q = 0; q < 4; q++)
+        D[(4u * (l / 16u) + q) * 16u + l % 16u] = d[q];
+}
+
+// Edge forward: S_i = sum_j SiLU(W1 f_ij), deg_i.  One wavefront per particle.
+__global__ void __launch_bounds__(256)
+k_edge_fwd(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS, unsigned slots,
+           const float2 *__restrict__ POS, const float2 *__restrict__ VEL,
+           const unsigned *__restrict__ K, const unsigned *__restrict__ start,
+           const unsigned short *__restrict__ W1b, float *__restrict__ S,
+           float *__restrict__ deg, unsigned *__restrict__ dropped)
+{
+    __shared__ unsigned short ebuf[4][kEdgeCap][kKE];
+    const unsigned w = threadIdx.x / 64u;
+    bf16x4 B[4];
+    for (int t = 0; t < 4; t++)
+        B[t] = w1_b(W1b, t);
+    const unsigned nw = gridDim.x * 4u;
+    for (unsigned node = blockIdx.x * 4u + w; node < slots; node += nw) {
+        const sph_dev_params &P = Pp[BS[node / kSphTPB]];
+        if (node >= P.poff + (unsigned)P.n) { // padding slot: zero rows for the node passes
+            if (lane() < 16u)
+                for (int t = 0; t < 4; t++)
+                    S[(size_t)node * kH + 16u * t + lane()] = 0.0f;
+            if (lane() == 0)
+                deg[node] = 0.0f;
+            continue;
+        }
+        const unsigned ne = gather_edges(node, P, POS, VEL, K, start, ebuf[w], dropped);
+        float acc[4] = {0, 0, 0, 0};
+        for (unsigned e0 = 0; e0 < ne; e0 += 16) {
+            const bf16x4 a = edge_tile_a(ebuf[w], e0);
+            for (int t = 0; t < 4; t++) {
+                f32x4 d = {0, 0, 0, 0};
+                d = mfma(a, B[t], d);
+                for (int q = 0; q < 4; q++) {
+                    const unsigned e = e0 + 4u * (lane() / 16u) + q;
+                    acc[t] += e < ne ? silu(d[q]) : 0.0f;
+                }
+            }
+        }
+        for (int t = 0; t < 4; t++) {
+            acc[t] += __shfl_xor(acc[t], 16, 64);
+            acc[t] += __shfl_xor(acc[t], 32, 64);
+        }
+        if (lane() < 16u)
+            for (int t = 0; t < 4; t++)
+                S[(size_t)node * kH + 16u * t + lane()] = acc[t];
+        if (lane() == 0)
+            deg[node] = (float)ne;
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+namespace {
+
+// Node-pass state of one 16-particle tile held by one wavefront.
+struct NodeTile {
+    f32x4 pre3[4]; // D layout: row 4*(l/16)+q, unit 16t + l%16
+    float out[4][2];
+};
+
+// Build u = [m, feats, 1, 0..] for a tile in LDS (bf16 [16][80]) and run the
+// node forward.  Rows of invalid slots are zero.
+__device__ void node_forward(unsigned n0, const sph_dev_params &P, unsigned long long clock,
+                             const float2 *VEL, const float *S, const float *deg,
+                             const unsigned short *W2b, const float *params,
+                             const unsigned short *W3b, unsigned short (*ul)[kKN],
+                             NodeTile &T)
+{
+    const unsigned l = lane();
+    const unsigned nvalid_end = P.poff + (unsigned)P.n;
+    const NodeCtx nc = node_ctx(P, clock);
+    // m = S W2^T + deg b2
+    bf16x4 A[4];
+    {
+        const unsigned row = n0 + l % 16u;
+        const bool ok = row < nvalid_end;
+        for (int kc = 0; kc < 4; kc++) {
+            const unsigned k = 16u * kc + 4u * (l / 16u);
+            if (ok) {
+                const float *sp = S + (size_t)row * kH + k;
+                A[kc] = pack4(sp[0], sp[1], sp[2], sp[3]);
+            } else {
+                A[kc] = pack4(0, 0, 0, 0);
+            }
+        }
+    }
+    for (int t = 0; t < 4; t++) {
+        f32x4 d = {0, 0, 0, 0};
+        for (int kc = 0; kc < 4; kc++) {
+            bf16x4 b; // B[k][c] = W2[16t + c][k]
+            const unsigned c = 16u * t + l % 16u, k = 16u * kc + 4u * (l / 16u);
+            for (int q = 0; q < 4; q++)
+                b[q] = (short)W2b[c * kH + k + q];
+            d = mfma(A[kc], b, d);
+        }
+        const float b2 = params[ob2 + 16u * t + l % 16u];
+        for (int q = 0; q < 4; q++) {
+            const unsigned r = 4u * (l / 16u) + q;
+            const unsigned row = n0 + r;
+            const float m = row < nvalid_end ? d[q] + deg[row] * b2 : 0.0f;
+            ul[r][16u * t + l % 16u] = f2bf(m);
+        }
+    }
+    // node features, bias column, padding
+    if (l < 16u) {
+        const unsigned row = n0 + l;
+        const bool ok = row < nvalid_end;
+        const float2 v = ok ? VEL[row] : make_float2(0, 0);
+        ul[l][64] = f2bf(v.x * kVScale);
+        ul[l][65] = f2bf(v.y * kVScale);
+        ul[l][66] = f2bf(ok ? nc.fx * kAScale : 0.0f);
+        ul[l][67] = f2bf(ok ? nc.g_neg * kAScale : 0.0f);
+        ul[l][68] = f2bf(ok ? 1.0f : 0.0f);
+        for (int q = 69; q < kKN; q++)
+            ul[l][q] = 0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    // pre3 = u W3^T
+    bf16x4 Au[5];
+    for (int kc = 0; kc < 5; kc++) {
+        const unsigned k = 16u * kc + 4u * (l / 16u);
+        for (int q = 0; q < 4; q++)
+            Au[kc][q] = (short)ul[l % 16u][k + q];
+    }
+    for (int t = 0; t < 4; t++) {
+        f32x4 d = {0, 0, 0, 0};
+        for (int kc = 0; kc < 5; kc++) {
+            bf16x4 b; // B[k][c] = W3[16t + c][k]
+            const unsigned c = 16u * t + l % 16u, k = 16u * kc + 4u * (l / 16u);
+            for (int q = 0; q < 4; q++)
+                b[q] = (short)W3b[c * kKN + k + q];
+            d = mfma(Au[kc], b, d);
+        }
+        T.pre3[t] = d;
+    }
+    // out = W4 SiLU(pre3) + b4, reduced over the 16 lanes of each row group
+    for (int q = 0; q < 4; q++)
+        for (int o = 0; o < 2; o++) {
+            float p = 0.0f;
+            for (int t = 0; t < 4; t++)
+                p += silu(T.pre3[t][q]) * params[oW4 + o * kH + 16u * t + l % 16u];
+            for (int x = 1; x < 16; x <<= 1)
+                p += __shfl_xor(p, x, 64);
+            T.out[q][o] = p + params[ob4 + o];
+        }
+}
+
+__device__ __forceinline__ float2 ballistic(const NodeCtx &nc, float2 v)
+{
+    // four steps of v += a dt; x += v dt from rest of interactions
+    return make_float2(4.0f * nc.dt * v.x + 10.0f * nc.dt * nc.dt * nc.fx,
+                       4.0f * nc.dt * v.y + 10.0f * nc.dt * nc.dt * nc.g_neg);
+}
+
+} // namespace
+
+// Node forward: predicted 4-step displacement per particle (slot order).
+__global__ void __launch_bounds__(256)
+k_node_fwd(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS, unsigned slots,
+           const unsigned long long *__restrict__ clock, const float2 *__restrict__ VEL,
+           const float *__restrict__ S, const float *__restrict__ deg,
+           const unsigned short *__restrict__ W2b, const float *__restrict__ params,
+           const unsigned short *__restrict__ W3b, float2 *__restrict__ disp)
+{
+    __shared__ unsigned short ul[4][16][kKN];
+    const unsigned w = threadIdx.x / 64u, l = lane();
+    const unsigned ntiles = slots / 16u, nw = gridDim.x * 4u;
+    const unsigned long long ck = *clock;
+    for (unsigned tile = blockIdx.x * 4u + w; tile < ntiles; tile += nw) {
+        const unsigned n0 = tile * 16u;
+        const sph_dev_params &P = Pp[BS[n0 / kSphTPB]];
+        if (n0 >= P.poff + (unsigned)P.n)
+            continue;
+        NodeTile T;
+        node_forward(n0, P, ck, VEL, S, deg, W2b, params, W3b, ul[w], T);
+        const NodeCtx nc = node_ctx(P, ck);
+        if (l % 16u == 0)
+            for (int q = 0; q < 4; q++) {
+                const unsigned row = n0 + 4u * (l / 16u) + q;
+                if (row < P.poff + (unsigned)P.n) {
+                    const float2 b = ballistic(nc, VEL[row]);
+                    disp[row] = make_float2(b.x + kDScale * T.out[q][0], b.y + kDScale * T.out[q][1]);
+                }
+            }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Node backward: loss, dout, dW4/db4 (+ atomics), and per particle dS (fp32),
+// dPre3, dM, U (tile-major bf16 [tile][col][16]) for the weight-gradient
+// reduction.  tgt holds the target displacement per slot.
+__global__ void __launch_bounds__(256)
+k_node_bwd(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS, unsigned slots,
+           const unsigned long long *__restrict__ clock, const float2 *__restrict__ VEL,
+           const float *__restrict__ S, const float *__restrict__ deg,
+           const unsigned short *__restrict__ W2b, const float *__restrict__ params,
+           const unsigned short *__restrict__ W3b, const float2 *__restrict__ tgt, float inv_n,
+           float *__restrict__ dS, unsigned short *__restrict__ U,
+           unsigned short *__restrict__ dP3, unsigned short *__restrict__ dM,
+           float *__restrict__ grads, float *__restrict__ loss)
+{
+    __shared__ unsigned short ul[4][16][kKN];
+    __shared__ unsigned short dl[4][16][kH];
+    __shared__ float du_l[4][16][kH];
+    const unsigned w = threadIdx.x / 64u, l = lane();
+    const unsigned ntiles = slots / 16u, nw = gridDim.x * 4u;
+    const unsigned long long ck = *clock;
+    float gW4[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    float gb4[2] = {0, 0}, gb2[4] = {0, 0, 0, 0}, lsum = 0.0f;
+    for (unsigned tile = blockIdx.x * 4u + w; tile < ntiles; tile += nw) {
+        const unsigned n0 = tile * 16u;
+        const sph_dev_params &P = Pp[BS[n0 / kSphTPB]];
+        const unsigned vend = P.poff + (unsigned)P.n;
+        unsigned short *Ut = U + (size_t)tile * kKN * 16u;
+        unsigned short *Pt = dP3 + (size_t)tile * kH * 16u;
+        unsigned short *Mt = dM + (size_t)tile * kH * 16u;
+        if (n0 >= vend) { // padding tile: zero operands for the reduction
+            for (unsigned x = l; x < kKN * 16u; x += 64)
+                Ut[x] = 0;
+            for (unsigned x = l; x < kH * 16u; x += 64) {
+                Pt[x] = 0;
+                Mt[x] = 0;
+            }
+            continue;
+        }
+        NodeTile T;
+        node_forward(n0, P, ck, VEL, S, deg, W2b, params, W3b, ul[w], T);
+        const NodeCtx nc = node_ctx(P, ck);
+        // dout for the 4 rows of this lane group
+        float dout[4][2];
+        for (int q = 0; q < 4; q++) {
+            const unsigned row = n0 + 4u * (l / 16u) + q;
+            const bool ok = row < vend;
+            float2 pred = make_float2(0, 0), y = make_float2(0, 0);
+            if (ok) {
+                const float2 b = ballistic(nc, VEL[row]);
+                pred = make_float2(b.x + kDScale * T.out[q][0], b.y + kDScale * T.out[q][1]);
+                y = tgt[row];
+            }
+            // loss = mean over particles of |d - y|^2 / kDScale^2
+            const float ex = (pred.x - y.x) / kDScale, ey = (pred.y - y.y) / kDScale;
+            dout[q][0] = ok ? 2.0f * ex * inv_n : 0.0f;
+            dout[q][1] = ok ? 2.0f * ey * inv_n : 0.0f;
+            if (ok && l % 16u == 0)
+                lsum += (ex * ex + ey * ey) * inv_n;
+        }
+        // dW4, db4, dz, dpre3
+        for (int t = 0; t < 4; t++) {
+            const unsigned c = 16u * t + l % 16u;
+            for (int q = 0; q < 4; q++) {
+                const float x = T.pre3[t][q];
+                const float z = silu(x);
+                gW4[0][t] += dout[q][0] * z;
+                gW4[1][t] += dout[q][1] * z;
+                const float dz = dout[q][0] * params[oW4 + c] + dout[q][1] * params[oW4 + kH + c];
+                const float dp = dz * silu_grad(x);
+                dl[w][4u * (l / 16u) + q][c] = f2bf(dp);
+            }
+            // tile-major dPre3: [unit][16 rows]
+            bf16x4 v;
+            for (int q = 0; q < 4; q++)
+                v[q] = (short)dl[w][4u * (l / 16u) + q][c];
+            *(bf16x4 *)(Pt + (size_t)c * 16u + 4u * (l / 16u)) = v;
+        }
+        if (l % 16u == 0)
+            for (int q = 0; q < 4; q++) {
+                gb4[0] += dout[q][0];
+                gb4[1] += dout[q][1];
+            }
+        // U tile-major [col][16 rows]
+        for (unsigned x = l; x < kKN * 4u; x += 64) {
+            const unsigned col = x / 4u, r0 = 4u * (x % 4u);
+            bf16x4 v;
+            for (int q = 0; q < 4; q++)
+                v[q] = (short)ul[w][r0 + q][col];
+            *(bf16x4 *)(Ut + (size_t)col * 16u + r0) = v;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        // du = dpre3 W3  (16 x 80); only the first 64 columns (dm) are needed
+        bf16x4 Ad[4];
+        for (int kc = 0; kc < 4; kc++)
+            for (int q = 0; q < 4; q++)
+                Ad[kc][q] = (short)dl[w][l % 16u][16u * kc + 4u * (l / 16u) + q];
+        for (int t = 0; t < 4; t++) {
+            f32x4 d = {0, 0, 0, 0};
+            for (int kc = 0; kc < 4; kc++) {
+                bf16x4 b; // B[k][c] = W3[k][16t + c]  (k: hidden unit)
+                const unsigned c = 16u * t + l % 16u, k = 16u * kc + 4u * (l / 16u);
+                for (int q = 0; q < 4; q++)
+                    b[q] = (short)W3b[(k + q) * kKN + c];
+                d = mfma(Ad[kc], b, d);
+            }
+            for (int q = 0; q < 4; q++) {
+                const unsigned r = 4u * (l / 16u) + q;
+                du_l[w][r][16u * t + l % 16u] = n0 + r < vend ? d[q] : 0.0f;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        // dM tile-major, db2 partials, dS = dm W2
+        for (int t = 0; t < 4; t++) {
+            const unsigned c = 16u * t + l % 16u;
+            bf16x4 v;
+            for (int q = 0; q < 4; q++) {
+                const unsigned r = 4u * (l / 16u) + q;
+                const float dm = du_l[w][r][c];
+                v[q] = (short)f2bf(dm);
+                if (n0 + r < vend)
+                    gb2[t] += deg[n0 + r] * dm;
+            }
+            *(bf16x4 *)(Mt + (size_t)c * 16u + 4u * (l / 16u)) = v;
+        }
+        bf16x4 Am[4];
+        for (int kc = 0; kc < 4; kc++)
+            for (int q = 0; q < 4; q++)
+                Am[kc][q] = (short)f2bf(du_l[w][l % 16u][16u * kc + 4u * (l / 16u) + q]);
+        for (int t = 0; t < 4; t++) {
+            f32x4 d = {0, 0, 0, 0};
+            for (int kc = 0; kc < 4; kc++) {
+                bf16x4 b; // B[k][c] = W2[k][16t + c]  (k: output unit of layer 2)
+                const unsigned c = 16u * t + l % 16u, k = 16u * kc + 4u * (l / 16u);
+                for (int q = 0; q < 4; q++)
+                    b[q] = (short)W2b[(k + q) * kH + c];
+                d = mfma(Am[kc], b, d);
+            }
+            for (int q = 0; q < 4; q++) {
+                const unsigned row = n0 + 4u * (l / 16u) + q;
+                if (row < vend)
+                    dS[(size_t)row * kH + 16u * t + l % 16u] = d[q];
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    // dW4 partials: sum over the 4 row groups
+    for (int o = 0; o < 2; o++)
+        for (int t = 0; t < 4; t++) {
+            float v = gW4[o][t];
+            v += __shfl_xor(v, 16, 64);
+            v += __shfl_xor(v, 32, 64);
+            if (l < 16u)
+                atomicAdd(&grads[oW4 + o * kH + 16u * t + l], v);
+        }
+    for (int t = 0; t < 4; t++) {
+        float v = gb2[t];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        if (l < 16u)
+            atomicAdd(&grads[ob2 + 16u * t + l], v);
+    }
+    if (l % 16u == 0) {
+        atomicAdd(&grads[ob4], gb4[0]);
+        atomicAdd(&grads[ob4 + 1], gb4[1]);
+        atomicAdd(loss, lsum);
+    }
+}
+
+// Weight gradients of the node layers by split-K MFMA over particle tiles:
+// dW3 (64x80) = sum dPre3^T U, dW2 (64x64) = sum dM^T S.  36 output tiles
+// of 16x16; wave w of a workgroup owns output tiles w, w+4, ...
+__global__ void __launch_bounds__(256)
+k_node_wgrad(unsigned ntiles, const unsigned short *__restrict__ U,
+             const unsigned short *__restrict__ dP3, const unsigned short *__restrict__ dM,
+             const float *__restrict__ S, float *__restrict__ grads)
+{
+    const unsigned w = threadIdx.x / 64u, l = lane();
+    f32x4 acc[9];
+    for (int i = 0; i < 9; i++)
+        acc[i] = {0, 0, 0, 0};
+    for (unsigned tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const unsigned short *Ut = U + (size_t)tile * kKN * 16u;
+        const unsigned short *Pt = dP3 + (size_t)tile * kH * 16u;
+        const unsigned short *Mt = dM + (size_t)tile * kH * 16u;
+        for (int i = 0; i < 9; i++) {
+            const unsigned ot = w + 4u * i; // 0..35
+            const unsigned mt = ot < 20u ? ot / 5u : (ot - 20u) / 4u;
+            const unsigned nt = ot < 20u ? ot % 5u : (ot - 20u) % 4u;
+            // A[r][k] = dX[node k][unit 16mt + r]; B[k][c] = Y[node k][col 16nt + c]
+            const unsigned short *Ab = ot < 20u ? Pt : Mt;
+            const bf16x4 a = *(const bf16x4 *)(Ab + (size_t)(16u * mt + l % 16u) * 16u + 4u * (l / 16u));
+            bf16x4 b;
+            if (ot < 20u) {
+                b = *(const bf16x4 *)(Ut + (size_t)(16u * nt + l % 16u) * 16u + 4u * (l / 16u));
+            } else {
+                const unsigned col = 16u * nt + l % 16u;
+                const size_t n = (size_t)tile * 16u + 4u * (l / 16u);
+                b = pack4(S[n * kH + col], S[(n + 1) * kH + col], S[(n + 2) * kH + col],
+                          S[(n + 3) * kH + col]);
+            }
+            acc[i] = mfma(a, b, acc[i]);
+        }
+    }
+    for (int i = 0; i < 9; i++) {
+        const unsigned ot = w + 4u * i;
+        const unsigned mt = ot < 20u ? ot / 5u : (ot - 20u) / 4u;
+        const unsigned nt = ot < 20u ? ot % 5u : (ot - 20u) % 4u;
+        for (int q = 0; q < 4; q++) {
+            const unsigned r = 16u * mt + 4u * (l / 16u) + q, c = 16u * nt + l % 16u;
+            if (ot < 20u)
+                atomicAdd(&grads[oW3 + r * kKN + c], acc[i][q]);
+            else
+                atomicAdd(&grads[oW2 + r * kH + c], acc[i][q]);
+        }
+    }
+}
+
+// Edge backward: dW1 += sum over edges of (dS_i * SiLU'(W1 f_ij)) f_ij^T.
+__global__ void __launch_bounds__(256)
+k_edge_bwd(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS, unsigned slots,
+           const float2 *__restrict__ POS, const float2 *__restrict__ VEL,
+           const unsigned *__restrict__ K, const unsigned *__restrict__ start,
+           const unsigned short *__restrict__ W1b, const float *__restrict__ dS,
+           float *__restrict__ grads, unsigned *__restrict__ dropped)
+{
+    __shared__ unsigned short ebuf[4][kEdgeCap][kKE];
+    const unsigned w = threadIdx.x / 64u, l = lane();
+    bf16x4 B[4];
+    for (int t = 0; t < 4; t++)
+        B[t] = w1_b(W1b, t);
+    f32x4 g[4];
+    for (int t = 0; t < 4; t++)
+        g[t] = {0, 0, 0, 0};
+    const unsigned nw = gridDim.x * 4u;
+    for (unsigned node = blockIdx.x * 4u + w; node < slots; node += nw) {
+        const sph_dev_params &P = Pp[BS[node / kSphTPB]];
+        if (node >= P.poff + (unsigned)P.n)
+            continue;
+        const unsigned ne = gather_edges(node, P, POS, VEL, K, start, ebuf[w], dropped);
+        float ds[4];
+        for (int t = 0; t < 4; t++)
+            ds[t] = dS[(size_t)node * kH + 16u * t + l % 16u];
+        for (unsigned e0 = 0; e0 < ne; e0 += 16) {
+            const bf16x4 a = edge_tile_a(ebuf[w], e0);
+            // B-operand of the gradient product: F[edge][feature]
+            bf16x4 f;
+            for (int q = 0; q < 4; q++)
+                f[q] = (short)ebuf[w][e0 + 4u * (l / 16u) + q][l % 16u];
+            for (int t = 0; t < 4; t++) {
+                f32x4 d = {0, 0, 0, 0};
+                d = mfma(a, B[t], d);
+                // D layout = A layout of dPre1^T (unit x edge)
+                float gp[4];
+                for (int q = 0; q < 4; q++) {
+                    const unsigned e = e0 + 4u * (l / 16u) + q;
+                    gp[q] = e < ne ? ds[t] * silu_grad(d[q]) : 0.0f;
+                }
+                g[t] = mfma(pack4(gp[0], gp[1], gp[2], gp[3]), f, g[t]);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    for (int t = 0; t < 4; t++)
+        for (int q = 0; q < 4; q++) {
+            const unsigned unit = 16u * t + 4u * (l / 16u) + q, feat = l % 16u;
+            atomicAdd(&grads[oW1 + unit * kKE + feat], g[t][q]);
+        }
+}
+
+// Adam on the fp32 master weights, then refresh the bf16 copies.  Weights
+// of the padding columns (W1 beyond 6 inputs, W3 beyond 69) stay zero.
+__global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, int n, float lr, float b1, float b2, float eps,
+                       float c1, float c2, unsigned short *__restrict__ W1b,
+                       unsigned short *__restrict__ W2b, unsigned short *__restrict__ W3b)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    bool live = true;
+    if (i >= oW1 && i < oW2)
+        live = (i - oW1) % kKE < 6;
+    else if (i >= oW3 && i < oW4)
+        live = (i - oW3) % kKN < 69;
+    float x = p[i];
+    if (live) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gi;
+        const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        x -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        p[i] = x;
+    }
+    if (i >= oW1 && i < oW2)
+        W1b[i - oW1] = f2bf(x);
+    else if (i >= oW2 && i < ob2)
+        W2b[i - oW2] = f2bf(x);
+    else if (i >= oW3 && i < oW4)
+        W3b[i - oW3] = f2bf(x);
+}
+
+__global__ void k_to_bf16(const float *__restrict__ p, unsigned short *__restrict__ W1b,
+                          unsigned short *__restrict__ W2b, unsigned short *__restrict__ W3b)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= oW1 && i < oW2)
+        W1b[i - oW1] = f2bf(p[i]);
+    else if (i >= oW2 && i < ob2)
+        W2b[i - oW2] = f2bf(p[i]);
+    else if (i >= oW3 && i < oW4)
+        W3b[i - oW3] = f2bf(p[i]);
+}
+
+// by-id (concatenated caller order) -> slot order, and back.
+__global__ void k_by_id_to_slot(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+                                unsigned slots, const int *__restrict__ ID,
+                                const float2 *__restrict__ by_id, float2 *__restrict__ by_slot)
+{
+    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= slots)
+        return;
+    const sph_dev_params &P = Pp[BS[s / kSphTPB]];
+    if (s < P.poff + (unsigned)P.n)
+        by_slot[s] = by_id[P.uoff + (unsigned)ID[s]];
+}
+
+__global__ void k_slot_to_by_id(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+                                unsigned slots, const int *__restrict__ ID,
+                                const float2 *__restrict__ by_slot, float2 *__restrict__ by_id)
+{
+    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= slots)
+        return;
+    const sph_dev_params &P = Pp[BS[s / kSphTPB]];
+    if (s < P.poff + (unsigned)P.n)
+        by_id[P.uoff + (unsigned)ID[s]] = by_slot[s];
+}
+
+// Apply a predicted macro-step: x += d (then the box clamp of
+// src/fluid.c:729-743), v = d / (4 dt) + 1.5 a dt (the last step's velocity
+// under constant acceleration), clamped like src/fluid.c:613-625.
+__global__ void k_apply(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+                        unsigned slots, const unsigned long long *__restrict__ clock,
+                        const float2 *__restrict__ disp, float2 *__restrict__ POS,
+                        float2 *__restrict__ VEL)
+{
+    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= slots)
+        return;
+    const sph_dev_params &P = Pp[BS[s / kSphTPB]];
+    if (s >= P.poff + (unsigned)P.n || P.frozen)
+        return;
+    const NodeCtx nc = node_ctx(P, *clock);
+    const float2 d = disp[s];
+    float2 p = POS[s];
+    p.x += d.x;
+    p.y += d.y;
+    if (p.x < P.min_x)
+        p.x = P.min_x;
+    else if (p.x > P.max_x)
+        p.x = P.max_x - 0.001f;
+    if (p.y < P.min_y)
+        p.y = P.min_y;
+    else if (p.y > P.max_y)
+        p.y = P.max_y - 0.001f;
+    float vx = d.x / (4.0f * nc.dt) + 1.5f * nc.fx * nc.dt;
+    float vy = d.y / (4.0f * nc.dt) + 1.5f * nc.g_neg * nc.dt;
+    sph_check_velocity(vx, vy);
+    POS[s] = p;
+    VEL[s] = make_float2(vx, vy);
+}
+
+__global__ void k_clock_add(unsigned long long *clock, unsigned long long n)
+{
+    *clock += n; // one thread: a plain vector load/store pair
+}
+
+// Per-scene density sums and maxima of the last density pass.
+__global__ void k_scene_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+                                unsigned slots, const float2 *__restrict__ RHO,
+                                float *__restrict__ sum, unsigned *__restrict__ maxbits)
+{
+    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int scene = BS[(blockIdx.x * blockDim.x) / kSphTPB];
+    const sph_dev_params &P = Pp[scene];
+    float r = 0.0f;
+    if (s < slots && s < P.poff + (unsigned)P.n)
+        r = RHO[s].x;
+    float t = r, m = r;
+    for (int x = 32; x > 0; x >>= 1) {
+        t += __shfl_xor(t, x, 64);
+        m = fmaxf(m, __shfl_xor(m, x, 64));
+    }
+    if (lane() == 0) {
+        atomicAdd(&sum[scene], t);
+        atomicMax(&maxbits[scene], __float_as_uint(m)); // densities are >= 0
+    }
+}
+
+// Judge of a macro-step: mean density of each scene against its reference
+// (the mean density of the scene's last accepted state).  Accepted scenes
+// take the new mean as their reference; with set_all every scene does, with
+// `only` the flagged ones do and nothing is judged.
+__global__ void k_judge(const sph_dev_params *__restrict__ Pp, int nscenes,
+                        const float *__restrict__ sum, float *__restrict__ ref, float tol,
+                        int set_all, const int *__restrict__ only, int *__restrict__ flags)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nscenes)
+        return;
+    const int n = Pp[s].n;
+    const float mean = n > 0 ? sum[s] / (float)n : 0.0f;
+    if (set_all) {
+        ref[s] = mean;
+        return;
+    }
+    if (only) { // after a fallback: only the re-taken scenes take a new reference
+        if (only[s])
+            ref[s] = mean;
+        return;
+    }
+    const float r = ref[s];
+    const bool bad = n > 0 && !(fabsf(mean - r) <= tol * r);
+    flags[s] = bad ? 1 : 0;
+    if (!bad)
+        ref[s] = mean;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+namespace {
+
+thread_local char m_err[256] = "";
+
+int merr(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(m_err, sizeof m_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define MCHK(call)                                                                               \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return merr(SPH_EHIP, "%s failed: %s", #call, hipGetErrorString(e_));                \
+    } while (0)
+
+template <class T>
+int malloc_dev(T **p, size_t n)
+{
+    if (hipMalloc((void **)p, sizeof(T) * (n ? n : 1)) != hipSuccess)
+        return merr(SPH_ENOMEM, "hipMalloc of %zu x %zu B failed", n, sizeof(T));
+    return SPH_OK;
+}
+
+int ensure_buffers(sph_surrogate *m, size_t slots)
+{
+    if (m->slots >= slots)
+        return SPH_OK;
+    void *old[] = {m->S, m->deg, m->dS, m->disp, m->tgt, m->U, m->dP3, m->dM, m->tmp2};
+    for (void *p : old)
+        if (p)
+            (void)hipFree(p);
+    m->S = m->deg = m->dS = m->disp = m->tgt = nullptr;
+    m->U = m->dP3 = m->dM = nullptr;
+    m->tmp2 = nullptr;
+    int rc = SPH_OK;
+    if (!rc) rc = malloc_dev(&m->S, slots * kH);
+    if (!rc) rc = malloc_dev(&m->deg, slots);
+    if (!rc) rc = malloc_dev(&m->dS, slots * kH);
+    if (!rc) rc = malloc_dev(&m->disp, slots * 2);
+    if (!rc) rc = malloc_dev(&m->tgt, slots * 2);
+    if (!rc) rc = malloc_dev(&m->U, slots * kKN);
+    if (!rc) rc = malloc_dev(&m->dP3, slots * kH);
+    if (!rc) rc = malloc_dev(&m->dM, slots * kH);
+    if (!rc) rc = malloc_dev(&m->tmp2, slots);
+    m->slots = rc ? 0 : slots;
+    return rc;
+}
+
+int ensure_judge(sph_surrogate *m, const sph_ctx *c)
+{
+    const size_t ns = c->sc.size();
+    if (m->ns >= ns)
+        return SPH_OK;
+    void *old[] = {m->ref, m->jsum, m->jmax, m->jflags};
+    for (void *p : old)
+        if (p)
+            (void)hipFree(p);
+    m->ref = m->jsum = nullptr;
+    m->jmax = nullptr;
+    m->jflags = nullptr;
+    m->ref_ok = false;
+    int rc = SPH_OK;
+    if (!rc) rc = malloc_dev(&m->ref, ns);
+    if (!rc) rc = malloc_dev(&m->jsum, ns);
+    if (!rc) rc = malloc_dev(&m->jmax, ns);
+    if (!rc) rc = malloc_dev(&m->jflags, ns);
+    m->ns = rc ? 0 : ns;
+    return rc;
+}
+
+// Per-scene density sums (and maxima) of RHO into m->jsum / m->jmax.
+int scene_sums(sph_surrogate *m, sph_ctx *c)
+{
+    const size_t ns = c->sc.size();
+    MCHK(hipMemsetAsync(m->jsum, 0, sizeof(float) * ns, c->st));
+    MCHK(hipMemsetAsync(m->jmax, 0, sizeof(unsigned) * ns, c->st));
+    k_scene_density<<<dim3(c->slots / 256), 256, 0, c->st>>>(c->dP, c->BS, c->slots, c->RHO,
+                                                              m->jsum, m->jmax);
+    MCHK(hipGetLastError());
+    return SPH_OK;
+}
+
+// Density of the current (hashed) state into RHO.
+int density_now(sph_ctx *c)
+{
+    if (int rc = sph_internal_canonicalize(c))
+        return rc;
+    if (!c->lists_valid)
+        return merr(SPH_ESTATE, "surrogate: state not hashed");
+    sph_launch_density(c->st, c->slots, c->dP, c->BS, c->POS0, c->K0, c->start, c->RHO, c->dC);
+    MCHK(hipGetLastError());
+    return SPH_OK;
+}
+
+int forward(sph_surrogate *m, sph_ctx *c)
+{
+    if (!c->lists_valid)
+        return merr(SPH_ESTATE, "surrogate: the context has no hashed state (step or hash first)");
+    if (int rc = ensure_buffers(m, c->slots))
+        return rc;
+    const unsigned S = c->slots;
+    k_edge_fwd<<<dim3(2048), 256, 0, c->st>>>(c->dP, c->BS, S, c->POS0, c->VEL, c->K0, c->start,
+                                              m->W1b, m->S, m->deg, m->dropped);
+    k_node_fwd<<<dim3(2048), 256, 0, c->st>>>(c->dP, c->BS, S, c->dclock, c->VEL, m->S, m->deg,
+                                              m->W2b, m->params, m->W3b, (float2 *)m->disp);
+    MCHK(hipGetLastError());
+    m->have_pred = true;
+    return SPH_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *sph_surrogate_last_error(void) { return m_err; }
+
+int sph_surrogate_num_params(void) { return kNParams; }
+
+int sph_surrogate_create(sph_surrogate **out, uint64_t seed)
+{
+    if (!out)
+        return merr(SPH_EINVAL, "null output");
+    *out = nullptr;
+    sph_surrogate *m = new (std::nothrow) sph_surrogate();
+    if (!m)
+        return merr(SPH_ENOMEM, "host allocation failed");
+    int rc = SPH_OK;
+    if (!rc) rc = malloc_dev(&m->params, kNParams);
+    if (!rc) rc = malloc_dev(&m->grads, kNParams);
+    if (!rc) rc = malloc_dev(&m->adam_m, kNParams);
+    if (!rc) rc = malloc_dev(&m->adam_v, kNParams);
+    if (!rc) rc = malloc_dev(&m->W1b, kH * kKE);
+    if (!rc) rc = malloc_dev(&m->W2b, kH * kH);
+    if (!rc) rc = malloc_dev(&m->W3b, kH * kKN);
+    if (!rc) rc = malloc_dev(&m->loss, 1);
+    if (!rc) rc = malloc_dev(&m->dropped, 1);
+    if (rc) {
+        sph_surrogate_destroy(m);
+        return rc;
+    }
+    // Initialisation: uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) from SplitMix64,
+    // padding columns zero, the output layer scaled down so the untrained
+    // network starts close to the ballistic estimate.
+    std::vector<float> p(kNParams, 0.0f);
+    uint64_t st = seed;
+    auto uni = [&]() {
+        uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (float)(z >> 40) * 0x1p-24f * 2.0f - 1.0f;
+    };
+    for (int r = 0; r < kH; r++)
+        for (int k = 0; k < 6; k++)
+            p[oW1 + r * kKE + k] = uni() / sqrtf(6.0f);
+    for (int i = 0; i < kH * kH; i++)
+        p[oW2 + i] = uni() / sqrtf((float)kH);
+    for (int r = 0; r < kH; r++)
+        for (int k = 0; k < 69; k++)
+            p[oW3 + r * kKN + k] = uni() / sqrtf(69.0f);
+    for (int i = 0; i < 2 * kH; i++)
+        p[oW4 + i] = 0.1f * uni() / sqrtf((float)kH);
+    if (hipMemcpy(m->params, p.data(), sizeof(float) * kNParams, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(m->grads, 0, sizeof(float) * kNParams) != hipSuccess ||
+        hipMemset(m->adam_m, 0, sizeof(float) * kNParams) != hipSuccess ||
+        hipMemset(m->adam_v, 0, sizeof(float) * kNParams) != hipSuccess ||
+        hipMemset(m->dropped, 0, sizeof(unsigned)) != hipSuccess) {
+        sph_surrogate_destroy(m);
+        return merr(SPH_EHIP, "surrogate initialisation failed");
+    }
+    k_to_bf16<<<dim3((kNParams + 255) / 256), 256>>>(m->params, m->W1b, m->W2b, m->W3b);
+    if (hipDeviceSynchronize() != hipSuccess) {
+        sph_surrogate_destroy(m);
+        return merr(SPH_EHIP, "surrogate initialisation failed");
+    }
+    *out = m;
+    return SPH_OK;
+}
+
+void sph_surrogate_destroy(sph_surrogate *m)
+{
+    if (!m)
+        return;
+    void *ptrs[] = {m->params, m->grads, m->adam_m, m->adam_v, m->W1b, m->W2b, m->W3b, m->S,
+                    m->deg,    m->dS,    m->disp,   m->tgt,    m->U,   m->dP3, m->dM,  m->loss,
+                    m->dropped, m->tmp2, m->ref, m->jsum, m->jmax, m->jflags};
+    for (void *p : ptrs)
+        if (p)
+            (void)hipFree(p);
+    delete m;
+}
+
+int sph_surrogate_get_params(sph_surrogate *m, float *out)
+{
+    if (!m || !out)
+        return merr(SPH_EINVAL, "null argument");
+    MCHK(hipDeviceSynchronize());
+    MCHK(hipMemcpy(out, m->params, sizeof(float) * kNParams, hipMemcpyDeviceToHost));
+    return SPH_OK;
+}
+
+int sph_surrogate_set_params(sph_surrogate *m, const float *in)
+{
+    if (!m || !in)
+        return merr(SPH_EINVAL, "null argument");
+    MCHK(hipDeviceSynchronize());
+    MCHK(hipMemcpy(m->params, in, sizeof(float) * kNParams, hipMemcpyHostToDevice));
+    k_to_bf16<<<dim3((kNParams + 255) / 256), 256>>>(m->params, m->W1b, m->W2b, m->W3b);
+    MCHK(hipDeviceSynchronize());
+    return SPH_OK;
+}
+
+int sph_surrogate_get_grads(sph_surrogate *m, float *out)
+{
+    if (!m || !out)
+        return merr(SPH_EINVAL, "null argument");
+    MCHK(hipDeviceSynchronize());
+    MCHK(hipMemcpy(out, m->grads, sizeof(float) * kNParams, hipMemcpyDeviceToHost));
+    return SPH_OK;
+}
+
+int sph_surrogate_zero_grads(sph_surrogate *m)
+{
+    if (!m)
+        return merr(SPH_EINVAL, "null argument");
+    MCHK(hipMemset(m->grads, 0, sizeof(float) * kNParams));
+    return SPH_OK;
+}
+
+// Forward on the context's current hashed state; the predicted displacement
+// per particle (caller order, 2 floats each) is copied out if disp_by_id.
+int sph_surrogate_predict(sph_surrogate *m, sph_ctx *c, float *disp_by_id)
+{
+    if (!m || !c)
+        return merr(SPH_EINVAL, "null argument");
+    if (int rc = forward(m, c))
+        return rc;
+    if (disp_by_id && c->n_total) {
+        k_slot_to_by_id<<<dim3(c->slots / 256), 256, 0, c->st>>>(
+            c->dP, c->BS, c->slots, c->ID0, (const float2 *)m->disp, m->tmp2);
+        MCHK(hipMemcpyAsync(disp_by_id, m->tmp2, sizeof(float2) * (size_t)c->n_total,
+                            hipMemcpyDeviceToHost, c->st));
+        MCHK(hipStreamSynchronize(c->st));
+    }
+    return SPH_OK;
+}
+
+// One gradient evaluation: forward, loss against target displacements
+// (a device buffer in caller order, 2 floats per particle), backward; gradients are accumulated into the gradient
+// buffer (zero it with sph_surrogate_zero_grads).  *loss gets the mean of
+// |d - target|^2 / 0.05^2 over the particles.
+int sph_surrogate_backward(sph_surrogate *m, sph_ctx *c, const float *target_by_id, float *loss)
+{
+    if (!m || !c || (!target_by_id && c->n_total))
+        return merr(SPH_EINVAL, "null argument");
+    if (c->n_total == 0)
+        return SPH_OK;
+    if (int rc = forward(m, c))
+        return rc;
+    const unsigned S = c->slots;
+    k_by_id_to_slot<<<dim3(S / 256), 256, 0, c->st>>>(c->dP, c->BS, S, c->ID0,
+                                                     (const float2 *)target_by_id,
+                                                     (float2 *)m->tgt);
+    MCHK(hipMemsetAsync(m->loss, 0, sizeof(float), c->st));
+    k_node_bwd<<<dim3(2048), 256, 0, c->st>>>(c->dP, c->BS, S, c->dclock, c->VEL, m->S, m->deg,
+                                              m->W2b, m->params, m->W3b, (const float2 *)m->tgt,
+                                              1.0f / (float)c->n_total, m->dS, m->U, m->dP3,
+                                              m->dM, m->grads, m->loss);
+    k_node_wgrad<<<dim3(1024), 256, 0, c->st>>>(S / 16u, m->U, m->dP3, m->dM, m->S, m->grads);
+    k_edge_bwd<<<dim3(2048), 256, 0, c->st>>>(c->dP, c->BS, S, c->POS0, c->VEL, c->K0, c->start,
+                                              m->W1b, m->dS, m->grads, m->dropped);
+    MCHK(hipGetLastError());
+    if (loss) {
+        MCHK(hipMemcpyAsync(loss, m->loss, sizeof(float), hipMemcpyDeviceToHost, c->st));
+        MCHK(hipStreamSynchronize(c->st));
+    }
+    return SPH_OK;
+}
+
+int sph_surrogate_adam(sph_surrogate *m, float lr, float beta1, float beta2, float eps)
+{
+    if (!m)
+        return merr(SPH_EINVAL, "null argument");
+    m->adam_t++;
+    const float c1 = 1.0f - powf(beta1, (float)m->adam_t);
+    const float c2 = 1.0f - powf(beta2, (float)m->adam_t);
+    k_adam<<<dim3((kNParams + 255) / 256), 256>>>(m->params, m->grads, m->adam_m, m->adam_v,
+                                                  kNParams, lr, beta1, beta2, eps, c1, c2, m->W1b,
+                                                  m->W2b, m->W3b);
+    MCHK(hipGetLastError());
+    MCHK(hipMemset(m->grads, 0, sizeof(float) * kNParams));
+    MCHK(hipDeviceSynchronize());
+    return SPH_OK;
+}
+
+// Apply the last prediction to the context (positions and velocities move
+// four steps); frozen scenes are left alone.  The state must be hashed
+// again before the next step (the context knows this).
+int sph_surrogate_apply(sph_surrogate *m, sph_ctx *c)
+{
+    if (!m || !c)
+        return merr(SPH_EINVAL, "null argument");
+    if (!m->have_pred)
+        return merr(SPH_ESTATE, "sph_surrogate_apply: no prediction");
+    if (int rc = sph_internal_canonicalize(c))
+        return rc;
+    const unsigned S = c->slots;
+    k_apply<<<dim3(S / 256), 256, 0, c->st>>>(c->dP, c->BS, S, c->dclock,
+                                             (const float2 *)m->disp, c->POS0, c->VEL);
+    MCHK(hipGetLastError());
+    k_clock_add<<<1, 1, 0, c->st>>>(c->dclock, 4ull);
+    MCHK(hipGetLastError());
+    c->lists_valid = false;
+    m->have_pred = false;
+    return SPH_OK;
+}
+
+// One surrogate macro-step (four plain steps of simulated time) of every
+// scene, judged by the plain solver: the predicted state is hashed and its
+// density computed; a scene whose mean density moved by more than tol
+// (relative) from its last accepted state is restored and re-taken with
+// four plain steps.  flags (optional, nscenes ints) receives 1 for re-taken
+// scenes; *nfallback their number.
+int sph_surrogate_macro_step(sph_surrogate *m, sph_ctx *c, float tol, int *flags, int *nfallback)
+{
+    if (!m || !c)
+        return merr(SPH_EINVAL, "null argument");
+    if (!(tol >= 0.0f))
+        return merr(SPH_EINVAL, "tolerance must be >= 0");
+    const int ns = (int)c->sc.size();
+    if (nfallback)
+        *nfallback = 0;
+    if (c->n_total == 0)
+        return SPH_OK;
+    if (int rc = ensure_judge(m, c))
+        return rc;
+    if (!c->lists_valid)
+        if (sph_phase_hash(c, 0))
+            return merr(SPH_EHIP, "hash failed: %s", sph_last_error());
+    if (!m->ref_ok || m->ref_ctx != c) {
+        if (int rc = density_now(c))
+            return rc;
+        if (int rc = scene_sums(m, c))
+            return rc;
+        k_judge<<<dim3((ns + 255) / 256), 256, 0, c->st>>>(c->dP, ns, m->jsum, m->ref, tol, 1,
+                                                           nullptr, nullptr);
+        MCHK(hipGetLastError());
+        m->ref_ok = true;
+        m->ref_ctx = c;
+    }
+    if (sph_snapshot_save(c))
+        return merr(SPH_EHIP, "snapshot failed: %s", sph_last_error());
+    if (int rc = forward(m, c))
+        return rc;
+    if (int rc = sph_surrogate_apply(m, c))
+        return rc;
+    if (sph_phase_hash(c, 1))
+        return merr(SPH_EHIP, "hash failed: %s", sph_last_error());
+    if (int rc = scene_sums(m, c))
+        return rc;
+    k_judge<<<dim3((ns + 255) / 256), 256, 0, c->st>>>(c->dP, ns, m->jsum, m->ref, tol, 0,
+                                                       nullptr, m->jflags);
+    MCHK(hipGetLastError());
+    std::vector<int> hf((size_t)ns);
+    MCHK(hipMemcpyAsync(hf.data(), m->jflags, sizeof(int) * ns, hipMemcpyDeviceToHost, c->st));
+    MCHK(hipStreamSynchronize(c->st));
+    int nf = 0;
+    for (int s = 0; s < ns; s++)
+        nf += hf[s] != 0;
+    if (nf) {
+        std::vector<int> frozen((size_t)ns), none((size_t)ns, 0);
+        for (int s = 0; s < ns; s++)
+            frozen[s] = hf[s] ? 0 : 1;
+        if (sph_snapshot_restore(c, hf.data()) || sph_set_scene_frozen(c, frozen.data()) ||
+            sph_step(c, 4) || sph_set_scene_frozen(c, none.data()))
+            return merr(SPH_EHIP, "fallback failed: %s", sph_last_error());
+        if (int rc = density_now(c))
+            return rc;
+        if (int rc = scene_sums(m, c))
+            return rc;
+        MCHK(hipMemcpyAsync(m->jflags, hf.data(), sizeof(int) * ns, hipMemcpyHostToDevice, c->st));
+        k_judge<<<dim3((ns + 255) / 256), 256, 0, c->st>>>(c->dP, ns, m->jsum, m->ref, tol, 0,
+                                                           m->jflags, nullptr);
+        MCHK(hipGetLastError());
+        MCHK(hipStreamSynchronize(c->st));
+    }
+    if (flags)
+        memcpy(flags, hf.data(), sizeof(int) * ns);
+    if (nfallback)
+        *nfallback = nf;
+    return SPH_OK;
+}
+
+unsigned sph_surrogate_dropped_edges(sph_surrogate *m)
+{
+    unsigned d = 0;
+    if (m && hipMemcpy(&d, m->dropped, sizeof d, hipMemcpyDeviceToHost) == hipSuccess)
+        return d;
+    return 0;
+}
+
+int sph_surrogate_mfma_selftest(float *out256)
+{
+    if (!out256)
+        return merr(SPH_EINVAL, "null argument");
+    float *d = nullptr;
+    MCHK(hipMalloc(&d, 256 * sizeof(float)));
+    k_mfma_selftest<<<1, 64>>>(d);
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess)
+        e = hipMemcpy(out256, d, 256 * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess)
+        return merr(SPH_EHIP, "mfma selftest: %s", hipGetErrorString(e));
+    return SPH_OK;
+}
+
+// Per-scene mean and maximum density of the last density pass (sph_step's
+// or sph_phase_hash(ctx, 1)'s).
+int sph_scene_density(sph_ctx *c, float *mean, float *maxv)
+{
+    if (!c || !mean || !maxv)
+        return merr(SPH_EINVAL, "null argument");
+    if (int rc = sph_internal_canonicalize(c))
+        return rc;
+    const size_t ns = c->sc.size();
+    float *sum = nullptr;
+    unsigned *mx = nullptr;
+    if (int rc = malloc_dev(&sum, ns))
+        return rc;
+    if (int rc = malloc_dev(&mx, ns)) {
+        (void)hipFree(sum);
+        return rc;
+    }
+    hipError_t e = hipMemsetAsync(sum, 0, sizeof(float) * ns, c->st);
+    if (e == hipSuccess)
+        e = hipMemsetAsync(mx, 0, sizeof(unsigned) * ns, c->st);
+    if (e == hipSuccess) {
+        k_scene_density<<<dim3(c->slots / 256), 256, 0, c->st>>>(c->dP, c->BS, c->slots, c->RHO,
+                                                                  sum, mx);
+        e = hipGetLastError();
+    }
+    std::vector<float> hs(ns);
+    std::vector<unsigned> hm(ns);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(c->st);
+    if (e == hipSuccess)
+        e = hipMemcpy(hs.data(), sum, sizeof(float) * ns, hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+        e = hipMemcpy(hm.data(), mx, sizeof(unsigned) * ns, hipMemcpyDeviceToHost);
+    (void)hipFree(sum);
+    (void)hipFree(mx);
+    if (e != hipSuccess)
+        return merr(SPH_EHIP, "sph_scene_density: %s", hipGetErrorString(e));
+    for (size_t s = 0; s < ns; s++) {
+        mean[s] = c->sc[s].n ? hs[s] / (float)c->sc[s].n : 0.0f;
+        float f;
+        memcpy(&f, &hm[s], sizeof f);
+        maxv[s] = f;
+    }
+    return SPH_OK;
+}
+
+} // extern "C"

@@ -8,6 +8,11 @@
   U[0.2, 0.5], one of four parameter presets (configs/presets.json) each;
   box area = N (0.5h)^2 / fraction, W / H = aspect, the fluid a block of the
   box's aspect in its lower-left corner.
+* ``sloshing_dam_breaks(nscenes, n, box, seed)`` - the learned macro-step's
+  training and inference scenes: n particles, the first n of an nx-wide
+  lattice (nx seeded, so blocks range from tall to wide) in the lower-left
+  of the box, preset 0, and a seeded horizontal box acceleration of
+  amplitude U[0, 2] and period U[1, 4] s.
 """
 from __future__ import annotations
 
@@ -68,3 +73,23 @@ def concat(scenes):
     parts = np.concatenate([p for _, _, p, _ in scenes]) if scenes else np.zeros(0, PARTICLE)
     counts = np.array([len(p) for _, _, p, _ in scenes], np.int32)
     return parts, counts
+
+
+def sloshing_dam_breaks(nscenes, n=16384, box=(128.0, 96.0), seed=7, spacing=0.5, jitter=0.01,
+                        amp=(0.0, 2.0), period=(1.0, 4.0)):
+    """List of (tunables, box, particles, amplitude, period)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    wmax = box[0] * 0.75 / spacing
+    for s in range(nscenes):
+        nx_lo = int(math.ceil(math.sqrt(n) * 0.75))
+        nx_hi = int(min(wmax, math.sqrt(n) * 1.5))
+        nx = int(rng.integers(nx_lo, max(nx_lo + 1, nx_hi)))
+        ny = int(math.ceil(n / nx))
+        assert ny * spacing < box[1] * 0.9, "fluid block taller than the box"
+        parts = scene_lattice(nx, ny, spacing, jitter, seed=seed * 1000003 + s,
+                              x0=spacing / 2, y0=spacing / 2)[:n].copy()
+        a = float(rng.uniform(*amp))
+        T = float(rng.uniform(*period))
+        out.append((preset_tunables(0), make_box(*box), parts, a, T))
+    return out

@@ -84,3 +84,45 @@ def test_scene_tunables_are_independent():
     a, b = split(s.download(), [len(p), len(p)])
     assert_bitexact(a, oracle_run(t0, box, p, 10), what="scene 0")
     assert_bitexact(b, oracle_run(t1, box, p, 10), what="scene 1")
+
+
+def test_frozen_scene_is_left_unchanged():
+    """Frozen scenes pass through sph_step untouched; the others advance
+    exactly as alone."""
+    t = sc.preset_tunables(1)
+    box = sa.make_box(24, 20)
+    ps = [sa.scene_lattice(24, 16, 0.5, 0.01, seed=30 + i, x0=0.25, y0=0.25) for i in range(3)]
+    s = sa.Solver.batch([(t, box, len(p)) for p in ps])
+    s.upload_batch(np.concatenate(ps), [len(p) for p in ps])
+    s.step(3)
+    before = split(s.download(), [len(p) for p in ps])
+    s.set_frozen([0, 1, 0])
+    s.step(7)
+    after = split(s.download(), [len(p) for p in ps])
+    for f in ("x", "y", "v_x", "v_y"):
+        assert np.array_equal(after[1][f], before[1][f]), f
+    for i in (0, 2):
+        assert_bitexact(after[i], oracle_run(t, box, ps[i], 10), what=f"scene {i}")
+    s.set_frozen([0, 0, 0])
+    s.step(2)
+    got = split(s.download(), [len(p) for p in ps])
+    assert_bitexact(got[0], oracle_run(t, box, ps[0], 12), what="scene 0 after thaw")
+
+
+def test_sloshing_forcing_closed_form():
+    """A lone particle far from the walls under sloshing: v_x after n steps is
+    the sum of -A sin(omega k dt) dt over the steps taken (float tolerance)."""
+    t = sc.preset_tunables(0)
+    box = sa.make_box(200, 200)
+    p = np.zeros(1, sa.PARTICLE)
+    p["x"], p["y"] = 3.0, 150.0  # small |x|: fine float spacing for (x - x_prev) / dt
+    s = sa.Solver.batch([(t, box, 1)])
+    s.upload_batch(p, [1])
+    A, T = 1.5, 2.0
+    s.set_scene_forcing(0, A, T)
+    dt = 1.0 / 60.0
+    s.step(30)
+    got = s.download()
+    want = sum(-A * np.sin(2 * np.pi / T * k * dt) * dt for k in range(30))
+    assert abs(got["v_x"][0] - want) < 1e-3, (got["v_x"][0], want)
+    assert got["v_x"][0] < 0
