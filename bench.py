@@ -173,6 +173,50 @@ def calibration_copy(mbytes):
     hip.hipFree(b)
 
 
+def profile_summary(db, last=0):
+    """Text summary of a rocprofv3 results database (its default output):
+    per kernel, the dispatch count and mean / total duration from the kernel
+    trace, and the mean per dispatch of every collected counter (FETCH_SIZE
+    and WRITE_SIZE are in KiB).  `last` keeps only the last N dispatches of
+    the trace (the steady state)."""
+    import re
+    import sqlite3
+    con = sqlite3.connect(db)
+    names = {r[0] for r in con.execute("select name from sqlite_master")}
+    lines = []
+    short = lambda k: re.sub(r"\(.*", "", k).replace("void ", "")  # noqa: E731
+    if "kernels" in names:
+        rows = con.execute("select name, dispatch_id, start, end, duration from kernels "
+                           "order by start").fetchall()
+        rows = rows[-last:] if last else rows
+        agg = {}
+        for name, _, _, _, dur in rows:
+            agg.setdefault(short(name), []).append(dur)
+        if rows:
+            lines.append(f"{'kernel':42s} {'calls':>6s} {'mean_us':>10s} {'total_ms':>10s}")
+            for k, v in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
+                lines.append(f"{k:42s} {len(v):6d} {sum(v) / len(v) / 1e3:10.2f} "
+                             f"{sum(v) / 1e6:10.3f}")
+            span = rows[-1][3] - rows[0][2]
+            busy = sum(r[4] for r in rows)
+            lines.append(f"span {span / 1e3:.1f} us, kernels busy {busy / 1e3:.1f} us "
+                         f"({len(rows)} dispatches)")
+    if "counters_collection" in names:
+        rows = con.execute("select dispatch_id, kernel_name, counter_name, value "
+                           "from counters_collection").fetchall()
+        agg, disp = {}, {}
+        for d, k, cn, v in rows:
+            agg.setdefault(short(k), {}).setdefault(cn, 0.0)
+            agg[short(k)][cn] += v
+            disp.setdefault(short(k), set()).add(d)
+        lines.append(f"{'kernel':42s} {'dispatches':>10s} counters (mean per dispatch)")
+        for k, m in sorted(agg.items()):
+            nd = len(disp[k])
+            cs = ", ".join(f"{cn}={v / nd:.1f}" for cn, v in sorted(m.items()))
+            lines.append(f"{k:42s} {nd:10d} {cs}")
+    return "\n".join(lines)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,7 +229,14 @@ def main():
                     help="plain-step equivalents of the surrogate line")
     ap.add_argument("--calib-copy-mb", type=float, default=0.0,
                     help="also run a device copy of this many MB (profiler calibration)")
+    ap.add_argument("--profile-summary", metavar="DB",
+                    help="print a summary of a rocprofv3 results database and exit")
+    ap.add_argument("--last", type=int, default=0,
+                    help="with --profile-summary: keep the last N trace dispatches")
     args = ap.parse_args()
+    if args.profile_summary:
+        print(profile_summary(args.profile_summary, args.last))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs one scene on one GPU; --gpus must be 1"}))
         sys.exit(2)

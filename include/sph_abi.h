@@ -401,8 +401,8 @@ int sph_surrogate_zero_grads(sph_surrogate *m);
 /* Forward on the context's hashed state; disp_by_id (host, optional)
  * receives the predicted displacements. */
 int sph_surrogate_predict(sph_surrogate *m, sph_ctx *ctx, float *disp_by_id);
-/* Forward + loss + backward on the hashed state; target_by_id is a DEVICE
- * pointer (caller order, 2 floats per particle).  Gradients accumulate;
+/* Forward + loss + backward on the hashed state against target
+ * displacements target_by_id (host, caller order).  Gradients accumulate;
  * loss (host, optional) = mean |d - target|^2 / 0.05^2. */
 int sph_surrogate_backward(sph_surrogate *m, sph_ctx *ctx, const float *target_by_id,
                            float *loss);
@@ -422,6 +422,23 @@ unsigned sph_surrogate_dropped_edges(sph_surrogate *m);
 int sph_surrogate_mfma_selftest(float *out256);
 /* Per-scene mean and maximum density of the last density pass. */
 int sph_scene_density(sph_ctx *ctx, float *mean, float *maxv);
+
+/* Trajectory store for training, resident in device memory: frames of the
+ * state (x, y, v_x, v_y per particle) of nscenes scenes of n particles.
+ * record stores a context's state (nscenes x n) as frame f; load puts
+ * count samples (scene[k] at frame[k]) into the count scenes of a training
+ * context (n particles each; x_prev = x, the state unhashed) and keeps their
+ * 4-step targets (frame + 1 minus frame positions) for
+ * sph_surrogate_backward_trajectory; read copies a frame to the host. */
+typedef struct sph_trajectory sph_trajectory;
+int sph_trajectory_create(sph_trajectory **tr, int nscenes, int n, int frames);
+void sph_trajectory_destroy(sph_trajectory *tr);
+int sph_trajectory_record(sph_trajectory *tr, sph_ctx *ctx, int frame);
+int sph_trajectory_read(sph_trajectory *tr, int frame, float *out);
+int sph_trajectory_load(sph_trajectory *tr, sph_ctx *ctx, int count, const int *scene,
+                        const int *frame);
+int sph_surrogate_backward_trajectory(sph_surrogate *m, sph_ctx *ctx, sph_trajectory *tr,
+                                      float *loss);
 
 #ifdef __cplusplus
 }

@@ -18,7 +18,7 @@ import os
 import numpy as np
 
 __all__ = ["PARTICLE", "Tunables", "Param", "AABB", "SceneDesc", "Counters", "Solver", "SphError",
-           "Surrogate", "mfma_selftest", "make_tunables", "make_box", "scene_lattice", "lib",
+           "Surrogate", "Trajectory", "mfma_selftest", "make_tunables", "make_box", "scene_lattice", "lib",
            "LIB_PATH"]
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
@@ -146,6 +146,12 @@ def lib():
             "sph_surrogate_macro_step": (ip, [vp, vp, ctypes.c_float, vp, ctypes.POINTER(ip)]),
             "sph_surrogate_dropped_edges": (ctypes.c_uint, [vp]),
             "sph_surrogate_mfma_selftest": (ip, [vp]),
+            "sph_surrogate_backward_trajectory": (ip, [vp, vp, vp, ctypes.POINTER(ctypes.c_float)]),
+            "sph_trajectory_create": (ip, [ctypes.POINTER(vp), ip, ip, ip]),
+            "sph_trajectory_destroy": (None, [vp]),
+            "sph_trajectory_record": (ip, [vp, vp, ip]),
+            "sph_trajectory_read": (ip, [vp, ip, vp]),
+            "sph_trajectory_load": (ip, [vp, vp, ip, vp, vp]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -165,19 +171,6 @@ def _check_m(rc, what):
     if rc != 0:
         msg = lib().sph_surrogate_last_error().decode(errors="replace")
         raise SphError(f"{what} failed ({rc}): {msg}")
-
-
-def _dev_ptr(t, nfloat4=None):
-    """Device address of a contiguous float32 CUDA tensor (torch is used as the
-    device allocator only)."""
-    import torch
-    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32
-            and t.is_contiguous()):
-        raise SphError("expected a contiguous float32 tensor on the GPU")
-    if nfloat4 is not None and t.numel() < 4 * nfloat4:
-        raise SphError(f"device buffer holds {t.numel()} floats, {4 * nfloat4} needed")
-    torch.cuda.synchronize(t.device)  # the library works on its own stream
-    return t.data_ptr()
 
 
 def make_tunables(**kw) -> Tunables:
@@ -254,15 +247,6 @@ class Solver:
     def snapshot_restore(self, flags):
         f = np.ascontiguousarray(flags, dtype=np.int32)
         _check(lib().sph_snapshot_restore(self._h, f.ctypes.data), "sph_snapshot_restore")
-
-    def gather_state(self, out):
-        """(x, y, v_x, v_y) per particle in caller order into a float32 CUDA
-        tensor of n x 4."""
-        _check(lib().sph_gather_state_dev(self._h, _dev_ptr(out, self.n)), "sph_gather_state_dev")
-
-    def load_state(self, xv):
-        """Replace positions/velocities from a float32 CUDA tensor of n x 4."""
-        _check(lib().sph_load_state_dev(self._h, _dev_ptr(xv, self.n)), "sph_load_state_dev")
 
     def scene_density(self):
         ns = self.nscenes
@@ -427,10 +411,18 @@ class Surrogate:
         return out
 
     def backward(self, solver: Solver, target) -> float:
-        """target: float32 CUDA tensor n x 2 (caller order)."""
+        """target: n x 2 float32 displacements (caller order)."""
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        assert t.shape == (solver.n, 2)
         loss = ctypes.c_float()
-        _check_m(lib().sph_surrogate_backward(self._h, solver._h, _dev_ptr(target),
+        _check_m(lib().sph_surrogate_backward(self._h, solver._h, t.ctypes.data,
                                               ctypes.byref(loss)), "backward")
+        return float(loss.value)
+
+    def backward_trajectory(self, solver: Solver, traj: "Trajectory") -> float:
+        loss = ctypes.c_float()
+        _check_m(lib().sph_surrogate_backward_trajectory(self._h, solver._h, traj._h,
+                                                         ctypes.byref(loss)), "backward")
         return float(loss.value)
 
     def adam(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
@@ -448,6 +440,42 @@ class Surrogate:
 
     def dropped_edges(self) -> int:
         return int(lib().sph_surrogate_dropped_edges(self._h))
+
+
+class Trajectory:
+    """Device-resident trajectory store (sph_trajectory_*)."""
+
+    def __init__(self, nscenes: int, n: int, frames: int):
+        self._h = ctypes.c_void_p()
+        self.nscenes, self.n, self.frames = nscenes, n, frames
+        _check_m(lib().sph_trajectory_create(ctypes.byref(self._h), nscenes, n, frames),
+                 "sph_trajectory_create")
+
+    def close(self):
+        if self._h:
+            lib().sph_trajectory_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def record(self, solver: Solver, frame: int):
+        _check_m(lib().sph_trajectory_record(self._h, solver._h, frame), "sph_trajectory_record")
+
+    def read(self, frame: int) -> np.ndarray:
+        out = np.zeros((self.nscenes, self.n, 4), np.float32)
+        _check_m(lib().sph_trajectory_read(self._h, frame, out.ctypes.data), "sph_trajectory_read")
+        return out
+
+    def load(self, solver: Solver, scenes, frames):
+        s = np.ascontiguousarray(scenes, dtype=np.int32)
+        f = np.ascontiguousarray(frames, dtype=np.int32)
+        assert s.shape == f.shape
+        _check_m(lib().sph_trajectory_load(self._h, solver._h, len(s), s.ctypes.data, f.ctypes.data),
+                 "sph_trajectory_load")
 
 
 def mfma_selftest() -> np.ndarray:

@@ -39,7 +39,8 @@ namespace {
 constexpr int kH = 64;      // hidden width
 constexpr int kKE = 16;     // edge input width (6 used)
 constexpr int kKN = 80;     // node input width (64 + 4 + bias, padded)
-constexpr int kEdgeCap = 192; // edges kept per particle (more are counted, dropped)
+constexpr unsigned kGroup = 16;         // consecutive slots per wavefront in the edge passes
+constexpr unsigned kGroupEdgeCap = 512; // edges of one group kept (more are counted, dropped)
 constexpr float kVScale = 0.2f, kAScale = 0.1f, kDScale = 0.05f;
 
 // parameter block (fp32 master copy)
@@ -62,11 +63,18 @@ __device__ __forceinline__ unsigned short f2bf(float f)
 
 __device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
 
-__device__ __forceinline__ float silu(float x) { return x / (1.0f + __expf(-x)); }
+// Logistic function from the hardware exp2 and reciprocal (1 ulp each): the
+// network is trained and run with this definition on both passes.
+__device__ __forceinline__ float sigm(float x)
+{
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(x * -1.44269504088896341f));
+}
+
+__device__ __forceinline__ float silu(float x) { return x * sigm(x); }
 
 __device__ __forceinline__ float silu_grad(float x)
 {
-    const float s = 1.0f / (1.0f + __expf(-x));
+    const float s = sigm(x);
     return s * (1.0f + x * (1.0f - s));
 }
 
@@ -102,73 +110,128 @@ __device__ __forceinline__ NodeCtx node_ctx(const sph_dev_params &P, unsigned lo
     return c;
 }
 
-// Compact the neighbours of particle `node` (one wavefront) into the LDS
-// edge buffer, features as bf16 rows of 16.  Returns the number of edges
-// kept (<= kEdgeCap); `dropped` counts the rest.
-__device__ unsigned gather_edges(unsigned node, const sph_dev_params &P, const float2 *POS,
+// LDS edge list of one wavefront's group of kGroup consecutive slots:
+// features (dx/h, dy/h, 0.2 dvx, 0.2 dvy, r/h, 1, 0, 0) as two bf16x4, and
+// the owning particle (0..15 within the group; 0xff for padding rows).
+struct EdgeBuf {
+    bf16x4 f[kGroupEdgeCap][2];
+    unsigned char owner[kGroupEdgeCap];
+};
+
+// Compact every edge of the particles [n0, n0 + 16) into L (one wavefront).
+// Lane l tests particle l % 16 against every fourth candidate (offset l / 16)
+// of each of the particle's three candidate rows (cells i-1..i+1 of rows
+// j-1..j+1, contiguous in the sorted arrays), so all 64 lanes test useful
+// pairs.  Edge order in L is arbitrary.  Lanes 0..15 get the degree and, with
+// DENS, the density sum (src/fluid.c:527-539: (1 - r/h)^2 for r/h < 1) of
+// particle n0 + lane.  Returns the number of edges kept; the rest are
+// counted in *dropped.
+template <bool DENS>
+__device__ unsigned gather_group(unsigned n0, const sph_dev_params &P, const float2 *POS,
                                  const float2 *VEL, const unsigned *K, const unsigned *start,
-                                 unsigned short (*ebuf)[kKE], unsigned *dropped)
+                                 EdgeBuf &L, unsigned *dropped, float &deg_out, float &rho_out)
 {
-    const float2 a = POS[node], av = VEL[node];
-    const unsigned key = K[node] - P.cell_base;
-    const unsigned sx = P.sx, sy = P.sy;
-    const unsigned i = key % sx, j = key / sx;
-    const unsigned i0 = i > 0 ? i - 1 : 0, i1 = i + 1 < sx ? i + 1 : sx - 1;
-    unsigned count = 0;
+    const unsigned l = lane(), p = l % 16u, sub = l / 16u;
+    const unsigned me = n0 + p;
+    const bool valid = me < P.poff + (unsigned)P.n;
+    float2 a = make_float2(0, 0), av = make_float2(0, 0);
+    unsigned i = 0, j = 0;
+    if (valid) {
+        a = POS[me];
+        av = VEL[me];
+        const unsigned key = K[me] - P.cell_base;
+        i = key % P.sx;
+        j = key / P.sx;
+    }
+    const unsigned lo = i > 0 ? i - 1 : 0, hi = i + 1 < P.sx ? i + 1 : P.sx - 1;
+    unsigned count = 0, mine = 0;
+    float dens = 0.0f;
     for (int dy = -1; dy <= 1; dy++) {
         const long r = (long)j + dy;
-        if (r < 0 || r >= (long)sy)
-            continue;
-        const unsigned base = P.cell_base + (unsigned)r * sx;
-        const unsigned b0 = start[base + i0], b3 = start[base + i1 + 1];
-        for (unsigned c = b0; c < b3; c += 64) {
-            const unsigned k = c + lane();
+        unsigned b0 = 0, b3 = 0;
+        if (valid && r >= 0 && r < (long)P.sy) {
+            const unsigned base = P.cell_base + (unsigned)r * P.sx;
+            b0 = start[base + lo];
+            b3 = start[base + hi + 1];
+        }
+        for (unsigned c = b0 + sub;; c += 4) {
+            const bool act = c < b3;
+            if (__ballot(act) == 0ull)
+                break;
             bool acc = false;
             float dx = 0, dyy = 0, r2 = 0;
-            if (k < b3 && k != node) {
-                const float2 q = POS[k];
+            if (act && c != me) {
+                const float2 q = POS[c];
                 dx = q.x - a.x;
                 dyy = q.y - a.y;
                 r2 = dx * dx + dyy * dyy;
                 acc = !(r2 > P.h2);
             }
             const unsigned long long m = __ballot(acc);
-            const unsigned before = (unsigned)__popcll(m & ((1ull << lane()) - 1ull));
             if (acc) {
-                const unsigned pos = count + before;
-                if (pos < (unsigned)kEdgeCap) {
-                    const float2 qv = VEL[k];
-                    const float f[6] = {dx * P.h_recip, dyy * P.h_recip, (qv.x - av.x) * kVScale,
-                                        (qv.y - av.y) * kVScale, sqrtf(r2) * P.h_recip, 1.0f};
-                    for (int q = 0; q < kKE; q++)
-                        ebuf[pos][q] = q < 6 ? f2bf(f[q]) : (unsigned short)0;
+                const unsigned pos =
+                    count + (unsigned)__popcll(m & ((1ull << l) - 1ull));
+                const float rr = sqrtf(r2);
+                if (DENS) {
+                    const float ratio = rr * P.h_recip;
+                    if (ratio < 1.0f)
+                        dens += (1.0f - ratio) * (1.0f - ratio);
+                }
+                mine++;
+                if (pos < kGroupEdgeCap) {
+                    const float2 qv = VEL[c];
+                    L.f[pos][0] = pack4(dx * P.h_recip, dyy * P.h_recip, (qv.x - av.x) * kVScale,
+                                        (qv.y - av.y) * kVScale);
+                    L.f[pos][1] = pack4(rr * P.h_recip, 1.0f, 0.0f, 0.0f);
+                    L.owner[pos] = (unsigned char)p;
                 }
             }
             count += (unsigned)__popcll(m);
         }
     }
-    if (count > (unsigned)kEdgeCap) {
-        if (lane() == 0)
-            atomicAdd(dropped, count - (unsigned)kEdgeCap);
-        count = kEdgeCap;
+    if (count > kGroupEdgeCap) {
+        if (l == 0)
+            atomicAdd(dropped, count - kGroupEdgeCap);
+        count = kGroupEdgeCap;
     }
-    // zero the rows that pad the last tile
+    // padding rows of the last 16-edge tile
     const unsigned padded = (count + 15u) & ~15u;
-    for (unsigned e = count + lane() / 16u; e < padded; e += 4)
-        ebuf[e][lane() % 16u] = 0;
+    if (count + l < padded) {
+        L.f[count + l][0] = pack4(0, 0, 0, 0);
+        L.f[count + l][1] = pack4(0, 0, 0, 0);
+        L.owner[count + l] = 0xffu;
+    }
+    float fm = (float)mine;
+    fm += __shfl_xor(fm, 16, 64);
+    fm += __shfl_xor(fm, 32, 64);
+    deg_out = fm;
+    if (DENS) {
+        dens += __shfl_xor(dens, 16, 64);
+        dens += __shfl_xor(dens, 32, 64);
+        rho_out = dens;
+    }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     return count;
 }
 
-// A-operand of an edge tile: lane l holds row (edge) e0 + l%16, features
-// 4*(l/16) .. +3.
-__device__ __forceinline__ bf16x4 edge_tile_a(const unsigned short (*ebuf)[kKE], unsigned e0)
+// A-operand of an edge tile: lane l holds edge e0 + l%16, features
+// 4*(l/16) .. +3 (features 8..15 are zero).
+__device__ __forceinline__ bf16x4 edge_tile_a(const EdgeBuf &L, unsigned e0)
 {
-    const unsigned e = e0 + lane() % 16u, k = 4u * (lane() / 16u);
+    const unsigned sub = lane() / 16u;
+    return sub < 2u ? L.f[e0 + lane() % 16u][sub] : pack4(0, 0, 0, 0);
+}
+
+// A-operand of the group membership matrix of an edge tile: lane l holds
+// row (particle) l%16, columns (edges) e0 + 4*(l/16) .. +3: 1 where the edge
+// belongs to the particle.
+__device__ __forceinline__ bf16x4 member_a(const EdgeBuf &L, unsigned e0)
+{
+    const unsigned p = lane() % 16u, e = e0 + 4u * (lane() / 16u);
     bf16x4 r;
     for (int q = 0; q < 4; q++)
-        r[q] = (short)ebuf[e][k + q];
+        r[q] = L.owner[e + q] == p ? (short)0x3f80 : (short)0;
     return r;
 }
 
@@ -204,6 +267,15 @@ struct sph_surrogate {
     bool ref_ok = false;
 };
 
+struct sph_trajectory {
+    int nscenes = 0, n = 0, frames = 0;
+    float4 *data = nullptr;    // [frames][nscenes][n]
+    float4 *staging = nullptr; // [cap_samples * n]
+    float2 *target = nullptr;  // [cap_samples * n]
+    int *idx = nullptr;        // scene[cap], frame[cap]
+    int cap = 0, loaded = 0;
+};
+
 // ---------------------------------------------------------------------------
 // kernels
 
@@ -225,52 +297,55 @@ __global__ void k_mfma_selftest(float *D)
         D[(4u * (l / 16u) + q) * 16u + l % 16u] = d[q];
 }
 
-// Edge forward: S_i = sum_j SiLU(W1 f_ij), deg_i.  One wavefront per particle.
+// Edge forward: S_i = sum_j SiLU(W1 f_ij) and deg_i (and, with DENS, the
+// density sum of the judge).  One wavefront per group of 16 slots: the
+// group's edges are compacted into LDS, each 16-edge tile runs
+// W1 on MFMA (4 hidden tiles), SiLU, and the per-particle sum as a second
+// MFMA against the tile's membership matrix.
+template <bool DENS>
 __global__ void __launch_bounds__(256)
 k_edge_fwd(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS, unsigned slots,
            const float2 *__restrict__ POS, const float2 *__restrict__ VEL,
            const unsigned *__restrict__ K, const unsigned *__restrict__ start,
            const unsigned short *__restrict__ W1b, float *__restrict__ S,
-           float *__restrict__ deg, unsigned *__restrict__ dropped)
+           float *__restrict__ deg, float *__restrict__ rho, unsigned *__restrict__ dropped)
 {
-    __shared__ unsigned short ebuf[4][kEdgeCap][kKE];
-    const unsigned w = threadIdx.x / 64u;
+    __shared__ EdgeBuf Lb[4];
+    const unsigned w = threadIdx.x / 64u, l = lane(), p = l % 16u, sub = l / 16u;
+    EdgeBuf &L = Lb[w];
     bf16x4 B[4];
     for (int t = 0; t < 4; t++)
         B[t] = w1_b(W1b, t);
-    const unsigned nw = gridDim.x * 4u;
-    for (unsigned node = blockIdx.x * 4u + w; node < slots; node += nw) {
-        const sph_dev_params &P = Pp[BS[node / kSphTPB]];
-        if (node >= P.poff + (unsigned)P.n) { // padding slot: zero rows for the node passes
-            if (lane() < 16u)
-                for (int t = 0; t < 4; t++)
-                    S[(size_t)node * kH + 16u * t + lane()] = 0.0f;
-            if (lane() == 0)
-                deg[node] = 0.0f;
-            continue;
-        }
-        const unsigned ne = gather_edges(node, P, POS, VEL, K, start, ebuf[w], dropped);
-        float acc[4] = {0, 0, 0, 0};
-        for (unsigned e0 = 0; e0 < ne; e0 += 16) {
-            const bf16x4 a = edge_tile_a(ebuf[w], e0);
-            for (int t = 0; t < 4; t++) {
-                f32x4 d = {0, 0, 0, 0};
-                d = mfma(a, B[t], d);
-                for (int q = 0; q < 4; q++) {
-                    const unsigned e = e0 + 4u * (lane() / 16u) + q;
-                    acc[t] += e < ne ? silu(d[q]) : 0.0f;
+    const unsigned ngroups = slots / kGroup, nw = gridDim.x * 4u;
+    for (unsigned g = blockIdx.x * 4u + w; g < ngroups; g += nw) {
+        const unsigned n0 = g * kGroup;
+        const sph_dev_params &P = Pp[BS[n0 / kSphTPB]];
+        f32x4 acc[4];
+        for (int t = 0; t < 4; t++)
+            acc[t] = {0, 0, 0, 0};
+        float dg = 0.0f, rh = 0.0f;
+        if (n0 < P.poff + (unsigned)P.n) {
+            const unsigned ne = gather_group<DENS>(n0, P, POS, VEL, K, start, L, dropped, dg, rh);
+            for (unsigned e0 = 0; e0 < ne; e0 += 16) {
+                const bf16x4 a = edge_tile_a(L, e0);
+                const bf16x4 mem = member_a(L, e0);
+                for (int t = 0; t < 4; t++) {
+                    f32x4 d = {0, 0, 0, 0};
+                    d = mfma(a, B[t], d);
+                    // D layout (edge 4*sub+q, unit 16t+p) is the B layout of the sum
+                    acc[t] = mfma(mem, pack4(silu(d[0]), silu(d[1]), silu(d[2]), silu(d[3])),
+                                  acc[t]);
                 }
             }
         }
-        for (int t = 0; t < 4; t++) {
-            acc[t] += __shfl_xor(acc[t], 16, 64);
-            acc[t] += __shfl_xor(acc[t], 32, 64);
+        for (int t = 0; t < 4; t++)
+            for (int q = 0; q < 4; q++)
+                S[(size_t)(n0 + 4u * sub + q) * kH + 16u * t + p] = acc[t][q];
+        if (l < 16u) {
+            deg[n0 + l] = dg;
+            if (DENS)
+                rho[n0 + l] = rh;
         }
-        if (lane() < 16u)
-            for (int t = 0; t < 4; t++)
-                S[(size_t)node * kH + 16u * t + lane()] = acc[t];
-        if (lane() == 0)
-            deg[node] = (float)ne;
         __builtin_amdgcn_wave_barrier();
     }
 }
@@ -550,10 +625,9 @@ k_node_bwd(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS, un
                     b[q] = (short)W2b[(k + q) * kH + c];
                 d = mfma(Am[kc], b, d);
             }
-            for (int q = 0; q < 4; q++) {
+            for (int q = 0; q < 4; q++) { // rows past the scene's end get 0
                 const unsigned row = n0 + 4u * (l / 16u) + q;
-                if (row < vend)
-                    dS[(size_t)row * kH + 16u * t + l % 16u] = d[q];
+                dS[(size_t)row * kH + 16u * t + l % 16u] = d[q];
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -631,6 +705,10 @@ k_node_wgrad(unsigned ntiles, const unsigned short *__restrict__ U,
 }
 
 // Edge backward: dW1 += sum over edges of (dS_i * SiLU'(W1 f_ij)) f_ij^T.
+// Same groups and tiles as the forward; dS_i is expanded to the tile's edges
+// by an MFMA of the one-hot edge->particle matrix with the group's dS, whose
+// D layout equals that of the recomputed pre-activations, and the product
+// with SiLU' is, in that layout, the A operand of dW1 += dPre1^T F.
 __global__ void __launch_bounds__(256)
 k_edge_bwd(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS, unsigned slots,
            const float2 *__restrict__ POS, const float2 *__restrict__ VEL,
@@ -638,48 +716,61 @@ k_edge_bwd(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS, un
            const unsigned short *__restrict__ W1b, const float *__restrict__ dS,
            float *__restrict__ grads, unsigned *__restrict__ dropped)
 {
-    __shared__ unsigned short ebuf[4][kEdgeCap][kKE];
-    const unsigned w = threadIdx.x / 64u, l = lane();
+    __shared__ EdgeBuf Lb[4];
+    const unsigned w = threadIdx.x / 64u, l = lane(), p = l % 16u, sub = l / 16u;
+    EdgeBuf &L = Lb[w];
     bf16x4 B[4];
     for (int t = 0; t < 4; t++)
         B[t] = w1_b(W1b, t);
     f32x4 g[4];
     for (int t = 0; t < 4; t++)
         g[t] = {0, 0, 0, 0};
-    const unsigned nw = gridDim.x * 4u;
-    for (unsigned node = blockIdx.x * 4u + w; node < slots; node += nw) {
-        const sph_dev_params &P = Pp[BS[node / kSphTPB]];
-        if (node >= P.poff + (unsigned)P.n)
+    const unsigned ngroups = slots / kGroup, nw = gridDim.x * 4u;
+    for (unsigned gi = blockIdx.x * 4u + w; gi < ngroups; gi += nw) {
+        const unsigned n0 = gi * kGroup;
+        const sph_dev_params &P = Pp[BS[n0 / kSphTPB]];
+        if (n0 >= P.poff + (unsigned)P.n)
             continue;
-        const unsigned ne = gather_edges(node, P, POS, VEL, K, start, ebuf[w], dropped);
-        float ds[4];
-        for (int t = 0; t < 4; t++)
-            ds[t] = dS[(size_t)node * kH + 16u * t + l % 16u];
+        float dg, rh;
+        const unsigned ne = gather_group<false>(n0, P, POS, VEL, K, start, L, dropped, dg, rh);
+        // B operand of the expansion: dS[particle 4*sub+i][unit 16t+p]
+        bf16x4 Bds[4];
+        for (int t = 0; t < 4; t++) {
+            const float *d0 = dS + (size_t)(n0 + 4u * sub) * kH + 16u * t + p;
+            Bds[t] = pack4(d0[0], d0[kH], d0[2 * kH], d0[3 * kH]);
+        }
         for (unsigned e0 = 0; e0 < ne; e0 += 16) {
-            const bf16x4 a = edge_tile_a(ebuf[w], e0);
-            // B-operand of the gradient product: F[edge][feature]
-            bf16x4 f;
+            const bf16x4 a = edge_tile_a(L, e0);
+            // one-hot A: row edge e0 + p, columns particles 4*sub .. +3
+            bf16x4 oh;
+            const unsigned own = L.owner[e0 + p];
             for (int q = 0; q < 4; q++)
-                f[q] = (short)ebuf[w][e0 + 4u * (l / 16u) + q][l % 16u];
+                oh[q] = own == 4u * sub + q ? (short)0x3f80 : (short)0;
+            // B operand of the gradient product: F[edge 4*sub+i][feature p]
+            bf16x4 f;
+            for (int q = 0; q < 4; q++) {
+                const bf16x4 fr = L.f[e0 + 4u * sub + q][(p / 4u) & 1u];
+                f[q] = p < 8u ? fr[p % 4u] : (short)0;
+            }
             for (int t = 0; t < 4; t++) {
                 f32x4 d = {0, 0, 0, 0};
                 d = mfma(a, B[t], d);
-                // D layout = A layout of dPre1^T (unit x edge)
+                f32x4 ds = {0, 0, 0, 0};
+                ds = mfma(oh, Bds[t], ds);
                 float gp[4];
-                for (int q = 0; q < 4; q++) {
-                    const unsigned e = e0 + 4u * (l / 16u) + q;
-                    gp[q] = e < ne ? ds[t] * silu_grad(d[q]) : 0.0f;
-                }
+                for (int q = 0; q < 4; q++)
+                    gp[q] = ds[q] * silu_grad(d[q]);
                 g[t] = mfma(pack4(gp[0], gp[1], gp[2], gp[3]), f, g[t]);
             }
         }
         __builtin_amdgcn_wave_barrier();
     }
-    for (int t = 0; t < 4; t++)
-        for (int q = 0; q < 4; q++) {
-            const unsigned unit = 16u * t + 4u * (l / 16u) + q, feat = l % 16u;
-            atomicAdd(&grads[oW1 + unit * kKE + feat], g[t][q]);
-        }
+    if (p < 6u)
+        for (int t = 0; t < 4; t++)
+            for (int q = 0; q < 4; q++) {
+                const unsigned unit = 16u * t + 4u * sub + q;
+                atomicAdd(&grads[oW1 + unit * kKE + p], g[t][q]);
+            }
 }
 
 // Adam on the fp32 master weights, then refresh the bf16 copies.  Weights
@@ -842,6 +933,24 @@ __global__ void k_judge(const sph_dev_params *__restrict__ Pp, int nscenes,
         ref[s] = mean;
 }
 
+// Trajectory sampling: sample k takes frame f = frame[k] of scene s =
+// scene[k]; staging gets its (x, y, v_x, v_y) records, target the 4-step
+// displacement to frame f + 1.  data is [frames][nscenes][n] float4.
+__global__ void k_traj_sample(const float4 *__restrict__ data, unsigned nscenes, unsigned n,
+                              unsigned count, const int *__restrict__ scene,
+                              const int *__restrict__ frame, float4 *__restrict__ staging,
+                              float2 *__restrict__ target)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)count * n)
+        return;
+    const unsigned k = (unsigned)(i / n), j = (unsigned)(i % n);
+    const size_t src = ((size_t)frame[k] * nscenes + (unsigned)scene[k]) * n + j;
+    const float4 a = data[src], b = data[src + (size_t)nscenes * n];
+    staging[i] = a;
+    target[i] = make_float2(b.x - a.x, b.y - a.y);
+}
+
 // ---------------------------------------------------------------------------
 // host side
 
@@ -951,8 +1060,9 @@ int forward(sph_surrogate *m, sph_ctx *c)
     if (int rc = ensure_buffers(m, c->slots))
         return rc;
     const unsigned S = c->slots;
-    k_edge_fwd<<<dim3(2048), 256, 0, c->st>>>(c->dP, c->BS, S, c->POS0, c->VEL, c->K0, c->start,
-                                              m->W1b, m->S, m->deg, m->dropped);
+    k_edge_fwd<false><<<dim3(2048), 256, 0, c->st>>>(c->dP, c->BS, S, c->POS0, c->VEL, c->K0,
+                                                     c->start, m->W1b, m->S, m->deg, nullptr,
+                                                     m->dropped);
     k_node_fwd<<<dim3(2048), 256, 0, c->st>>>(c->dP, c->BS, S, c->dclock, c->VEL, m->S, m->deg,
                                               m->W2b, m->params, m->W3b, (float2 *)m->disp);
     MCHK(hipGetLastError());
@@ -1101,17 +1211,14 @@ int sph_surrogate_predict(sph_surrogate *m, sph_ctx *c, float *disp_by_id)
 // (a device buffer in caller order, 2 floats per particle), backward; gradients are accumulated into the gradient
 // buffer (zero it with sph_surrogate_zero_grads).  *loss gets the mean of
 // |d - target|^2 / 0.05^2 over the particles.
-int sph_surrogate_backward(sph_surrogate *m, sph_ctx *c, const float *target_by_id, float *loss)
+static int backward_dev(sph_surrogate *m, sph_ctx *c, const float2 *target_dev, float *loss)
 {
-    if (!m || !c || (!target_by_id && c->n_total))
-        return merr(SPH_EINVAL, "null argument");
     if (c->n_total == 0)
         return SPH_OK;
     if (int rc = forward(m, c))
         return rc;
     const unsigned S = c->slots;
-    k_by_id_to_slot<<<dim3(S / 256), 256, 0, c->st>>>(c->dP, c->BS, S, c->ID0,
-                                                     (const float2 *)target_by_id,
+    k_by_id_to_slot<<<dim3(S / 256), 256, 0, c->st>>>(c->dP, c->BS, S, c->ID0, target_dev,
                                                      (float2 *)m->tgt);
     MCHK(hipMemsetAsync(m->loss, 0, sizeof(float), c->st));
     k_node_bwd<<<dim3(2048), 256, 0, c->st>>>(c->dP, c->BS, S, c->dclock, c->VEL, m->S, m->deg,
@@ -1126,6 +1233,135 @@ int sph_surrogate_backward(sph_surrogate *m, sph_ctx *c, const float *target_by_
         MCHK(hipMemcpyAsync(loss, m->loss, sizeof(float), hipMemcpyDeviceToHost, c->st));
         MCHK(hipStreamSynchronize(c->st));
     }
+    return SPH_OK;
+}
+
+int sph_surrogate_backward(sph_surrogate *m, sph_ctx *c, const float *target_by_id, float *loss)
+{
+    if (!m || !c || (!target_by_id && c->n_total))
+        return merr(SPH_EINVAL, "null argument");
+    if (c->n_total == 0)
+        return SPH_OK;
+    if (int rc = ensure_buffers(m, c->slots))
+        return rc;
+    MCHK(hipStreamSynchronize(c->st)); // tmp2 may still be read by earlier work
+    MCHK(hipMemcpy(m->tmp2, target_by_id, sizeof(float2) * (size_t)c->n_total,
+                   hipMemcpyHostToDevice));
+    return backward_dev(m, c, m->tmp2, loss);
+}
+
+int sph_surrogate_backward_trajectory(sph_surrogate *m, sph_ctx *c, sph_trajectory *tr,
+                                      float *loss)
+{
+    if (!m || !c || !tr)
+        return merr(SPH_EINVAL, "null argument");
+    if (tr->loaded != (int)c->sc.size() || (long)tr->loaded * tr->n != (long)c->n_total)
+        return merr(SPH_ESTATE, "the context does not hold the trajectory's last loaded samples");
+    return backward_dev(m, c, tr->target, loss);
+}
+
+// ---- trajectory store ------------------------------------------------------
+
+int sph_trajectory_create(sph_trajectory **out, int nscenes, int n, int frames)
+{
+    if (!out || nscenes <= 0 || n <= 0 || frames < 2)
+        return merr(SPH_EINVAL, "sph_trajectory_create: bad sizes");
+    *out = nullptr;
+    sph_trajectory *t = new (std::nothrow) sph_trajectory();
+    if (!t)
+        return merr(SPH_ENOMEM, "host allocation failed");
+    t->nscenes = nscenes;
+    t->n = n;
+    t->frames = frames;
+    if (int rc = malloc_dev(&t->data, (size_t)frames * nscenes * n)) {
+        sph_trajectory_destroy(t);
+        return rc;
+    }
+    *out = t;
+    return SPH_OK;
+}
+
+void sph_trajectory_destroy(sph_trajectory *t)
+{
+    if (!t)
+        return;
+    void *ptrs[] = {t->data, t->staging, t->target, t->idx};
+    for (void *p : ptrs)
+        if (p)
+            (void)hipFree(p);
+    delete t;
+}
+
+int sph_trajectory_record(sph_trajectory *t, sph_ctx *c, int frame)
+{
+    if (!t || !c || frame < 0 || frame >= t->frames)
+        return merr(SPH_EINVAL, "sph_trajectory_record: bad argument");
+    if ((int)c->sc.size() != t->nscenes || (long)c->n_total != (long)t->nscenes * t->n)
+        return merr(SPH_EINVAL, "sph_trajectory_record: context is not %d scenes x %d particles",
+                    t->nscenes, t->n);
+    for (const auto &S : c->sc)
+        if (S.n != t->n)
+            return merr(SPH_EINVAL, "sph_trajectory_record: scenes of unequal size");
+    if (sph_gather_state_dev(c, (float *)(t->data + (size_t)frame * t->nscenes * t->n)))
+        return merr(SPH_EHIP, "gather failed: %s", sph_last_error());
+    return SPH_OK;
+}
+
+int sph_trajectory_read(sph_trajectory *t, int frame, float *out)
+{
+    if (!t || !out || frame < 0 || frame >= t->frames)
+        return merr(SPH_EINVAL, "sph_trajectory_read: bad argument");
+    MCHK(hipDeviceSynchronize());
+    MCHK(hipMemcpy(out, t->data + (size_t)frame * t->nscenes * t->n,
+                   sizeof(float4) * (size_t)t->nscenes * t->n, hipMemcpyDeviceToHost));
+    return SPH_OK;
+}
+
+int sph_trajectory_load(sph_trajectory *t, sph_ctx *c, int count, const int *scene,
+                        const int *frame)
+{
+    if (!t || !c || !scene || !frame || count <= 0)
+        return merr(SPH_EINVAL, "sph_trajectory_load: bad argument");
+    if ((int)c->sc.size() != count || (long)c->n_total != (long)count * t->n)
+        return merr(SPH_EINVAL, "sph_trajectory_load: context must hold %d scenes of %d particles",
+                    count, t->n);
+    for (const auto &S : c->sc)
+        if (S.n != t->n)
+            return merr(SPH_EINVAL, "sph_trajectory_load: scenes of unequal size");
+    for (int k = 0; k < count; k++)
+        if (scene[k] < 0 || scene[k] >= t->nscenes || frame[k] < 0 || frame[k] + 1 >= t->frames)
+            return merr(SPH_EINVAL, "sph_trajectory_load: sample %d (scene %d, frame %d) out of range",
+                        k, scene[k], frame[k]);
+    if (t->cap < count) {
+        void *old[] = {t->staging, t->target, t->idx};
+        for (void *p : old)
+            if (p)
+                (void)hipFree(p);
+        t->staging = nullptr;
+        t->target = nullptr;
+        t->idx = nullptr;
+        t->cap = 0;
+        int rc = malloc_dev(&t->staging, (size_t)count * t->n);
+        if (!rc) rc = malloc_dev(&t->target, (size_t)count * t->n);
+        if (!rc) rc = malloc_dev(&t->idx, 2 * (size_t)count);
+        if (rc)
+            return rc;
+        t->cap = count;
+    }
+    std::vector<int> hidx((size_t)2 * count);
+    memcpy(hidx.data(), scene, sizeof(int) * count);
+    memcpy(hidx.data() + count, frame, sizeof(int) * count);
+    MCHK(hipStreamSynchronize(c->st));
+    MCHK(hipMemcpyAsync(t->idx, hidx.data(), sizeof(int) * 2 * count, hipMemcpyHostToDevice, c->st));
+    const size_t tot = (size_t)count * t->n;
+    k_traj_sample<<<dim3((unsigned)((tot + 255) / 256)), 256, 0, c->st>>>(
+        t->data, (unsigned)t->nscenes, (unsigned)t->n, (unsigned)count, t->idx, t->idx + count,
+        t->staging, t->target);
+    MCHK(hipGetLastError());
+    MCHK(hipStreamSynchronize(c->st));
+    if (sph_load_state_dev(c, (const float *)t->staging))
+        return merr(SPH_EHIP, "load failed: %s", sph_last_error());
+    t->loaded = count;
     return SPH_OK;
 }
 

@@ -1,11 +1,11 @@
 """Training and inference drivers of the learned macro-step (sph_surrogate_*).
 
-Everything here runs on the GPU through libsph_hip.so; torch only holds the
-trajectory data in HBM and does index arithmetic on it.
+Everything runs on the GPU through libsph_hip.so: the trajectories stay in
+device memory in the library's own trajectory store (sph_trajectory_*).
 
-* ``generate``  - run the plain GPU solver over a batch of scenes and keep
+* ``generate``  - run the plain GPU solver over a batch of scenes and record
   the state (x, y, v_x, v_y) every MACRO steps: the training trajectories.
-* ``train``     - Adam on batches of (scene, time) samples drawn from the
+* ``train``     - Adam on batches of (scene, frame) samples drawn from the
   trajectories: load the sampled states into a batch context, hash, one
   forward/backward on the device, one Adam step.
 * ``evaluate``  - 4-step displacement error of the network against the
@@ -22,20 +22,13 @@ import time
 
 import numpy as np
 
-from . import Solver, Surrogate
+from . import Solver, Surrogate, Trajectory
 from . import scenes as sc
 
 MACRO = 4
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(PKG_DIR, "configs", "surrogate_params.npy")
 WEIGHTS_META = os.path.join(PKG_DIR, "configs", "surrogate_params.json")
-
-
-def _torch():
-    import torch
-    if not torch.cuda.is_available():
-        raise RuntimeError("the learned macro-step needs a GPU")
-    return torch
 
 
 def make_batch(scenes, upload=True):
@@ -50,91 +43,85 @@ def make_batch(scenes, upload=True):
 
 
 def generate(scenes, steps=200, every=MACRO):
-    """Plain-solver trajectories: a float32 CUDA tensor [steps/every + 1,
-    nscenes, n, 4] of (x, y, v_x, v_y) by particle id (all scenes of equal n)."""
-    torch = _torch()
+    """Plain-solver trajectories of equal-size scenes: a Trajectory with
+    steps / every + 1 frames, and the solver's health counters."""
     n = len(scenes[0][2])
     assert all(len(p) == n for _, _, p, _, _ in scenes)
     s = make_batch(scenes)
     T = steps // every
-    data = torch.empty((T + 1, len(scenes), n, 4), dtype=torch.float32, device="cuda")
-    s.gather_state(data[0].view(-1, 4))
+    tr = Trajectory(len(scenes), n, T + 1)
+    tr.record(s, 0)
     for t in range(1, T + 1):
         s.step(every)
-        s.gather_state(data[t].view(-1, 4))
+        tr.record(s, t)
     c = s.counters()
     s.close()
-    return data, c
+    return tr, c
 
 
-def _load_samples(ctx, data, scenes, s_sel, t_sel, every=MACRO):
-    """Load states data[t, s] of the sampled (scene, time) pairs into the batch
-    context (one sample per context scene) and hash them; returns the target
-    4-step displacements as a CUDA tensor [B * n, 2]."""
-    torch = _torch()
-    ts = torch.as_tensor(t_sel, device="cuda")
-    ss = torch.as_tensor(s_sel, device="cuda")
-    xs = data[ts, ss].reshape(-1, 4).contiguous()
-    tgt = (data[ts + 1, ss, :, :2].reshape(-1, 2) - xs[:, :2]).contiguous()
-    ctx.load_state(xs)
-    for k, (s, t) in enumerate(zip(s_sel, t_sel)):
+def _load_samples(ctx, tr, scenes, s_sel, f_sel, every=MACRO):
+    """Load frames f_sel of scenes s_sel into the batch context (one sample
+    per context scene), set each sample's sloshing and clock, and hash."""
+    tr.load(ctx, s_sel, f_sel)
+    for k, (s, f) in enumerate(zip(s_sel, f_sel)):
         _, _, _, a, T = scenes[int(s)]
         ctx.set_scene_forcing(k, a, T)
-        ctx.set_scene_time_offset(k, int(t) * every)
+        ctx.set_scene_time_offset(k, int(f) * every)
     ctx.phase("hash")
-    return tgt
 
 
-def train(model, data, scenes, iters, batch=64, lr=1e-3, seed=0, log=None, log_every=100):
-    """Adam on random (scene, time) samples; returns the loss history."""
-    rng = np.random.default_rng(seed)
-    T = data.shape[0] - 1
-    S = data.shape[1]
-    ctx = make_batch(scenes[:batch], upload=False)
-    # one upload to give every scene its particle count
+def _batch_ctx(scenes, batch):
+    ctx = Solver.batch([(t, b, len(p)) for t, b, p, _, _ in scenes[:batch]])
     ctx.upload_batch(np.concatenate([p for _, _, p, _, _ in scenes[:batch]]),
                      [len(p) for _, _, p, _, _ in scenes[:batch]])
+    return ctx
+
+
+def train(model, tr, scenes, iters, batch=64, lr=1e-3, seed=0, log=None, log_every=100):
+    """Adam on random (scene, frame) samples; returns the loss history."""
+    rng = np.random.default_rng(seed)
+    ctx = _batch_ctx(scenes, batch)
     hist = []
     t0 = time.time()
     for it in range(iters):
-        s_sel = rng.choice(S, size=batch, replace=False)
-        t_sel = rng.integers(0, T, size=batch)
-        tgt = _load_samples(ctx, data, scenes, s_sel, t_sel)
+        s_sel = rng.choice(tr.nscenes, size=batch, replace=False)
+        f_sel = rng.integers(0, tr.frames - 1, size=batch)
+        _load_samples(ctx, tr, scenes, s_sel, f_sel)
         model.zero_grads()
-        loss = model.backward(ctx, tgt)
+        hist.append(model.backward_trajectory(ctx, tr))
         model.adam(lr)
-        hist.append(loss)
         if log and (it % log_every == 0 or it == iters - 1):
             log(f"iter {it} loss {np.mean(hist[-log_every:]):.4f} ({time.time() - t0:.1f} s)")
     ctx.close()
     return hist
 
 
-def evaluate(model, data, scenes, times, batch=None):
+def ballistic_model(model):
+    """The method without the network: the same model with its output layer
+    (W4, b4: the last 2 x 64 + 2 parameters) zeroed."""
+    z = Surrogate(seed=0)
+    p = model.params.copy()
+    p[-(2 * 64 + 2):] = 0.0
+    z.params = p
+    return z
+
+
+def evaluate(model, tr, scenes, frames, batch=None):
     """RMS 4-step displacement error (units of h) of the network and of the
-    ballistic extrapolation (4 dt v + 10 dt^2 a: the network's output
-    zeroed) over the given scenes and times."""
-    torch = _torch()
-    S = data.shape[1]
-    batch = batch or S
-    ctx = make_batch(scenes[:batch], upload=False)
-    ctx.upload_batch(np.concatenate([p for _, _, p, _, _ in scenes[:batch]]),
-                     [len(p) for _, _, p, _, _ in scenes[:batch]])
-    zero = Surrogate(seed=0)
-    p0 = model.params
-    pz = p0.copy()
-    n_out = 2 * 64 + 2  # W4 and b4 are the last 130 parameters
-    pz[-n_out:] = 0.0
-    zero.params = pz
+    ballistic extrapolation over all scenes of the trajectory at the given
+    frames."""
+    S = tr.nscenes
+    batch = min(batch or S, S)
+    ctx = _batch_ctx(scenes, batch)
+    zero = ballistic_model(model)
     se_net = se_bal = 0.0
     cnt = 0
-    for t in times:
-        for b0 in range(0, S, batch):
-            s_sel = np.arange(b0, min(S, b0 + batch))
-            if len(s_sel) < batch:
-                break
-            tgt = _load_samples(ctx, data, scenes, s_sel, np.full(len(s_sel), t))
-            y = tgt.cpu().numpy().astype(np.float64)
+    for f in frames:
+        a, b = tr.read(f), tr.read(f + 1)
+        for b0 in range(0, S - batch + 1, batch):
+            s_sel = np.arange(b0, b0 + batch)
+            _load_samples(ctx, tr, scenes, s_sel, np.full(batch, f))
+            y = (b[s_sel, :, :2] - a[s_sel, :, :2]).reshape(-1, 2).astype(np.float64)
             d_net = model.predict(ctx).astype(np.float64)
             d_bal = zero.predict(ctx).astype(np.float64)
             se_net += ((d_net - y) ** 2).sum()
@@ -147,10 +134,8 @@ def evaluate(model, data, scenes, times, batch=None):
 
 def run_judged(model, solver, n_macro, tol=0.01):
     """n_macro judged macro-steps; returns (seconds, fallback flags per macro-step)."""
-    torch = _torch()
     flags = []
     solver.sync()
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(n_macro):
         flags.append(model.macro_step(solver, tol))
@@ -190,14 +175,15 @@ def main(argv=None):
     t0 = time.time()
     scn = sc.sloshing_dam_breaks(args.scenes + args.holdout, n=args.n, seed=args.seed)
     train_s, hold_s = scn[:args.scenes], scn[args.scenes:]
-    data, c = generate(train_s, steps=args.steps)
-    log(f"generated {tuple(data.shape)} in {time.time() - t0:.1f} s; counters {c}")
+    tr, c = generate(train_s, steps=args.steps)
+    log(f"generated {tr.nscenes} x {tr.n} x {tr.frames} frames in {time.time() - t0:.1f} s; "
+        f"counters {c}")
     hold, _ = generate(hold_s, steps=args.steps)
     model = Surrogate(seed=args.seed)
-    hist = train(model, data, train_s, args.iters, batch=args.batch, lr=args.lr, seed=args.seed,
+    hist = train(model, tr, train_s, args.iters, batch=args.batch, lr=args.lr, seed=args.seed,
                  log=log)
-    times = list(range(0, hold.shape[0] - 1, 5))
-    net, bal = evaluate(model, hold, hold_s, times, batch=min(args.batch, args.holdout))
+    frames = list(range(0, hold.frames - 1, 5))
+    net, bal = evaluate(model, hold, hold_s, frames, batch=min(args.batch, args.holdout))
     log(f"held-out 4-step displacement rms: network {net:.5f} h, ballistic {bal:.5f} h")
     meta = dict(scenes=args.scenes, n=args.n, steps=args.steps, iters=args.iters, batch=args.batch,
                 lr=args.lr, seed=args.seed, final_loss=float(np.mean(hist[-100:])),
