@@ -416,7 +416,6 @@ int sph_surrogate_apply(sph_surrogate *m, sph_ctx *ctx);
  * nscenes ints) marks the scenes re-taken by four plain steps. */
 int sph_surrogate_macro_step(sph_surrogate *m, sph_ctx *ctx, float tol, int *flags,
                              int *nfallback);
-unsigned sph_surrogate_dropped_edges(sph_surrogate *m);
 /* D = A B of one 16x16x16 BF16 MFMA with fixed small-integer operands,
  * row-major (checks the lane layout the kernels assume). */
 int sph_surrogate_mfma_selftest(float *out256);

@@ -144,7 +144,6 @@ def lib():
                                         ctypes.c_float]),
             "sph_surrogate_apply": (ip, [vp, vp]),
             "sph_surrogate_macro_step": (ip, [vp, vp, ctypes.c_float, vp, ctypes.POINTER(ip)]),
-            "sph_surrogate_dropped_edges": (ctypes.c_uint, [vp]),
             "sph_surrogate_mfma_selftest": (ip, [vp]),
             "sph_surrogate_backward_trajectory": (ip, [vp, vp, vp, ctypes.POINTER(ctypes.c_float)]),
             "sph_trajectory_create": (ip, [ctypes.POINTER(vp), ip, ip, ip]),
@@ -437,9 +436,6 @@ class Surrogate:
         _check_m(lib().sph_surrogate_macro_step(self._h, solver._h, tol, flags.ctypes.data,
                                                 ctypes.byref(nf)), "macro_step")
         return flags
-
-    def dropped_edges(self) -> int:
-        return int(lib().sph_surrogate_dropped_edges(self._h))
 
 
 class Trajectory:

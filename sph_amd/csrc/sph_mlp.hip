@@ -33,6 +33,7 @@
 
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 typedef __attribute__((ext_vector_type(4))) short bf16x4;
+typedef __attribute__((ext_vector_type(4))) __bf16 bfx4;
 
 namespace {
 
@@ -40,8 +41,9 @@ constexpr int kH = 64;      // hidden width
 constexpr int kKE = 16;     // edge input width (6 used)
 constexpr int kKN = 80;     // node input width (64 + 4 + bias, padded)
 constexpr unsigned kGroup = 16;         // consecutive slots per wavefront in the edge passes
-constexpr unsigned kGroupEdgeCap = 512; // edges of one group kept (more are counted, dropped)
+constexpr unsigned kEdgeBuf = 256;      // LDS edge rows per wavefront (flushed as it fills)
 constexpr float kVScale = 0.2f, kAScale = 0.1f, kDScale = 0.05f;
+constexpr int kTrendRefresh = 16; // forced plain macro-step per scene every this many
 
 // parameter block (fp32 master copy)
 constexpr int oW1 = 0;                  // [64][16]
@@ -52,13 +54,10 @@ constexpr int oW4 = oW3 + kH * kKN;     // [2][64]
 constexpr int ob4 = oW4 + 2 * kH;       // [2]
 constexpr int kNParams = ob4 + 2;
 
+// float -> bf16 bits, round to nearest even (v_cvt_pk_bf16_f32 on gfx950)
 __device__ __forceinline__ unsigned short f2bf(float f)
 {
-    unsigned u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u) // inf / nan: truncate, keep a nan a nan
-        return (unsigned short)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-    u += 0x7fffu + ((u >> 16) & 1u); // round to nearest even
-    return (unsigned short)(u >> 16);
+    return __builtin_bit_cast(unsigned short, (__bf16)f);
 }
 
 __device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
@@ -80,12 +79,8 @@ __device__ __forceinline__ float silu_grad(float x)
 
 __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d)
 {
-    bf16x4 r;
-    r[0] = (short)f2bf(a);
-    r[1] = (short)f2bf(b);
-    r[2] = (short)f2bf(c);
-    r[3] = (short)f2bf(d);
-    return r;
+    const f32x4 v = {a, b, c, d};
+    return __builtin_bit_cast(bf16x4, __builtin_convertvector(v, bfx4));
 }
 
 __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c)
@@ -114,22 +109,24 @@ __device__ __forceinline__ NodeCtx node_ctx(const sph_dev_params &P, unsigned lo
 // features (dx/h, dy/h, 0.2 dvx, 0.2 dvy, r/h, 1, 0, 0) as two bf16x4, and
 // the owning particle (0..15 within the group; 0xff for padding rows).
 struct EdgeBuf {
-    bf16x4 f[kGroupEdgeCap][2];
-    unsigned char owner[kGroupEdgeCap];
+    bf16x4 f[kEdgeBuf][2];
+    unsigned char owner[kEdgeBuf];
 };
 
-// Compact every edge of the particles [n0, n0 + 16) into L (one wavefront).
-// Lane l tests particle l % 16 against every fourth candidate (offset l / 16)
-// of each of the particle's three candidate rows (cells i-1..i+1 of rows
-// j-1..j+1, contiguous in the sorted arrays), so all 64 lanes test useful
-// pairs.  Edge order in L is arbitrary.  Lanes 0..15 get the degree and, with
-// DENS, the density sum (src/fluid.c:527-539: (1 - r/h)^2 for r/h < 1) of
-// particle n0 + lane.  Returns the number of edges kept; the rest are
-// counted in *dropped.
-template <bool DENS>
-__device__ unsigned gather_group(unsigned n0, const sph_dev_params &P, const float2 *POS,
-                                 const float2 *VEL, const unsigned *K, const unsigned *start,
-                                 EdgeBuf &L, unsigned *dropped, float &deg_out, float &rho_out)
+// Stream every edge of the particles [n0, n0 + 16) through L (one
+// wavefront).  Lane l tests particle l % 16 against every fourth candidate
+// (offset l / 16) of each of the particle's three candidate rows (cells
+// i-1..i+1 of rows j-1..j+1, contiguous in the sorted arrays), so all 64
+// lanes test useful pairs.  Accepted edges are appended to L in arbitrary
+// order; whenever L is nearly full, flush(nedges) consumes its first nedges
+// (a multiple of 16) and the remainder moves to the front.  The final call
+// gets the rest, padded to a multiple of 16 with zero rows owned by 0xff.
+// Lanes 0..15 get the degree and, with DENS, the density sum
+// (src/fluid.c:527-539: (1 - r/h)^2 for r/h < 1) of particle n0 + lane.
+template <bool DENS, class Flush>
+__device__ void walk_group(unsigned n0, const sph_dev_params &P, const float2 *POS,
+                           const float2 *VEL, const unsigned *K, const unsigned *start,
+                           EdgeBuf &L, Flush &&flush, float &deg_out, float &rho_out)
 {
     const unsigned l = lane(), p = l % 16u, sub = l / 16u;
     const unsigned me = n0 + p;
@@ -169,8 +166,7 @@ __device__ unsigned gather_group(unsigned n0, const sph_dev_params &P, const flo
             }
             const unsigned long long m = __ballot(acc);
             if (acc) {
-                const unsigned pos =
-                    count + (unsigned)__popcll(m & ((1ull << l) - 1ull));
+                const unsigned pos = count + (unsigned)__popcll(m & ((1ull << l) - 1ull));
                 const float rr = sqrtf(r2);
                 if (DENS) {
                     const float ratio = rr * P.h_recip;
@@ -178,29 +174,49 @@ __device__ unsigned gather_group(unsigned n0, const sph_dev_params &P, const flo
                         dens += (1.0f - ratio) * (1.0f - ratio);
                 }
                 mine++;
-                if (pos < kGroupEdgeCap) {
-                    const float2 qv = VEL[c];
-                    L.f[pos][0] = pack4(dx * P.h_recip, dyy * P.h_recip, (qv.x - av.x) * kVScale,
-                                        (qv.y - av.y) * kVScale);
-                    L.f[pos][1] = pack4(rr * P.h_recip, 1.0f, 0.0f, 0.0f);
-                    L.owner[pos] = (unsigned char)p;
-                }
+                const float2 qv = VEL[c];
+                L.f[pos][0] = pack4(dx * P.h_recip, dyy * P.h_recip, (qv.x - av.x) * kVScale,
+                                    (qv.y - av.y) * kVScale);
+                L.f[pos][1] = pack4(rr * P.h_recip, 1.0f, 0.0f, 0.0f);
+                L.owner[pos] = (unsigned char)p;
             }
             count += (unsigned)__popcll(m);
+            if (count > kEdgeBuf - 64u) { // the next batch might not fit: flush full tiles
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                const unsigned full = count & ~15u, rem = count - full;
+                flush(full);
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                bf16x4 f0 = pack4(0, 0, 0, 0), f1 = f0;
+                unsigned char ow = 0xffu;
+                if (l < rem) {
+                    f0 = L.f[full + l][0];
+                    f1 = L.f[full + l][1];
+                    ow = L.owner[full + l];
+                }
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                if (l < rem) {
+                    L.f[l][0] = f0;
+                    L.f[l][1] = f1;
+                    L.owner[l] = ow;
+                }
+                count = rem;
+            }
         }
     }
-    if (count > kGroupEdgeCap) {
-        if (l == 0)
-            atomicAdd(dropped, count - kGroupEdgeCap);
-        count = kGroupEdgeCap;
-    }
-    // padding rows of the last 16-edge tile
+    // the rest, padded to a whole 16-edge tile
     const unsigned padded = (count + 15u) & ~15u;
     if (count + l < padded) {
         L.f[count + l][0] = pack4(0, 0, 0, 0);
         L.f[count + l][1] = pack4(0, 0, 0, 0);
         L.owner[count + l] = 0xffu;
     }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    if (padded)
+        flush(padded);
     float fm = (float)mine;
     fm += __shfl_xor(fm, 16, 64);
     fm += __shfl_xor(fm, 32, 64);
@@ -211,8 +227,6 @@ __device__ unsigned gather_group(unsigned n0, const sph_dev_params &P, const flo
         rho_out = dens;
     }
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    return count;
 }
 
 // A-operand of an edge tile: lane l holds edge e0 + l%16, features
@@ -254,17 +268,17 @@ struct sph_surrogate {
     float *S = nullptr, *deg = nullptr, *dS = nullptr, *disp = nullptr, *tgt = nullptr;
     unsigned short *U = nullptr, *dP3 = nullptr, *dM = nullptr; // tile-major bf16
     float *loss = nullptr;
-    unsigned *dropped = nullptr;
     float2 *tmp2 = nullptr;
     int adam_t = 0;
     bool have_pred = false;
     // judge state (per scene)
     size_t ns = 0;
     float *ref = nullptr, *jsum = nullptr;
-    unsigned *jmax = nullptr;
+    float *jmax = nullptr;
     int *jflags = nullptr;
     const sph_ctx *ref_ctx = nullptr;
     bool ref_ok = false;
+    std::vector<int> since_plain; // macro-steps since each scene's last plain macro-step
 };
 
 struct sph_trajectory {
@@ -308,7 +322,7 @@ k_edge_fwd(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS, un
            const float2 *__restrict__ POS, const float2 *__restrict__ VEL,
            const unsigned *__restrict__ K, const unsigned *__restrict__ start,
            const unsigned short *__restrict__ W1b, float *__restrict__ S,
-           float *__restrict__ deg, float *__restrict__ rho, unsigned *__restrict__ dropped)
+           float *__restrict__ deg, float *__restrict__ rho)
 {
     __shared__ EdgeBuf Lb[4];
     const unsigned w = threadIdx.x / 64u, l = lane(), p = l % 16u, sub = l / 16u;
@@ -325,18 +339,20 @@ k_edge_fwd(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS, un
             acc[t] = {0, 0, 0, 0};
         float dg = 0.0f, rh = 0.0f;
         if (n0 < P.poff + (unsigned)P.n) {
-            const unsigned ne = gather_group<DENS>(n0, P, POS, VEL, K, start, L, dropped, dg, rh);
-            for (unsigned e0 = 0; e0 < ne; e0 += 16) {
-                const bf16x4 a = edge_tile_a(L, e0);
-                const bf16x4 mem = member_a(L, e0);
-                for (int t = 0; t < 4; t++) {
-                    f32x4 d = {0, 0, 0, 0};
-                    d = mfma(a, B[t], d);
-                    // D layout (edge 4*sub+q, unit 16t+p) is the B layout of the sum
-                    acc[t] = mfma(mem, pack4(silu(d[0]), silu(d[1]), silu(d[2]), silu(d[3])),
-                                  acc[t]);
+            auto tiles = [&](unsigned ne) {
+                for (unsigned e0 = 0; e0 < ne; e0 += 16) {
+                    const bf16x4 a = edge_tile_a(L, e0);
+                    const bf16x4 mem = member_a(L, e0);
+                    for (int t = 0; t < 4; t++) {
+                        f32x4 d = {0, 0, 0, 0};
+                        d = mfma(a, B[t], d);
+                        // D layout (edge 4*sub+q, unit 16t+p) is the B layout of the sum
+                        acc[t] = mfma(mem, pack4(silu(d[0]), silu(d[1]), silu(d[2]), silu(d[3])),
+                                      acc[t]);
+                    }
                 }
-            }
+            };
+            walk_group<DENS>(n0, P, POS, VEL, K, start, L, tiles, dg, rh);
         }
         for (int t = 0; t < 4; t++)
             for (int q = 0; q < 4; q++)
@@ -358,46 +374,81 @@ struct NodeTile {
     float out[4][2];
 };
 
+// Node-layer weights staged in LDS once per workgroup: the B operands of
+// every tile's MFMAs are read from here as 8-byte rows.
+struct NodeW {
+    unsigned short W2[kH * kH];
+    unsigned short W3[kH * kKN];
+    float b2[kH], W4[2 * kH], b4[2];
+};
+
+__device__ void stage_node_weights(NodeW &sw, const unsigned short *W2b,
+                                   const unsigned short *W3b, const float *params)
+{
+    for (unsigned i = threadIdx.x; i < kH * kH / 4u; i += blockDim.x)
+        ((uint2 *)sw.W2)[i] = ((const uint2 *)W2b)[i];
+    for (unsigned i = threadIdx.x; i < kH * kKN / 4u; i += blockDim.x)
+        ((uint2 *)sw.W3)[i] = ((const uint2 *)W3b)[i];
+    for (unsigned i = threadIdx.x; i < kH; i += blockDim.x)
+        sw.b2[i] = params[ob2 + i];
+    for (unsigned i = threadIdx.x; i < 2u * kH; i += blockDim.x)
+        sw.W4[i] = params[oW4 + i];
+    if (threadIdx.x < 2u)
+        sw.b4[threadIdx.x] = params[ob4 + threadIdx.x];
+    __syncthreads();
+}
+
+__device__ __forceinline__ bf16x4 lds_row4(const unsigned short *base)
+{
+    return *(const bf16x4 *)base;
+}
+
 // Build u = [m, feats, 1, 0..] for a tile in LDS (bf16 [16][80]) and run the
 // node forward.  Rows of invalid slots are zero.
-__device__ void node_forward(unsigned n0, const sph_dev_params &P, unsigned long long clock,
-                             const float2 *VEL, const float *S, const float *deg,
-                             const unsigned short *W2b, const float *params,
-                             const unsigned short *W3b, unsigned short (*ul)[kKN],
-                             NodeTile &T)
+// A operands of a tile's S rows (row l%16, columns 16kc + 4(l/16) ..) from
+// the fp32 S array, and the degrees of rows 4(l/16) + q.
+__device__ __forceinline__ void load_s_tile(unsigned n0, unsigned vend, const float *S,
+                                            const float *deg, bf16x4 A[4], float degv[4])
+{
+    const unsigned l = lane();
+    const unsigned row = n0 + l % 16u;
+    for (int kc = 0; kc < 4; kc++) {
+        const unsigned k = 16u * kc + 4u * (l / 16u);
+        if (row < vend) {
+            const float4 sv = *(const float4 *)(S + (size_t)row * kH + k);
+            A[kc] = pack4(sv.x, sv.y, sv.z, sv.w);
+        } else {
+            A[kc] = pack4(0, 0, 0, 0);
+        }
+    }
+    for (int q = 0; q < 4; q++) {
+        const unsigned r = n0 + 4u * (l / 16u) + q;
+        degv[q] = r < vend ? deg[r] : 0.0f;
+    }
+}
+
+__device__ __forceinline__ void node_forward(unsigned n0, const sph_dev_params &P,
+                                             unsigned long long clock, const float2 *VEL,
+                                             const bf16x4 A[4], const float degv[4],
+                                             const NodeW &sw, unsigned short (*ul)[kKN],
+                                             NodeTile &T)
 {
     const unsigned l = lane();
     const unsigned nvalid_end = P.poff + (unsigned)P.n;
     const NodeCtx nc = node_ctx(P, clock);
     // m = S W2^T + deg b2
-    bf16x4 A[4];
-    {
-        const unsigned row = n0 + l % 16u;
-        const bool ok = row < nvalid_end;
-        for (int kc = 0; kc < 4; kc++) {
-            const unsigned k = 16u * kc + 4u * (l / 16u);
-            if (ok) {
-                const float *sp = S + (size_t)row * kH + k;
-                A[kc] = pack4(sp[0], sp[1], sp[2], sp[3]);
-            } else {
-                A[kc] = pack4(0, 0, 0, 0);
-            }
-        }
-    }
     for (int t = 0; t < 4; t++) {
         f32x4 d = {0, 0, 0, 0};
         for (int kc = 0; kc < 4; kc++) {
-            bf16x4 b; // B[k][c] = W2[16t + c][k]
+            // B[k][c] = W2[16t + c][k]
             const unsigned c = 16u * t + l % 16u, k = 16u * kc + 4u * (l / 16u);
-            for (int q = 0; q < 4; q++)
-                b[q] = (short)W2b[c * kH + k + q];
-            d = mfma(A[kc], b, d);
+            d = mfma(A[kc], lds_row4(&sw.W2[c * kH + k]), d);
         }
-        const float b2 = params[ob2 + 16u * t + l % 16u];
+        const float b2 = sw.b2[16u * t + l % 16u];
         for (int q = 0; q < 4; q++) {
             const unsigned r = 4u * (l / 16u) + q;
             const unsigned row = n0 + r;
-            const float m = row < nvalid_end ? d[q] + deg[row] * b2 : 0.0f;
+            const float m = row < nvalid_end ? d[q] + degv[q] * b2 : 0.0f;
             ul[r][16u * t + l % 16u] = f2bf(m);
         }
     }
@@ -426,11 +477,9 @@ __device__ void node_forward(unsigned n0, const sph_dev_params &P, unsigned long
     for (int t = 0; t < 4; t++) {
         f32x4 d = {0, 0, 0, 0};
         for (int kc = 0; kc < 5; kc++) {
-            bf16x4 b; // B[k][c] = W3[16t + c][k]
+            // B[k][c] = W3[16t + c][k]
             const unsigned c = 16u * t + l % 16u, k = 16u * kc + 4u * (l / 16u);
-            for (int q = 0; q < 4; q++)
-                b[q] = (short)W3b[c * kKN + k + q];
-            d = mfma(Au[kc], b, d);
+            d = mfma(Au[kc], lds_row4(&sw.W3[c * kKN + k]), d);
         }
         T.pre3[t] = d;
     }
@@ -439,10 +488,10 @@ __device__ void node_forward(unsigned n0, const sph_dev_params &P, unsigned long
         for (int o = 0; o < 2; o++) {
             float p = 0.0f;
             for (int t = 0; t < 4; t++)
-                p += silu(T.pre3[t][q]) * params[oW4 + o * kH + 16u * t + l % 16u];
+                p += silu(T.pre3[t][q]) * sw.W4[o * kH + 16u * t + l % 16u];
             for (int x = 1; x < 16; x <<= 1)
                 p += __shfl_xor(p, x, 64);
-            T.out[q][o] = p + params[ob4 + o];
+            T.out[q][o] = p + sw.b4[o];
         }
 }
 
@@ -455,32 +504,86 @@ __device__ __forceinline__ float2 ballistic(const NodeCtx &nc, float2 v)
 
 } // namespace
 
-// Node forward: predicted 4-step displacement per particle (slot order).
+// Inference: the edge pass and the node MLP of each 16-slot group in one
+// kernel (the group is one node tile).  S and deg stay on chip: the edge
+// pass's per-particle sums are transposed through LDS into the node MLP's A
+// operands, in the LDS the edge buffer used.
+union InferLds {
+    EdgeBuf e;
+    struct {
+        unsigned short ul[16][kKN];
+        unsigned short sS[16][kH];
+        float deg[16];
+    } n;
+};
+
 __global__ void __launch_bounds__(256)
-k_node_fwd(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS, unsigned slots,
-           const unsigned long long *__restrict__ clock, const float2 *__restrict__ VEL,
-           const float *__restrict__ S, const float *__restrict__ deg,
-           const unsigned short *__restrict__ W2b, const float *__restrict__ params,
-           const unsigned short *__restrict__ W3b, float2 *__restrict__ disp)
+k_infer(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS, unsigned slots,
+        const unsigned long long *__restrict__ clock, const float2 *__restrict__ POS,
+        const float2 *__restrict__ VEL, const unsigned *__restrict__ K,
+        const unsigned *__restrict__ start, const unsigned short *__restrict__ W1b,
+        const unsigned short *__restrict__ W2b, const float *__restrict__ params,
+        const unsigned short *__restrict__ W3b, float2 *__restrict__ disp)
 {
-    __shared__ unsigned short ul[4][16][kKN];
-    const unsigned w = threadIdx.x / 64u, l = lane();
-    const unsigned ntiles = slots / 16u, nw = gridDim.x * 4u;
+    __shared__ __align__(16) InferLds Lb[4];
+    __shared__ __align__(16) NodeW sw;
+    stage_node_weights(sw, W2b, W3b, params);
+    const unsigned w = threadIdx.x / 64u, l = lane(), p = l % 16u, sub = l / 16u;
+    InferLds &L = Lb[w];
+    bf16x4 B[4];
+    for (int t = 0; t < 4; t++)
+        B[t] = w1_b(W1b, t);
+    const unsigned ngroups = slots / kGroup, nw = gridDim.x * 4u;
     const unsigned long long ck = *clock;
-    for (unsigned tile = blockIdx.x * 4u + w; tile < ntiles; tile += nw) {
-        const unsigned n0 = tile * 16u;
+    for (unsigned g = blockIdx.x * 4u + w; g < ngroups; g += nw) {
+        const unsigned n0 = g * kGroup;
         const sph_dev_params &P = Pp[BS[n0 / kSphTPB]];
-        if (n0 >= P.poff + (unsigned)P.n)
+        const unsigned vend = P.poff + (unsigned)P.n;
+        if (n0 >= vend)
             continue;
+        f32x4 acc[4];
+        for (int t = 0; t < 4; t++)
+            acc[t] = {0, 0, 0, 0};
+        float dg = 0.0f, rh = 0.0f;
+        auto tiles = [&](unsigned ne) {
+            for (unsigned e0 = 0; e0 < ne; e0 += 16) {
+                const bf16x4 a = edge_tile_a(L.e, e0);
+                const bf16x4 mem = member_a(L.e, e0);
+                for (int t = 0; t < 4; t++) {
+                    f32x4 d = {0, 0, 0, 0};
+                    d = mfma(a, B[t], d);
+                    acc[t] = mfma(mem, pack4(silu(d[0]), silu(d[1]), silu(d[2]), silu(d[3])),
+                                  acc[t]);
+                }
+            }
+        };
+        walk_group<false>(n0, P, POS, VEL, K, start, L.e, tiles, dg, rh);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        // S tile (rows 4*sub+q, units 16t+p) -> LDS, transposed on read below
+        for (int t = 0; t < 4; t++)
+            for (int q = 0; q < 4; q++)
+                L.n.sS[4u * sub + q][16u * t + p] = f2bf(acc[t][q]);
+        if (l < 16u)
+            L.n.deg[l] = dg;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        bf16x4 A[4];
+        for (int kc = 0; kc < 4; kc++)
+            A[kc] = lds_row4(&L.n.sS[p][16u * kc + 4u * sub]);
+        float degv[4];
+        for (int q = 0; q < 4; q++)
+            degv[q] = L.n.deg[4u * sub + q];
         NodeTile T;
-        node_forward(n0, P, ck, VEL, S, deg, W2b, params, W3b, ul[w], T);
+        node_forward(n0, P, ck, VEL, A, degv, sw, L.n.ul, T);
         const NodeCtx nc = node_ctx(P, ck);
-        if (l % 16u == 0)
+        if (p == 0)
             for (int q = 0; q < 4; q++) {
-                const unsigned row = n0 + 4u * (l / 16u) + q;
-                if (row < P.poff + (unsigned)P.n) {
+                const unsigned row = n0 + 4u * sub + q;
+                if (row < vend) {
                     const float2 b = ballistic(nc, VEL[row]);
-                    disp[row] = make_float2(b.x + kDScale * T.out[q][0], b.y + kDScale * T.out[q][1]);
+                    disp[row] = make_float2(b.x + kDScale * T.out[q][0],
+                                            b.y + kDScale * T.out[q][1]);
                 }
             }
         __builtin_amdgcn_wave_barrier();
@@ -503,6 +606,8 @@ k_node_bwd(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS, un
     __shared__ unsigned short ul[4][16][kKN];
     __shared__ unsigned short dl[4][16][kH];
     __shared__ float du_l[4][16][kH];
+    __shared__ __align__(16) NodeW sw;
+    stage_node_weights(sw, W2b, W3b, params);
     const unsigned w = threadIdx.x / 64u, l = lane();
     const unsigned ntiles = slots / 16u, nw = gridDim.x * 4u;
     const unsigned long long ck = *clock;
@@ -525,7 +630,10 @@ k_node_bwd(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS, un
             continue;
         }
         NodeTile T;
-        node_forward(n0, P, ck, VEL, S, deg, W2b, params, W3b, ul[w], T);
+        bf16x4 A[4];
+        float degv[4];
+        load_s_tile(n0, vend, S, deg, A, degv);
+        node_forward(n0, P, ck, VEL, A, degv, sw, ul[w], T);
         const NodeCtx nc = node_ctx(P, ck);
         // dout for the 4 rows of this lane group
         float dout[4][2];
@@ -553,7 +661,7 @@ k_node_bwd(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS, un
                 const float z = silu(x);
                 gW4[0][t] += dout[q][0] * z;
                 gW4[1][t] += dout[q][1] * z;
-                const float dz = dout[q][0] * params[oW4 + c] + dout[q][1] * params[oW4 + kH + c];
+                const float dz = dout[q][0] * sw.W4[c] + dout[q][1] * sw.W4[kH + c];
                 const float dp = dz * silu_grad(x);
                 dl[w][4u * (l / 16u) + q][c] = f2bf(dp);
             }
@@ -589,7 +697,7 @@ k_node_bwd(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS, un
                 bf16x4 b; // B[k][c] = W3[k][16t + c]  (k: hidden unit)
                 const unsigned c = 16u * t + l % 16u, k = 16u * kc + 4u * (l / 16u);
                 for (int q = 0; q < 4; q++)
-                    b[q] = (short)W3b[(k + q) * kKN + c];
+                    b[q] = (short)sw.W3[(k + q) * kKN + c];
                 d = mfma(Ad[kc], b, d);
             }
             for (int q = 0; q < 4; q++) {
@@ -622,7 +730,7 @@ k_node_bwd(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS, un
                 bf16x4 b; // B[k][c] = W2[k][16t + c]  (k: output unit of layer 2)
                 const unsigned c = 16u * t + l % 16u, k = 16u * kc + 4u * (l / 16u);
                 for (int q = 0; q < 4; q++)
-                    b[q] = (short)W2b[(k + q) * kH + c];
+                    b[q] = (short)sw.W2[(k + q) * kH + c];
                 d = mfma(Am[kc], b, d);
             }
             for (int q = 0; q < 4; q++) { // rows past the scene's end get 0
@@ -714,7 +822,7 @@ k_edge_bwd(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS, un
            const float2 *__restrict__ POS, const float2 *__restrict__ VEL,
            const unsigned *__restrict__ K, const unsigned *__restrict__ start,
            const unsigned short *__restrict__ W1b, const float *__restrict__ dS,
-           float *__restrict__ grads, unsigned *__restrict__ dropped)
+           float *__restrict__ grads)
 {
     __shared__ EdgeBuf Lb[4];
     const unsigned w = threadIdx.x / 64u, l = lane(), p = l % 16u, sub = l / 16u;
@@ -731,38 +839,40 @@ k_edge_bwd(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS, un
         const sph_dev_params &P = Pp[BS[n0 / kSphTPB]];
         if (n0 >= P.poff + (unsigned)P.n)
             continue;
-        float dg, rh;
-        const unsigned ne = gather_group<false>(n0, P, POS, VEL, K, start, L, dropped, dg, rh);
         // B operand of the expansion: dS[particle 4*sub+i][unit 16t+p]
         bf16x4 Bds[4];
         for (int t = 0; t < 4; t++) {
             const float *d0 = dS + (size_t)(n0 + 4u * sub) * kH + 16u * t + p;
             Bds[t] = pack4(d0[0], d0[kH], d0[2 * kH], d0[3 * kH]);
         }
-        for (unsigned e0 = 0; e0 < ne; e0 += 16) {
-            const bf16x4 a = edge_tile_a(L, e0);
-            // one-hot A: row edge e0 + p, columns particles 4*sub .. +3
-            bf16x4 oh;
-            const unsigned own = L.owner[e0 + p];
-            for (int q = 0; q < 4; q++)
-                oh[q] = own == 4u * sub + q ? (short)0x3f80 : (short)0;
-            // B operand of the gradient product: F[edge 4*sub+i][feature p]
-            bf16x4 f;
-            for (int q = 0; q < 4; q++) {
-                const bf16x4 fr = L.f[e0 + 4u * sub + q][(p / 4u) & 1u];
-                f[q] = p < 8u ? fr[p % 4u] : (short)0;
-            }
-            for (int t = 0; t < 4; t++) {
-                f32x4 d = {0, 0, 0, 0};
-                d = mfma(a, B[t], d);
-                f32x4 ds = {0, 0, 0, 0};
-                ds = mfma(oh, Bds[t], ds);
-                float gp[4];
+        auto tiles = [&](unsigned ne) {
+            for (unsigned e0 = 0; e0 < ne; e0 += 16) {
+                const bf16x4 a = edge_tile_a(L, e0);
+                // one-hot A: row edge e0 + p, columns particles 4*sub .. +3
+                bf16x4 oh;
+                const unsigned own = L.owner[e0 + p];
                 for (int q = 0; q < 4; q++)
-                    gp[q] = ds[q] * silu_grad(d[q]);
-                g[t] = mfma(pack4(gp[0], gp[1], gp[2], gp[3]), f, g[t]);
+                    oh[q] = own == 4u * sub + q ? (short)0x3f80 : (short)0;
+                // B operand of the gradient product: F[edge 4*sub+i][feature p]
+                bf16x4 f;
+                for (int q = 0; q < 4; q++) {
+                    const bf16x4 fr = L.f[e0 + 4u * sub + q][(p / 4u) & 1u];
+                    f[q] = p < 8u ? fr[p % 4u] : (short)0;
+                }
+                for (int t = 0; t < 4; t++) {
+                    f32x4 d = {0, 0, 0, 0};
+                    d = mfma(a, B[t], d);
+                    f32x4 ds = {0, 0, 0, 0};
+                    ds = mfma(oh, Bds[t], ds);
+                    float gp[4];
+                    for (int q = 0; q < 4; q++)
+                        gp[q] = ds[q] * silu_grad(d[q]);
+                    g[t] = mfma(pack4(gp[0], gp[1], gp[2], gp[3]), f, g[t]);
+                }
             }
-        }
+        };
+        float dg, rh;
+        walk_group<false>(n0, P, POS, VEL, K, start, L, tiles, dg, rh);
         __builtin_amdgcn_wave_barrier();
     }
     if (p < 6u)
@@ -882,54 +992,73 @@ __global__ void k_clock_add(unsigned long long *clock, unsigned long long n)
     *clock += n; // one thread: a plain vector load/store pair
 }
 
-// Per-scene density sums and maxima of the last density pass.
-__global__ void k_scene_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
-                                unsigned slots, const float2 *__restrict__ RHO,
-                                float *__restrict__ sum, unsigned *__restrict__ maxbits)
+// Per-scene density sums and maxima of the last density pass: one
+// workgroup per scene over its slots, no atomics (deterministic).
+__global__ void __launch_bounds__(256)
+k_scene_density(const sph_dev_params *__restrict__ Pp, const float2 *__restrict__ RHO,
+                float *__restrict__ sum, float *__restrict__ maxv)
 {
-    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
-    const int scene = BS[(blockIdx.x * blockDim.x) / kSphTPB];
-    const sph_dev_params &P = Pp[scene];
-    float r = 0.0f;
-    if (s < slots && s < P.poff + (unsigned)P.n)
-        r = RHO[s].x;
-    float t = r, m = r;
+    __shared__ float ps[4], pm[4];
+    const sph_dev_params &P = Pp[blockIdx.x];
+    float t = 0.0f, m = 0.0f;
+    for (unsigned s = P.poff + threadIdx.x; s < P.poff + (unsigned)P.n; s += blockDim.x) {
+        const float r = RHO[s].x;
+        t += r;
+        m = fmaxf(m, r);
+    }
     for (int x = 32; x > 0; x >>= 1) {
         t += __shfl_xor(t, x, 64);
         m = fmaxf(m, __shfl_xor(m, x, 64));
     }
     if (lane() == 0) {
-        atomicAdd(&sum[scene], t);
-        atomicMax(&maxbits[scene], __float_as_uint(m)); // densities are >= 0
+        ps[threadIdx.x / 64u] = t;
+        pm[threadIdx.x / 64u] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        sum[blockIdx.x] = (ps[0] + ps[1]) + (ps[2] + ps[3]);
+        maxv[blockIdx.x] = fmaxf(fmaxf(pm[0], pm[1]), fmaxf(pm[2], pm[3]));
     }
 }
 
-// Judge of a macro-step: mean density of each scene against its reference
-// (the mean density of the scene's last accepted state).  Accepted scenes
-// take the new mean as their reference; with set_all every scene does, with
-// `only` the flagged ones do and nothing is judged.
+// Judge of a macro-step.  Per scene, ref[s] is the mean density of the
+// current state and dref[s] = ref[nscenes + s] the change of the scene's last
+// macro-step computed by the plain solver (its density trend).  A predicted
+// state passes when its mean density is within tol (relative) of
+// ref + dref, the plain solver's trend extrapolated by one macro-step; a
+// passing scene takes the new mean as ref.  On input flags[s] = 1 forces a
+// re-take (a periodic plain macro-step that refreshes the trend); on output
+// flags[s] = 1 for a failed scene, 2 for a forced one.  mode 1 (set) starts
+// every scene at the current mean with no trend; mode 2 (after re-takes)
+// gives each re-taken scene (flags[s] != 0) the plain change as its trend
+// and the plain mean as ref.
 __global__ void k_judge(const sph_dev_params *__restrict__ Pp, int nscenes,
                         const float *__restrict__ sum, float *__restrict__ ref, float tol,
-                        int set_all, const int *__restrict__ only, int *__restrict__ flags)
+                        int mode, int *__restrict__ flags)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nscenes)
         return;
+    float *dref = ref + nscenes;
     const int n = Pp[s].n;
     const float mean = n > 0 ? sum[s] / (float)n : 0.0f;
-    if (set_all) {
+    if (mode == 1) {
         ref[s] = mean;
+        dref[s] = 0.0f;
         return;
     }
-    if (only) { // after a fallback: only the re-taken scenes take a new reference
-        if (only[s])
+    if (mode == 2) {
+        if (flags[s]) {
+            dref[s] = mean - ref[s];
             ref[s] = mean;
+        }
         return;
     }
-    const float r = ref[s];
-    const bool bad = n > 0 && !(fabsf(mean - r) <= tol * r);
-    flags[s] = bad ? 1 : 0;
-    if (!bad)
+    const float expect = ref[s] + dref[s];
+    const bool bad = n > 0 && !(fabsf(mean - expect) <= tol * expect);
+    const bool forced = flags[s] != 0;
+    flags[s] = forced ? 2 : (bad ? 1 : 0);
+    if (!bad && !forced)
         ref[s] = mean;
 }
 
@@ -1021,7 +1150,7 @@ int ensure_judge(sph_surrogate *m, const sph_ctx *c)
     m->jflags = nullptr;
     m->ref_ok = false;
     int rc = SPH_OK;
-    if (!rc) rc = malloc_dev(&m->ref, ns);
+    if (!rc) rc = malloc_dev(&m->ref, 2 * ns); // ref, then the per-scene trend
     if (!rc) rc = malloc_dev(&m->jsum, ns);
     if (!rc) rc = malloc_dev(&m->jmax, ns);
     if (!rc) rc = malloc_dev(&m->jflags, ns);
@@ -1032,11 +1161,8 @@ int ensure_judge(sph_surrogate *m, const sph_ctx *c)
 // Per-scene density sums (and maxima) of RHO into m->jsum / m->jmax.
 int scene_sums(sph_surrogate *m, sph_ctx *c)
 {
-    const size_t ns = c->sc.size();
-    MCHK(hipMemsetAsync(m->jsum, 0, sizeof(float) * ns, c->st));
-    MCHK(hipMemsetAsync(m->jmax, 0, sizeof(unsigned) * ns, c->st));
-    k_scene_density<<<dim3(c->slots / 256), 256, 0, c->st>>>(c->dP, c->BS, c->slots, c->RHO,
-                                                              m->jsum, m->jmax);
+    k_scene_density<<<dim3((unsigned)c->sc.size()), 256, 0, c->st>>>(c->dP, c->RHO, m->jsum,
+                                                                       m->jmax);
     MCHK(hipGetLastError());
     return SPH_OK;
 }
@@ -1053,20 +1179,33 @@ int density_now(sph_ctx *c)
     return SPH_OK;
 }
 
+// Prediction of the 4-step displacement of every particle (fused kernel).
 int forward(sph_surrogate *m, sph_ctx *c)
 {
     if (!c->lists_valid)
         return merr(SPH_ESTATE, "surrogate: the context has no hashed state (step or hash first)");
     if (int rc = ensure_buffers(m, c->slots))
         return rc;
-    const unsigned S = c->slots;
-    k_edge_fwd<false><<<dim3(2048), 256, 0, c->st>>>(c->dP, c->BS, S, c->POS0, c->VEL, c->K0,
-                                                     c->start, m->W1b, m->S, m->deg, nullptr,
-                                                     m->dropped);
-    k_node_fwd<<<dim3(2048), 256, 0, c->st>>>(c->dP, c->BS, S, c->dclock, c->VEL, m->S, m->deg,
-                                              m->W2b, m->params, m->W3b, (float2 *)m->disp);
+    k_infer<<<dim3(2048), 256, 0, c->st>>>(c->dP, c->BS, c->slots, c->dclock, c->POS0, c->VEL,
+                                           c->K0, c->start, m->W1b, m->W2b, m->params, m->W3b,
+                                           (float2 *)m->disp);
     MCHK(hipGetLastError());
     m->have_pred = true;
+    return SPH_OK;
+}
+
+// The edge pass of training: per-particle sums S and degrees for the node
+// backward (which recomputes the node forward itself).
+int forward_edges(sph_surrogate *m, sph_ctx *c)
+{
+    if (!c->lists_valid)
+        return merr(SPH_ESTATE, "surrogate: the context has no hashed state (step or hash first)");
+    if (int rc = ensure_buffers(m, c->slots))
+        return rc;
+    k_edge_fwd<false><<<dim3(2048), 256, 0, c->st>>>(c->dP, c->BS, c->slots, c->POS0, c->VEL,
+                                                     c->K0, c->start, m->W1b, m->S, m->deg,
+                                                     nullptr);
+    MCHK(hipGetLastError());
     return SPH_OK;
 }
 
@@ -1095,7 +1234,6 @@ int sph_surrogate_create(sph_surrogate **out, uint64_t seed)
     if (!rc) rc = malloc_dev(&m->W2b, kH * kH);
     if (!rc) rc = malloc_dev(&m->W3b, kH * kKN);
     if (!rc) rc = malloc_dev(&m->loss, 1);
-    if (!rc) rc = malloc_dev(&m->dropped, 1);
     if (rc) {
         sph_surrogate_destroy(m);
         return rc;
@@ -1125,8 +1263,7 @@ int sph_surrogate_create(sph_surrogate **out, uint64_t seed)
     if (hipMemcpy(m->params, p.data(), sizeof(float) * kNParams, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(m->grads, 0, sizeof(float) * kNParams) != hipSuccess ||
         hipMemset(m->adam_m, 0, sizeof(float) * kNParams) != hipSuccess ||
-        hipMemset(m->adam_v, 0, sizeof(float) * kNParams) != hipSuccess ||
-        hipMemset(m->dropped, 0, sizeof(unsigned)) != hipSuccess) {
+        hipMemset(m->adam_v, 0, sizeof(float) * kNParams) != hipSuccess) {
         sph_surrogate_destroy(m);
         return merr(SPH_EHIP, "surrogate initialisation failed");
     }
@@ -1145,7 +1282,7 @@ void sph_surrogate_destroy(sph_surrogate *m)
         return;
     void *ptrs[] = {m->params, m->grads, m->adam_m, m->adam_v, m->W1b, m->W2b, m->W3b, m->S,
                     m->deg,    m->dS,    m->disp,   m->tgt,    m->U,   m->dP3, m->dM,  m->loss,
-                    m->dropped, m->tmp2, m->ref, m->jsum, m->jmax, m->jflags};
+                    m->tmp2, m->ref, m->jsum, m->jmax, m->jflags};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -1215,7 +1352,7 @@ static int backward_dev(sph_surrogate *m, sph_ctx *c, const float2 *target_dev, 
 {
     if (c->n_total == 0)
         return SPH_OK;
-    if (int rc = forward(m, c))
+    if (int rc = forward_edges(m, c))
         return rc;
     const unsigned S = c->slots;
     k_by_id_to_slot<<<dim3(S / 256), 256, 0, c->st>>>(c->dP, c->BS, S, c->ID0, target_dev,
@@ -1227,7 +1364,7 @@ static int backward_dev(sph_surrogate *m, sph_ctx *c, const float2 *target_dev, 
                                               m->dM, m->grads, m->loss);
     k_node_wgrad<<<dim3(1024), 256, 0, c->st>>>(S / 16u, m->U, m->dP3, m->dM, m->S, m->grads);
     k_edge_bwd<<<dim3(2048), 256, 0, c->st>>>(c->dP, c->BS, S, c->POS0, c->VEL, c->K0, c->start,
-                                              m->W1b, m->dS, m->grads, m->dropped);
+                                              m->W1b, m->dS, m->grads);
     MCHK(hipGetLastError());
     if (loss) {
         MCHK(hipMemcpyAsync(loss, m->loss, sizeof(float), hipMemcpyDeviceToHost, c->st));
@@ -1404,11 +1541,14 @@ int sph_surrogate_apply(sph_surrogate *m, sph_ctx *c)
 }
 
 // One surrogate macro-step (four plain steps of simulated time) of every
-// scene, judged by the plain solver: the predicted state is hashed and its
-// density computed; a scene whose mean density moved by more than tol
-// (relative) from its last accepted state is restored and re-taken with
-// four plain steps.  flags (optional, nscenes ints) receives 1 for re-taken
-// scenes; *nfallback their number.
+// scene, judged by the plain solver.  The predicted state is hashed and its
+// density computed; a scene whose mean density is off by more than tol
+// (relative) from the plain solver's trend (its mean density plus the change
+// of its last plain macro-step) is restored and re-taken with four plain
+// steps.  Every scene is also re-taken plainly on its first macro-step and
+// then every kTrendRefresh macro-steps, which keeps its trend current.
+// flags (optional, nscenes ints) receives 1 for failed scenes, 2 for forced
+// re-takes, 0 for accepted ones; *nfallback the number re-taken.
 int sph_surrogate_macro_step(sph_surrogate *m, sph_ctx *c, float tol, int *flags, int *nfallback)
 {
     if (!m || !c)
@@ -1425,17 +1565,21 @@ int sph_surrogate_macro_step(sph_surrogate *m, sph_ctx *c, float tol, int *flags
     if (!c->lists_valid)
         if (sph_phase_hash(c, 0))
             return merr(SPH_EHIP, "hash failed: %s", sph_last_error());
+    const dim3 jg((ns + 255) / 256);
     if (!m->ref_ok || m->ref_ctx != c) {
         if (int rc = density_now(c))
             return rc;
         if (int rc = scene_sums(m, c))
             return rc;
-        k_judge<<<dim3((ns + 255) / 256), 256, 0, c->st>>>(c->dP, ns, m->jsum, m->ref, tol, 1,
-                                                           nullptr, nullptr);
+        k_judge<<<jg, 256, 0, c->st>>>(c->dP, ns, m->jsum, m->ref, tol, 1, nullptr);
         MCHK(hipGetLastError());
         m->ref_ok = true;
         m->ref_ctx = c;
+        m->since_plain.assign((size_t)ns, kTrendRefresh); // first macro-step: plain
     }
+    std::vector<int> hf((size_t)ns);
+    for (int s = 0; s < ns; s++)
+        hf[s] = m->since_plain[s] >= kTrendRefresh ? 1 : 0;
     if (sph_snapshot_save(c))
         return merr(SPH_EHIP, "snapshot failed: %s", sph_last_error());
     if (int rc = forward(m, c))
@@ -1446,29 +1590,32 @@ int sph_surrogate_macro_step(sph_surrogate *m, sph_ctx *c, float tol, int *flags
         return merr(SPH_EHIP, "hash failed: %s", sph_last_error());
     if (int rc = scene_sums(m, c))
         return rc;
-    k_judge<<<dim3((ns + 255) / 256), 256, 0, c->st>>>(c->dP, ns, m->jsum, m->ref, tol, 0,
-                                                       nullptr, m->jflags);
+    MCHK(hipMemcpyAsync(m->jflags, hf.data(), sizeof(int) * ns, hipMemcpyHostToDevice, c->st));
+    k_judge<<<jg, 256, 0, c->st>>>(c->dP, ns, m->jsum, m->ref, tol, 0, m->jflags);
     MCHK(hipGetLastError());
-    std::vector<int> hf((size_t)ns);
     MCHK(hipMemcpyAsync(hf.data(), m->jflags, sizeof(int) * ns, hipMemcpyDeviceToHost, c->st));
     MCHK(hipStreamSynchronize(c->st));
     int nf = 0;
-    for (int s = 0; s < ns; s++)
+    for (int s = 0; s < ns; s++) {
         nf += hf[s] != 0;
+        m->since_plain[s] = hf[s] ? 0 : m->since_plain[s] + 1;
+    }
     if (nf) {
-        std::vector<int> frozen((size_t)ns), none((size_t)ns, 0);
-        for (int s = 0; s < ns; s++)
+        std::vector<int> retake((size_t)ns), frozen((size_t)ns), none((size_t)ns, 0);
+        for (int s = 0; s < ns; s++) {
+            retake[s] = hf[s] ? 1 : 0;
             frozen[s] = hf[s] ? 0 : 1;
-        if (sph_snapshot_restore(c, hf.data()) || sph_set_scene_frozen(c, frozen.data()) ||
+        }
+        if (sph_snapshot_restore(c, retake.data()) || sph_set_scene_frozen(c, frozen.data()) ||
             sph_step(c, 4) || sph_set_scene_frozen(c, none.data()))
             return merr(SPH_EHIP, "fallback failed: %s", sph_last_error());
         if (int rc = density_now(c))
             return rc;
         if (int rc = scene_sums(m, c))
             return rc;
-        MCHK(hipMemcpyAsync(m->jflags, hf.data(), sizeof(int) * ns, hipMemcpyHostToDevice, c->st));
-        k_judge<<<dim3((ns + 255) / 256), 256, 0, c->st>>>(c->dP, ns, m->jsum, m->ref, tol, 0,
-                                                           m->jflags, nullptr);
+        MCHK(hipMemcpyAsync(m->jflags, retake.data(), sizeof(int) * ns, hipMemcpyHostToDevice,
+                            c->st));
+        k_judge<<<jg, 256, 0, c->st>>>(c->dP, ns, m->jsum, m->ref, tol, 2, m->jflags);
         MCHK(hipGetLastError());
         MCHK(hipStreamSynchronize(c->st));
     }
@@ -1477,14 +1624,6 @@ int sph_surrogate_macro_step(sph_surrogate *m, sph_ctx *c, float tol, int *flags
     if (nfallback)
         *nfallback = nf;
     return SPH_OK;
-}
-
-unsigned sph_surrogate_dropped_edges(sph_surrogate *m)
-{
-    unsigned d = 0;
-    if (m && hipMemcpy(&d, m->dropped, sizeof d, hipMemcpyDeviceToHost) == hipSuccess)
-        return d;
-    return 0;
 }
 
 int sph_surrogate_mfma_selftest(float *out256)
@@ -1512,39 +1651,22 @@ int sph_scene_density(sph_ctx *c, float *mean, float *maxv)
     if (int rc = sph_internal_canonicalize(c))
         return rc;
     const size_t ns = c->sc.size();
-    float *sum = nullptr;
-    unsigned *mx = nullptr;
-    if (int rc = malloc_dev(&sum, ns))
+    float *buf = nullptr;
+    if (int rc = malloc_dev(&buf, 2 * ns))
         return rc;
-    if (int rc = malloc_dev(&mx, ns)) {
-        (void)hipFree(sum);
-        return rc;
-    }
-    hipError_t e = hipMemsetAsync(sum, 0, sizeof(float) * ns, c->st);
-    if (e == hipSuccess)
-        e = hipMemsetAsync(mx, 0, sizeof(unsigned) * ns, c->st);
-    if (e == hipSuccess) {
-        k_scene_density<<<dim3(c->slots / 256), 256, 0, c->st>>>(c->dP, c->BS, c->slots, c->RHO,
-                                                                  sum, mx);
-        e = hipGetLastError();
-    }
-    std::vector<float> hs(ns);
-    std::vector<unsigned> hm(ns);
+    k_scene_density<<<dim3((unsigned)ns), 256, 0, c->st>>>(c->dP, c->RHO, buf, buf + ns);
+    hipError_t e = hipGetLastError();
+    std::vector<float> h(2 * ns);
     if (e == hipSuccess)
         e = hipStreamSynchronize(c->st);
     if (e == hipSuccess)
-        e = hipMemcpy(hs.data(), sum, sizeof(float) * ns, hipMemcpyDeviceToHost);
-    if (e == hipSuccess)
-        e = hipMemcpy(hm.data(), mx, sizeof(unsigned) * ns, hipMemcpyDeviceToHost);
-    (void)hipFree(sum);
-    (void)hipFree(mx);
+        e = hipMemcpy(h.data(), buf, sizeof(float) * 2 * ns, hipMemcpyDeviceToHost);
+    (void)hipFree(buf);
     if (e != hipSuccess)
         return merr(SPH_EHIP, "sph_scene_density: %s", hipGetErrorString(e));
     for (size_t s = 0; s < ns; s++) {
-        mean[s] = c->sc[s].n ? hs[s] / (float)c->sc[s].n : 0.0f;
-        float f;
-        memcpy(&f, &hm[s], sizeof f);
-        maxv[s] = f;
+        mean[s] = c->sc[s].n ? h[s] / (float)c->sc[s].n : 0.0f;
+        maxv[s] = h[ns + s];
     }
     return SPH_OK;
 }

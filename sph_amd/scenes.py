@@ -12,7 +12,9 @@
   training and inference scenes: n particles, the first n of an nx-wide
   lattice (nx seeded, so blocks range from tall to wide) in the lower-left
   of the box, preset 0, and a seeded horizontal box acceleration of
-  amplitude U[0, 2] and period U[1, 4] s.
+  amplitude U[0, 2] and period U[1, 4] s.  With ``widen`` the blocks (and
+  the box) are that many times wider at the same depth, which keeps large
+  scenes in the training scenes' regime.
 """
 from __future__ import annotations
 
@@ -76,15 +78,19 @@ def concat(scenes):
 
 
 def sloshing_dam_breaks(nscenes, n=16384, box=(128.0, 96.0), seed=7, spacing=0.5, jitter=0.01,
-                        amp=(0.0, 2.0), period=(1.0, 4.0)):
-    """List of (tunables, box, particles, amplitude, period)."""
+                        amp=(0.0, 2.0), period=(1.0, 4.0), widen=1):
+    """List of (tunables, box, particles, amplitude, period).  widen > 1
+    makes scenes of n particles whose fluid blocks are `widen` times wider
+    than those of n / widen particles (same depth, same local physics) in a
+    box `widen` times wider."""
     rng = np.random.default_rng(seed)
     out = []
-    wmax = box[0] * 0.75 / spacing
+    n0 = n // widen
+    wmax = box[0] / widen * 0.75 / spacing
     for s in range(nscenes):
-        nx_lo = int(math.ceil(math.sqrt(n) * 0.75))
-        nx_hi = int(min(wmax, math.sqrt(n) * 1.5))
-        nx = int(rng.integers(nx_lo, max(nx_lo + 1, nx_hi)))
+        nx_lo = int(math.ceil(math.sqrt(n0) * 0.75))
+        nx_hi = int(min(wmax, math.sqrt(n0) * 1.5))
+        nx = int(rng.integers(nx_lo, max(nx_lo + 1, nx_hi))) * widen
         ny = int(math.ceil(n / nx))
         assert ny * spacing < box[1] * 0.9, "fluid block taller than the box"
         parts = scene_lattice(nx, ny, spacing, jitter, seed=seed * 1000003 + s,

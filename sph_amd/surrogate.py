@@ -188,7 +188,7 @@ def main(argv=None):
     meta = dict(scenes=args.scenes, n=args.n, steps=args.steps, iters=args.iters, batch=args.batch,
                 lr=args.lr, seed=args.seed, final_loss=float(np.mean(hist[-100:])),
                 holdout_rms_network=net, holdout_rms_ballistic=bal,
-                seconds=time.time() - t0, dropped_edges=model.dropped_edges())
+                seconds=time.time() - t0)
     save_weights(model, meta, args.out, os.path.splitext(args.out)[0] + ".json")
     log(json.dumps(meta))
 
