@@ -41,6 +41,10 @@ KERNEL_BYTES = {
     "density": (20, 4),          # read x, y, key (12); write rho, rho_near (8); cell starts
     "relax": (52, 12),           # read x, y, xp, yp, rho, rhon, key (28); write x, y, vx, vy, key, slot (24)
 }
+# profiler names of the kernels (template instances of the fused step)
+KERNEL_SYMBOLS = {"visc_predict": "k_visc_predict<true,", "relax": "k_relax<true>",
+                  "density": "k_density", "reorder": "k_reorder<2>", "scatter": "k_scatter",
+                  "scan_apply": "k_scan_apply", "scan_reduce": "k_scan_reduce"}
 # launches of each kind per step
 KERNEL_LAUNCHES = {"visc_predict": 1, "scan_reduce": 2, "scan_apply": 2, "scatter": 2,
                    "reorder": 2, "density": 1, "relax": 1}
@@ -79,6 +83,43 @@ def cpu_baseline(tun, box, parts, budget_s=15.0):
     return {"value": len(parts) * k / dt, "unit": "particle-steps/s", "cores": 1, "kind": kind,
             "sample": f"{k} steady steps (after 2 warm-up) of the same N={len(parts)} dam break, "
                       f"{dt:.1f} s on one host core ({os.cpu_count()} visible)"}
+
+
+def _cpu_scene_worker(args):
+    """One batched scene stepped by the reference's own C step on one core."""
+    index, budget_s = args
+    from oracle import pyoracle as po
+    t, b, parts, _ = sc.batched_scenes(only={index})[0]
+    sim = po.CompiledRef(b, t, len(parts), fast=True)
+    sim.load(np.ascontiguousarray(parts).view(po.P32))
+    sim.step(1)
+    k, t0 = 0, time.perf_counter()
+    while True:
+        sim.step(1)
+        k += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            return len(parts), k, el
+
+
+def cpu_baseline_batched(cores, budget_s=10.0):
+    """The batched configuration on the host: one scene per core, the
+    reference's own C step (oracle/_ref, -O3 -ffast-math), `cores` scenes
+    spread over the 256 (every 256/cores-th) stepped concurrently for
+    budget_s.  Runs in forked workers before this process touches the GPU."""
+    import multiprocessing as mp
+    from oracle import pyoracle as po
+    if not po.have_compiled_ref(fast=True):
+        return None
+    idx = [i * 256 // cores for i in range(cores)]
+    with mp.get_context("fork").Pool(cores) as pool:
+        res = pool.map(_cpu_scene_worker, [(i, budget_s) for i in idx])
+    value = sum(n * k / el for n, k, el in res)
+    parts = sum(n for n, _, _ in res)
+    return {"value": value, "unit": "particle-steps/s", "cores": cores, "kind": "reference",
+            "sample": f"{cores} of the 256 scenes ({parts} particles, every {256 // cores}th), one "
+                      f"per core, {budget_s:.0f} s each of steady steps after one warm-up step; "
+                      f"throughput summed over the concurrent workers"}
 
 
 def batched(steps, warmup):
@@ -237,6 +278,67 @@ def profile_summary(db, last=0):
     return "\n".join(lines)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_c2_dam_break.txt")
+
+
+def _counter_means(db, counter):
+    """Per kernel: list of per-dispatch values of one counter."""
+    import re
+    import sqlite3
+    con = sqlite3.connect(db)
+    out = {}
+    for d, k, v in con.execute("select dispatch_id, kernel_name, value from counters_collection "
+                               "where counter_name = ?", (counter,)):
+        out.setdefault(re.sub(r"\(.*", "", k).replace("void ", ""), {})[d] = v
+    return out
+
+
+def traffic_summary(fetch_db, write_db, copy_mb):
+    """HBM bytes per launch of every kernel from a FETCH_SIZE and a WRITE_SIZE
+    counter run (KiB per dispatch), each scaled by the factor that makes the
+    same counter read the known size of the bench's calibration copy
+    (--calib-copy-mb: a device-to-device copy of copy_mb MB, which reads and
+    writes copy_mb MB)."""
+    fetch = _counter_means(fetch_db, "FETCH_SIZE")
+    write = _counter_means(write_db, "WRITE_SIZE")
+    known_kib = copy_mb * 1e6 / 1024.0
+    cal = {}
+    for name, tab in (("fetch", fetch), ("write", write)):
+        vals = [v for k, m in tab.items() if "copyBuffer" in k for v in m.values()
+                if v > 0.5 * known_kib]
+        if not vals:
+            raise SystemExit(f"no calibration copy of {copy_mb} MB found in the {name} run")
+        cal[name] = known_kib / (sum(vals) / len(vals))
+    lines = [f"calibration: {copy_mb:g} MB device copy; scale FETCH_SIZE x{cal['fetch']:.4f}, "
+             f"WRITE_SIZE x{cal['write']:.4f}",
+             f"{'kernel':42s} {'dispatches':>10s} {'FETCH_KiB':>12s} {'WRITE_KiB':>12s} "
+             f"{'HBM_MB_per_launch':>18s}"]
+    for k in sorted(set(fetch) | set(write)):
+        f = list(fetch.get(k, {}).values())
+        w = list(write.get(k, {}).values())
+        fm = sum(f) / len(f) if f else 0.0
+        wm = sum(w) / len(w) if w else 0.0
+        mb = (fm * cal["fetch"] + wm * cal["write"]) * 1024.0 / 1e6
+        lines.append(f"{k:42s} {max(len(f), len(w)):10d} {fm:12.1f} {wm:12.1f} {mb:18.3f}")
+    return "\n".join(lines)
+
+
+def traffic_of(kernel_name):
+    """Calibrated HBM bytes per launch of a kernel from the committed traffic
+    summary (None when absent)."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            parts = line.split()
+            if parts and parts[0] == kernel_name and len(parts) >= 5:
+                try:
+                    return float(parts[-1]) * 1e6
+                except ValueError:
+                    return None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -255,14 +357,25 @@ def main():
                     help="print a summary of a rocprofv3 results database and exit")
     ap.add_argument("--last", type=int, default=0,
                     help="with --profile-summary: keep the last N trace dispatches")
+    ap.add_argument("--traffic-summary", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
+                    help="print calibrated HBM traffic per kernel from two rocprofv3 --pmc "
+                         "databases and exit (uses --calib-copy-mb)")
     args = ap.parse_args()
     if args.profile_summary:
         print(profile_summary(args.profile_summary, args.last))
+        return
+    if args.traffic_summary:
+        print(traffic_summary(*args.traffic_summary, args.calib_copy_mb or 256.0))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs one scene on one GPU; --gpus must be 1"}))
         sys.exit(2)
 
+    # host-core baseline of the batched configuration, before any GPU use
+    # (forked workers)
+    cpu_batched = None
+    if not args.no_cpu and not args.no_batched:
+        cpu_batched = cpu_baseline_batched(min(16, os.cpu_count() or 1))
     if args.only_surrogate:
         print(json.dumps({"batched_surrogate": surrogate_leg(args.surrogate_steps)}))
         return
@@ -312,7 +425,10 @@ def main():
                    "particles": n, "cells": sx * sy},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS if ach else None,
-                     "traffic": None},
+                     "traffic": traffic_of(KERNEL_SYMBOLS.get(dom, dom)),
+                     "algorithmic_bytes": kernel_bw[dom]["bytes"],
+                     "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT)
+                     if traffic_of(KERNEL_SYMBOLS.get(dom, dom)) is not None else None},
         "step_roofline": {"bytes_per_particle_step": b_step,
                           "achieved_GBs": b_step * value / 1e9,
                           "frac": b_step * value / 1e9 / HBM_PEAK_GBS},
@@ -321,6 +437,8 @@ def main():
     }
     if not args.no_batched:
         out["batched"] = batched(max(10, args.steps // 2), max(2, args.warmup // 2))
+        if cpu_batched:
+            out["batched"]["cpu_baseline"] = cpu_batched
     if not args.no_surrogate:
         out["batched_surrogate"] = surrogate_leg(args.surrogate_steps)
     if args.calib_copy_mb > 0:

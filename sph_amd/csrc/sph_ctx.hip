@@ -115,13 +115,14 @@ void free_graphs(sph_ctx *c)
 }
 
 // Timing hook before each launch (a no-op unless sph_time_kernels is
-// enqueueing): records the next boundary event, so kernel i runs between
-// boundary events i and i+1.
+// enqueueing): hands the next start/stop event pair to the launch, which
+// records the kernel's own start and end.
 struct Tick {
     Tick(sph_ctx *c, int kind)
     {
         if (c->tev) {
-            (void)hipEventRecord((*c->tev)[c->tidx++], c->st);
+            sph_set_next_launch_events((*c->tev)[c->tidx], (*c->tev)[c->tidx + 1]);
+            c->tidx += 2;
             c->tkind->push_back(kind);
         }
     }
@@ -1073,12 +1074,11 @@ int sph_time_kernels(sph_ctx *c, int nsteps, int max_kinds, const char **names, 
         if (rc)
             return rc;
     }
-    // Boundary events are created up front and recorded before every kernel
-    // and after the last one of each step; all nsteps steps are enqueued
-    // without synchronising, so the host runs ahead of the device and the
-    // events bracket the kernels without launch gaps.
+    // One start/stop event pair per launch, created up front; the launches
+    // record them around the kernels themselves (hipExtLaunchKernelGGL), and
+    // all nsteps steps are enqueued before the one synchronisation.
     constexpr int kLaunches = 11;
-    const size_t nev = (size_t)nsteps * (kLaunches + 1);
+    const size_t nev = (size_t)nsteps * kLaunches * 2;
     std::vector<hipEvent_t> ev(nev, nullptr);
     std::vector<int> kind;
     hipError_t e = hipSuccess;
@@ -1091,8 +1091,6 @@ int sph_time_kernels(sph_ctx *c, int nsteps, int max_kinds, const char **names, 
         c->tidx = 0;
         for (int it = 0; it < nsteps; it++) {
             enqueue_fused_step(c, true);
-            (void)hipEventRecord(ev[c->tidx++], c->st);
-            kind.push_back(-1); // step boundary
             c->steps++;
             c->hashes += 2;
         }
@@ -1105,11 +1103,9 @@ int sph_time_kernels(sph_ctx *c, int nsteps, int max_kinds, const char **names, 
     }
     double tot[NKIND] = {0};
     int cntk[NKIND] = {0};
-    for (size_t i = 0; e == hipSuccess && i + 1 < c->tidx; i++) {
-        if (kind[i] < 0)
-            continue;
+    for (size_t i = 0; e == hipSuccess && i < kind.size(); i++) {
         float t = 0.0f;
-        e = hipEventElapsedTime(&t, ev[i], ev[i + 1]);
+        e = hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]);
         tot[kind[i]] += t;
         cntk[kind[i]]++;
     }

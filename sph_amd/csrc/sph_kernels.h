@@ -87,3 +87,7 @@ void sph_launch_gather_state(hipStream_t st, unsigned slots, const sph_dev_param
 void sph_launch_load_state(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
                            const float4 *in, float2 *POS, float2 *VEL, float2 *PREV, float2 *RHO,
                            int *ID);
+
+// Time the next kernel launch of this thread with a start/stop event pair
+// (recorded by the launch itself: the kernel's own duration).
+void sph_set_next_launch_events(hipEvent_t start, hipEvent_t stop);

@@ -36,6 +36,8 @@
 // stay in that XCD's L2.
 #include "sph_kernels.h"
 
+#include <hip/hip_ext.h>
+
 namespace {
 
 constexpr int kWave = 64;
@@ -772,6 +774,34 @@ k_permute_by_id(const sph_dev_params *__restrict__ Pp, const int *__restrict__ B
 
 #define GRID(slots) dim3((unsigned)((slots) / kTPB))
 
+namespace {
+
+// Start/stop events for the next launch (sph_set_next_launch_events).
+thread_local hipEvent_t g_next_start = nullptr, g_next_stop = nullptr;
+
+// Launch on `st`; when timing events are pending, through
+// hipExtLaunchKernelGGL, whose events carry the kernel's own start and end
+// times (no dispatch latency).
+template <class F, class... Args>
+void launch(F kernel, dim3 grid, dim3 block, hipStream_t st, Args... args)
+{
+    if (g_next_start) {
+        hipEvent_t a = g_next_start, b = g_next_stop;
+        g_next_start = g_next_stop = nullptr;
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, st, a, b, 0, args...);
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
+    }
+}
+
+} // namespace
+
+void sph_set_next_launch_events(hipEvent_t start, hipEvent_t stop)
+{
+    g_next_start = start;
+    g_next_stop = stop;
+}
+
 __global__ void __launch_bounds__(kTPB)
 k_restore(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
           const int *__restrict__ flags, sph_restore_args a)
@@ -833,7 +863,7 @@ void sph_launch_visc_predict(hipStream_t st, unsigned slots, bool grav, bool vis
                              float2 *held_vel)
 {
 #define VP(G, V, R)                                                                              \
-    k_visc_predict<G, V, R><<<GRID(slots), kTPB, 0, st>>>(P, BS, POS, VEL, K, start, opos, okey, \
+    launch(k_visc_predict<G, V, R>, GRID(slots), kTPB, st, P, BS, POS, VEL, K, start, opos, okey, \
                                                           oslot, cnt, C, clock, held_vel)
     if (grav && visc && pred)
         VP(true, true, true);
@@ -853,46 +883,46 @@ void sph_launch_visc_predict(hipStream_t st, unsigned slots, bool grav, bool vis
 void sph_launch_gravity(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
                         float2 *VEL, float2 *RHO)
 {
-    k_gravity<<<GRID(slots), kTPB, 0, st>>>(P, BS, VEL, RHO);
+    launch(k_gravity, GRID(slots), kTPB, st, P, BS, VEL, RHO);
 }
 
 void sph_launch_predict(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
                         float2 *POS, const float2 *VEL, float2 *PREV)
 {
-    k_predict<<<GRID(slots), kTPB, 0, st>>>(P, BS, POS, VEL, PREV);
+    launch(k_predict, GRID(slots), kTPB, st, P, BS, POS, VEL, PREV);
 }
 
 void sph_launch_update_velocities(hipStream_t st, unsigned slots, const sph_dev_params *P,
                                   const int *BS, float2 *POS, float2 *VEL, const float2 *PREV)
 {
-    k_update_velocities<<<GRID(slots), kTPB, 0, st>>>(P, BS, POS, VEL, PREV);
+    launch(k_update_velocities, GRID(slots), kTPB, st, P, BS, POS, VEL, PREV);
 }
 
 void sph_launch_key_hist(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
                          const float2 *POS, unsigned *okey, unsigned *oslot, unsigned *cnt,
                          sph_dev_counters *C)
 {
-    k_key_hist<<<GRID(slots), kTPB, 0, st>>>(P, BS, POS, okey, oslot, cnt, C);
+    launch(k_key_hist, GRID(slots), kTPB, st, P, BS, POS, okey, oslot, cnt, C);
 }
 
 unsigned sph_scan_blocks(unsigned m) { return (m + kScanTile - 1) / kScanTile; }
 
 void sph_launch_scan_reduce(hipStream_t st, const unsigned *cnt, unsigned m, unsigned *bsum)
 {
-    k_scan_reduce<<<dim3(sph_scan_blocks(m)), kScanThreads, 0, st>>>(cnt, m, bsum);
+    launch(k_scan_reduce, dim3(sph_scan_blocks(m)), kScanThreads, st, cnt, m, bsum);
 }
 
 void sph_launch_scan_apply(hipStream_t st, unsigned *cnt, unsigned m, const unsigned *bsum,
                            unsigned *start)
 {
-    k_scan_apply<<<dim3(sph_scan_blocks(m)), kScanThreads, 0, st>>>(cnt, m, bsum, start);
+    launch(k_scan_apply, dim3(sph_scan_blocks(m)), kScanThreads, st, cnt, m, bsum, start);
 }
 
 void sph_launch_scatter(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
                         const unsigned *key, const unsigned *slot, const unsigned *start,
                         const int *id_src, int *perm_src, int *perm_id)
 {
-    k_scatter<<<GRID(slots), kTPB, 0, st>>>(P, BS, key, slot, start, id_src, perm_src, perm_id);
+    launch(k_scatter, GRID(slots), kTPB, st, P, BS, key, slot, start, id_src, perm_src, perm_id);
 }
 
 void sph_launch_reorder2(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
@@ -900,7 +930,7 @@ void sph_launch_reorder2(hipStream_t st, unsigned slots, const sph_dev_params *P
                          const unsigned *key, sph_reorder_fields<2> f, int *id_dst,
                          unsigned *key_dst, sph_dev_counters *C)
 {
-    k_reorder<2><<<GRID(slots), kTPB, 0, st>>>(P, BS, start, perm_src, perm_id, key, f, id_dst,
+    launch(k_reorder<2>, GRID(slots), kTPB, st, P, BS, start, perm_src, perm_id, key, f, id_dst,
                                                key_dst, C);
 }
 
@@ -909,7 +939,7 @@ void sph_launch_reorder4(hipStream_t st, unsigned slots, const sph_dev_params *P
                          const unsigned *key, sph_reorder_fields<4> f, int *id_dst,
                          unsigned *key_dst, sph_dev_counters *C)
 {
-    k_reorder<4><<<GRID(slots), kTPB, 0, st>>>(P, BS, start, perm_src, perm_id, key, f, id_dst,
+    launch(k_reorder<4>, GRID(slots), kTPB, st, P, BS, start, perm_src, perm_id, key, f, id_dst,
                                                key_dst, C);
 }
 
@@ -917,7 +947,7 @@ void sph_launch_density(hipStream_t st, unsigned slots, const sph_dev_params *P,
                         const float2 *POS, const unsigned *K, const unsigned *start, float2 *RHO,
                         sph_dev_counters *C)
 {
-    k_density<<<GRID(slots), kTPB, 0, st>>>(P, BS, POS, K, start, RHO, C);
+    launch(k_density, GRID(slots), kTPB, st, P, BS, POS, K, start, RHO, C);
 }
 
 void sph_launch_relax(hipStream_t st, unsigned slots, bool fused, const sph_dev_params *P,
@@ -927,17 +957,17 @@ void sph_launch_relax(hipStream_t st, unsigned slots, bool fused, const sph_dev_
                       unsigned long long *clock)
 {
     if (fused)
-        k_relax<true><<<GRID(slots), kTPB, 0, st>>>(P, BS, POS, PREV, RHO, K, start, opos, ovel,
+        launch(k_relax<true>, GRID(slots), kTPB, st, P, BS, POS, PREV, RHO, K, start, opos, ovel,
                                                     okey, oslot, cnt, C, clock);
     else
-        k_relax<false><<<GRID(slots), kTPB, 0, st>>>(P, BS, POS, PREV, RHO, K, start, opos, ovel,
+        launch(k_relax<false>, GRID(slots), kTPB, st, P, BS, POS, PREV, RHO, K, start, opos, ovel,
                                                      okey, oslot, cnt, C, clock);
 }
 
 void sph_launch_pack_coords(hipStream_t st, unsigned slots, const sph_dev_params *P,
                             const int *BS, const float2 *POS, const int *ID, short *out)
 {
-    k_pack_coords<<<GRID(slots), kTPB, 0, st>>>(P, BS, POS, ID, out);
+    launch(k_pack_coords, GRID(slots), kTPB, st, P, BS, POS, ID, out);
 }
 
 void sph_launch_neighbor_lists(hipStream_t st, unsigned slots, const sph_dev_params *P,
@@ -945,39 +975,39 @@ void sph_launch_neighbor_lists(hipStream_t st, unsigned slots, const sph_dev_par
                                const unsigned *start, int stride, int *full_by_id,
                                int *half_by_id, int *ids_by_id)
 {
-    k_neighbor_lists<<<GRID(slots), kTPB, 0, st>>>(P, BS, POS, K, ID, start, stride, full_by_id,
+    launch(k_neighbor_lists, GRID(slots), kTPB, st, P, BS, POS, K, ID, start, stride, full_by_id,
                                                    half_by_id, ids_by_id);
 }
 
 void sph_launch_pos_of_id(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
                           const int *ID, int *pos_of_id)
 {
-    k_pos_of_id<<<GRID(slots), kTPB, 0, st>>>(P, BS, ID, pos_of_id);
+    launch(k_pos_of_id, GRID(slots), kTPB, st, P, BS, ID, pos_of_id);
 }
 
 void sph_launch_permute_by_id(hipStream_t st, unsigned slots, const sph_dev_params *P,
                               const int *BS, const int *id_src, const int *pos_of_id,
                               sph_reorder_fields<2> f)
 {
-    k_permute_by_id<<<GRID(slots), kTPB, 0, st>>>(P, BS, id_src, pos_of_id, f);
+    launch(k_permute_by_id, GRID(slots), kTPB, st, P, BS, id_src, pos_of_id, f);
 }
 
 void sph_launch_restore(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
                         const int *flags, sph_restore_args a)
 {
-    k_restore<<<dim3(slots / kTPB), kTPB, 0, st>>>(P, BS, flags, a);
+    launch(k_restore, dim3(slots / kTPB), kTPB, st, P, BS, flags, a);
 }
 
 void sph_launch_gather_state(hipStream_t st, unsigned slots, const sph_dev_params *P,
                              const int *BS, const float2 *POS, const float2 *VEL, const int *ID,
                              float4 *out)
 {
-    k_gather_state<<<GRID(slots), kTPB, 0, st>>>(P, BS, POS, VEL, ID, out);
+    launch(k_gather_state, GRID(slots), kTPB, st, P, BS, POS, VEL, ID, out);
 }
 
 void sph_launch_load_state(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
                            const float4 *in, float2 *POS, float2 *VEL, float2 *PREV, float2 *RHO,
                            int *ID)
 {
-    k_load_state<<<GRID(slots), kTPB, 0, st>>>(P, BS, in, POS, VEL, PREV, RHO, ID);
+    launch(k_load_state, GRID(slots), kTPB, st, P, BS, in, POS, VEL, PREV, RHO, ID);
 }

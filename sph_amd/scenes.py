@@ -49,8 +49,10 @@ def dam_break(nx=1024, ny=1024, box=(1200.0, 800.0), seed=12345, spacing=0.5, ji
     return preset_tunables(0), make_box(*box), scene_lattice(nx, ny, spacing, jitter, seed=seed)
 
 
-def batched_scenes(nscenes=256, seed=2026, nmin=2048, nmax=65536, spacing=0.5, jitter=0.01):
-    """List of (tunables, box, particles, preset) for the batched config."""
+def batched_scenes(nscenes=256, seed=2026, nmin=2048, nmax=65536, spacing=0.5, jitter=0.01,
+                   only=None):
+    """List of (tunables, box, particles, preset) for the batched config
+    (with `only`, just those scene indices, drawn from the same sequence)."""
     rng = np.random.default_rng(seed)
     out = []
     for s in range(nscenes):
@@ -63,6 +65,8 @@ def batched_scenes(nscenes=256, seed=2026, nmin=2048, nmax=65536, spacing=0.5, j
         h = w / aspect
         nx = int(math.ceil(math.sqrt(n * aspect)))
         ny = int(math.ceil(n / nx))
+        if only is not None and s not in only:
+            continue
         parts = scene_lattice(nx, ny, spacing, jitter, seed=seed * 100003 + s,
                               x0=spacing / 2, y0=spacing / 2)[:n].copy()
         assert parts["id"][-1] == n - 1
