@@ -422,7 +422,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "plain 2D dam break, N=1,048,576 (1024x1024 lattice at 0.5h, "
                                "box 1200x800, jitter 0.01h), fused step via HIP graph",
-                   "particles": n, "cells": sx * sy},
+                   "particles": n, "cells": sx * sy,
+                   "library": os.path.relpath(sa.LIB_PATH, ROOT)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS if ach else None,
                      "traffic": traffic_of(KERNEL_SYMBOLS.get(dom, dom)),

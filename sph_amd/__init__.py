@@ -22,8 +22,32 @@ __all__ = ["PARTICLE", "Tunables", "Param", "AABB", "SceneDesc", "Counters", "So
            "LIB_PATH"]
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-# SPH_AMD_LIB may name another build of the library (e.g. a tuning variant).
-LIB_PATH = os.environ.get("SPH_AMD_LIB") or os.path.join(PKG_DIR, "libsph_hip.so")
+
+
+TUNED = os.path.join(PKG_DIR, "configs", "tuned.json")
+
+
+def tuned_library(tuned=None):
+    """Path of the library variant that configs/tuned.json (written by
+    sph_amd.autotune) records for its default workload, or None."""
+    import json
+    tuned = tuned or TUNED
+    if not os.path.exists(tuned):
+        return None
+    with open(tuned) as f:
+        cfg = json.load(f)
+    best = cfg.get("workloads", {}).get(cfg.get("default", ""), {}).get("best")
+    path = os.path.join(PKG_DIR, "variants", best) if best else None
+    return path if path and os.path.exists(path) else None
+
+
+def _lib_path():
+    """SPH_AMD_LIB if set; else the tuned variant; else the default build."""
+    return (os.environ.get("SPH_AMD_LIB") or tuned_library()
+            or os.path.join(PKG_DIR, "libsph_hip.so"))
+
+
+LIB_PATH = _lib_path()
 
 _FIELDS = ("x_prev", "y_prev", "x", "y", "v_x", "v_y", "a_x", "a_y",
            "density", "density_near", "pressure", "pressure_near")

@@ -4,6 +4,9 @@
 
 * sph_amd/libsph_hip.so - HIP kernels + context + reference-named entry
   points + scene builder, for gfx950 only;
+* sph_amd/variants/libsph_hip_t<cap>_i<ilp>.so - the same library with other
+  launch shapes of the gather kernels (LDS tile capacity per workgroup x
+  accepted pairs per walk iteration), among which sph_amd.autotune picks;
 * sph_amd/sph_main - the C driver (a start_simulation-shaped loop);
 * oracle/*.so (test infrastructure) via oracle/Makefile, and oracle/_ref
   when the reference sources are present.
@@ -37,6 +40,14 @@ C_FLAGS = ["-O2", "-fPIC", "-std=c11", "-ffp-contract=off", "-fno-fast-math", "-
 
 HIP_SRCS = ["sph_kernels.hip", "sph_ctx.hip", "sph_mlp.hip"]
 C_SRCS = ["scene.c", "sph_shim.c"]
+VARIANT_DIR = os.path.join(PKG, "variants")
+# (LDS tile capacity, pair ILP) of the gather kernels; (1536, 1) is the default
+# library's shape.  Only sph_kernels.hip depends on them.
+VARIANTS = [(1024, 1), (1536, 1), (2048, 1), (1024, 2), (1536, 2), (2048, 2)]
+
+
+def variant_name(cap, ilp):
+    return f"libsph_hip_t{cap}_i{ilp}.so"
 
 
 def _run(cmd, cwd=None):
@@ -47,23 +58,46 @@ def _run(cmd, cwd=None):
     return r.stdout
 
 
-def build_library(verbose=False):
+def _objects(defines=()):
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
+    objs = {}
     for src in HIP_SRCS:
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
-        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+        _run([HIPCC, *HIP_FLAGS, *defines, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs[src] = obj
     for src in C_SRCS:
         obj = os.path.join(BUILD, src.replace(".c", ".o"))
         _run(["gcc", *C_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
-    tmp = LIB + ".tmp"
+        objs[src] = obj
+    return objs
+
+
+def _link(objs, out):
+    tmp = out + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp])
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
+
+
+def build_library(verbose=False):
+    objs = _objects()
+    _link(list(objs.values()), LIB)
     if verbose:
         print("built", LIB)
-    return LIB
+    return LIB, objs
+
+
+def build_variants(objs, verbose=False):
+    """The library once per (tile capacity, pair ILP) shape."""
+    os.makedirs(VARIANT_DIR, exist_ok=True)
+    for cap, ilp in VARIANTS:
+        kobj = os.path.join(BUILD, f"sph_kernels_t{cap}_i{ilp}.o")
+        _run([HIPCC, *HIP_FLAGS, f"-DSPH_TILE_CAP={cap}", f"-DSPH_PAIR_ILP={ilp}", "-c",
+              os.path.join(CSRC, "sph_kernels.hip"), "-o", kobj])
+        rest = [o for src, o in objs.items() if src != "sph_kernels.hip"]
+        out = os.path.join(VARIANT_DIR, variant_name(cap, ilp))
+        _link([kobj, *rest], out)
+        if verbose:
+            print("built", out)
 
 
 def build_driver(verbose=False):
@@ -89,7 +123,8 @@ def build_oracle(verbose=False):
 
 
 def build(verbose=False):
-    build_library(verbose)
+    _, objs = build_library(verbose)
+    build_variants(objs, verbose)
     build_driver(verbose)
     build_oracle(verbose)
     shutil.rmtree(BUILD, ignore_errors=True)

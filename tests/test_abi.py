@@ -68,6 +68,30 @@ def test_library_exports_every_declared_function():
         getattr(L, n)
 
 
+def test_every_tuning_variant_exports_the_abi():
+    """The autotuner's library variants (other launch shapes of the gather
+    kernels) are drop-in replacements: same exported entry points."""
+    from sph_amd import build as bd
+    declared = declared_functions()
+    for cap, ilp in bd.VARIANTS:
+        path = os.path.join(bd.VARIANT_DIR, bd.variant_name(cap, ilp))
+        assert os.path.exists(path), f"{path}: build the variants first"
+        exported = exported_symbols(path)
+        assert not [n for n in declared if n not in exported], path
+
+
+def test_tuned_library_choice(tmp_path):
+    """sph_amd.lib() takes SPH_AMD_LIB, else the variant tuned.json records
+    for the default workload, else the default build."""
+    t = tmp_path / "tuned.json"
+    assert sa.tuned_library(str(t)) is None
+    t.write_text('{"default": "dam_break_1m", "workloads": '
+                 '{"dam_break_1m": {"best": "libsph_hip_t2048_i1.so"}}}')
+    assert sa.tuned_library(str(t)).endswith("variants/libsph_hip_t2048_i1.so")
+    t.write_text('{"default": "dam_break_1m", "workloads": {"dam_break_1m": {"best": "no_such.so"}}}')
+    assert sa.tuned_library(str(t)) is None
+
+
 def test_header_layout_compiles_and_matches_python(tmp_path):
     c = tmp_path / "t.c"
     c.write_text('#include "sph_abi.h"\n#include <stdio.h>\nint main(void){printf("%zu %zu %zu %zu\\n",'
