@@ -220,7 +220,7 @@ def test_reference_loop_through_shims_matches_step(tmp_path):
     graph-replayed sph_step bit for bit."""
     import os
     import subprocess
-    exe = os.path.join(os.path.dirname(sa.LIB_PATH), "sph_main")
+    exe = os.path.join(sa.PKG_DIR, "sph_main")
     out = tmp_path / "out.bin"
     r = subprocess.run([exe, "32", "24", "40", str(out)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
