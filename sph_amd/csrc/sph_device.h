@@ -248,18 +248,43 @@ __device__ __forceinline__ void sph_walk(unsigned s, unsigned key, const sph_dev
 }
 
 // The same walk in two passes per row, for SIMD efficiency: a branch-free
-// pass tests up to 64 candidates at a time (test(k, rec) -> accepted) and
-// records the accepted ones in a 64-bit mask; a second pass runs body() on
+// pass tests up to W (32 or 64) candidates at a time (test(k, rec) -> within
+// h) and records the accepted ones in a W-bit mask, from which the
+// particle's own slot `self` is then cleared; a second pass runs body() on
 // the set bits only, lowest first.  The visiting order of accepted
 // candidates is exactly sph_walk's, so sums are bit-identical, but a wave no
 // longer executes the pair work for every candidate that any one of its
-// lanes accepts.  body(k, dy, b1, b2, rec, use) applies its updates only
-// when `use` is true.  Returns the number of accepted candidates.
-template <class Src, class Test, class Body>
-__device__ __forceinline__ unsigned sph_walk_masked(unsigned key, const sph_dev_params &P,
+// lanes accepts.  W = 32 makes the mask one register (cheapest per
+// candidate: the density pass, whose pair work is light); W = 64 halves the
+// number of second-pass loops (the heavy relaxation and viscosity pairs).
+// body(k, dy, b1, b2, rec, use) applies its updates only when `use` is
+// true.  Returns the number of accepted candidates.
+template <int W> struct sph_mask_t;
+template <> struct sph_mask_t<32> {
+    typedef unsigned type;
+    static __device__ __forceinline__ unsigned ctz(unsigned m) { return (unsigned)__builtin_ctz(m); }
+    static __device__ __forceinline__ unsigned popc(unsigned m) { return (unsigned)__popc(m); }
+};
+template <> struct sph_mask_t<64> {
+    typedef unsigned long long type;
+    static __device__ __forceinline__ unsigned ctz(unsigned long long m)
+    {
+        return (unsigned)__builtin_ctzll(m);
+    }
+    static __device__ __forceinline__ unsigned popc(unsigned long long m)
+    {
+        return (unsigned)__popcll(m);
+    }
+};
+
+template <int W = 64, class Src, class Test, class Body>
+__device__ __forceinline__ unsigned sph_walk_masked(unsigned key, unsigned self,
+                                                    const sph_dev_params &P,
                                                     const unsigned *__restrict__ start,
                                                     const Src &src, Test &&test, Body &&body)
 {
+    typedef sph_mask_t<W> M;
+    typedef typename M::type mask_t;
     const unsigned sx = P.sx, sy = P.sy;
     const unsigned local = key - P.cell_base;
     const unsigned i = local % sx, j = local / sx;
@@ -275,21 +300,23 @@ __device__ __forceinline__ unsigned sph_walk_masked(unsigned key, const sph_dev_
         const unsigned b0 = start[base + i0], b1 = start[base + i];
         const unsigned b2 = start[base + i + 1], b3 = start[base + i1 + 1];
         const int row = src.row(dy);
-        for (unsigned c = b0; c < b3; c += 64) {
-            const unsigned m = b3 - c < 64u ? b3 - c : 64u;
-            unsigned long long mask = 0;
+        for (unsigned c = b0; c < b3; c += W) {
+            const unsigned m = b3 - c < (unsigned)W ? b3 - c : (unsigned)W;
+            mask_t mask = 0;
             for (unsigned t = 0; t < m; t++)
-                mask |= (unsigned long long)test(c + t, src.at(c + t, row)) << t;
-            accepted += (unsigned)__popcll(mask);
+                mask |= (mask_t)test(c + t, src.at(c + t, row)) << t;
+            if (self - c < m) // the particle itself is no neighbour
+                mask &= ~((mask_t)1 << (self - c));
+            accepted += M::popc(mask);
 #if SPH_PAIR_ILP >= 2
             // two accepted candidates per iteration, as independent chains;
             // the second is applied only if it exists (use = false turns
             // its updates into no-ops), so the order of updates is unchanged
             while (mask) {
-                const unsigned t0 = (unsigned)__builtin_ctzll(mask);
+                const unsigned t0 = M::ctz(mask);
                 mask &= mask - 1;
                 const bool two = mask != 0;
-                const unsigned t1 = two ? (unsigned)__builtin_ctzll(mask) : t0;
+                const unsigned t1 = two ? M::ctz(mask) : t0;
                 mask &= two ? mask - 1 : mask;
                 const auto r0 = src.at(c + t0, row);
                 const auto r1 = src.at(c + t1, row);
@@ -298,7 +325,7 @@ __device__ __forceinline__ unsigned sph_walk_masked(unsigned key, const sph_dev_
             }
 #else
             while (mask) {
-                const unsigned t = (unsigned)__builtin_ctzll(mask);
+                const unsigned t = M::ctz(mask);
                 mask &= mask - 1;
                 body(c + t, dy, b1, b2, src.at(c + t, row), true);
             }

@@ -280,7 +280,7 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
         if (VISC) {
             auto test = [&](unsigned k, float4 b) {
                 const float dx = b.x - a.x, ddy = b.y - a.y;
-                return (k != s) & !(dx * dx + ddy * ddy > P.h2);
+                return !(dx * dx + ddy * ddy > P.h2);
             };
             auto body = [&](unsigned k, int dy, unsigned b1, unsigned b2, float4 b, bool use) {
                 half += use && sph_owns(k, s, dy, b1, b2);
@@ -294,9 +294,9 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
             };
             const unsigned key = K[s];
             if (T.ok)
-                full = sph_walk_masked(key, P, start, lds_src(tile, T, key, P), test, body);
+                full = sph_walk_masked(key, s, P, start, lds_src(tile, T, key, P), test, body);
             else
-                full = sph_walk_masked(key, P, start, global_src<float4>([&](unsigned g) {
+                full = sph_walk_masked(key, s, P, start, global_src<float4>([&](unsigned g) {
                                            const float2 p = POS[g], v = VEL[g];
                                            return make_float4(p.x, p.y, v.x, v.y);
                                        }), test, body);
@@ -564,7 +564,7 @@ k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
         float rho = 0.0f, rhon = 0.0f;
         auto test = [&](unsigned k, float2 b) {
             const float dx = a.x - b.x, ddy = a.y - b.y;
-            return (k != s) & !(dx * dx + ddy * ddy > P.h2);
+            return !(dx * dx + ddy * ddy > P.h2);
         };
         auto body = [&](unsigned k, int dy, unsigned b1, unsigned b2, float2 b, bool use) {
             half += use && sph_owns(k, s, dy, b1, b2);
@@ -579,9 +579,9 @@ k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
         };
         const unsigned key = K[s];
         if (T.ok)
-            full = sph_walk_masked(key, P, start, lds_src(tile, T, key, P), test, body);
+            full = sph_walk_masked<32>(key, s, P, start, lds_src(tile, T, key, P), test, body);
         else
-            full = sph_walk_masked(key, P, start,
+            full = sph_walk_masked<32>(key, s, P, start,
                                    global_src<float2>([&](unsigned g) { return POS[g]; }), test,
                                    body);
         RHO[s] = make_float2(rho, rhon);
@@ -647,7 +647,7 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     float nx = a.x, ny = a.y;
     auto test = [&](unsigned k, float4 b) {
         const float dx = a.x - b.x, ddy = a.y - b.y;
-        return (k != s) & !(dx * dx + ddy * ddy > P.h2);
+        return !(dx * dx + ddy * ddy > P.h2);
     };
     auto body = [&](unsigned k, int dy, unsigned b1, unsigned b2, float4 b, bool use) {
         const float dx = a.x - b.x, ddy = a.y - b.y;
@@ -672,9 +672,9 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     };
     const unsigned key = K[s];
     if (T.ok)
-        sph_walk_masked(key, P, start, lds_src(tile, T, key, P), test, body);
+        sph_walk_masked(key, s, P, start, lds_src(tile, T, key, P), test, body);
     else
-        sph_walk_masked(key, P, start, global_src<float4>(load), test, body);
+        sph_walk_masked(key, s, P, start, global_src<float4>(load), test, body);
     if (FUSED) {
         sph_boundary(nx, ny, P);
         const float2 q = PREV[s];
