@@ -329,6 +329,11 @@ int sph_time_kernels(sph_ctx *ctx, int nsteps, int max_kinds, const char **names
 int sph_scene_lattice(fluid_particle *out, int nx, int ny, float x0, float y0, float spacing,
                       float jitter, uint64_t seed);
 
+/* Self-check of the kernels' correctly rounded square root (no counterpart
+ * in the reference): counts the float bit patterns in [lo_bits, hi_bits)
+ * whose sqrt differs from IEEE sqrtf (all NaNs count as equal). */
+int sph_selftest_sqrt(unsigned lo_bits, unsigned hi_bits, unsigned long long *mismatches);
+
 /* ------------------------------------------------------------------------
  * 3. Reference-named entry points (drop-in for src/fluid.h:112-126 and
  *    src/hash.h:50-52).  They drive one process-global context that is

@@ -171,6 +171,8 @@ def lib():
             "sph_surrogate_mfma_selftest": (ip, [vp]),
             "sph_surrogate_backward_trajectory": (ip, [vp, vp, vp, ctypes.POINTER(ctypes.c_float)]),
             "sph_trajectory_create": (ip, [ctypes.POINTER(vp), ip, ip, ip]),
+            "sph_selftest_sqrt": (ip, [ctypes.c_uint, ctypes.c_uint,
+                                       ctypes.POINTER(ctypes.c_uint64)]),
             "sph_trajectory_destroy": (None, [vp]),
             "sph_trajectory_record": (ip, [vp, vp, ip]),
             "sph_trajectory_read": (ip, [vp, ip, vp]),
@@ -496,6 +498,14 @@ class Trajectory:
         assert s.shape == f.shape
         _check_m(lib().sph_trajectory_load(self._h, solver._h, len(s), s.ctypes.data, f.ctypes.data),
                  "sph_trajectory_load")
+
+
+def selftest_sqrt(lo_bits: int, hi_bits: int) -> int:
+    """Mismatches of the kernels' square root against IEEE sqrtf over the
+    float bit patterns [lo_bits, hi_bits)."""
+    bad = ctypes.c_uint64()
+    _check(lib().sph_selftest_sqrt(lo_bits, hi_bits, ctypes.byref(bad)), "sph_selftest_sqrt")
+    return int(bad.value)
 
 
 def mfma_selftest() -> np.ndarray:

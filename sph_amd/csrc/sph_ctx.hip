@@ -1122,6 +1122,30 @@ int sph_time_kernels(sph_ctx *c, int nsteps, int max_kinds, const char **names, 
     return SPH_OK;
 }
 
+int sph_selftest_sqrt(unsigned lo_bits, unsigned hi_bits, unsigned long long *mismatches)
+{
+    if (!mismatches || hi_bits < lo_bits)
+        return set_err(SPH_EINVAL, "sph_selftest_sqrt: bad argument");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return set_err(SPH_ENODEV, "no HIP device available");
+    unsigned long long *d = nullptr;
+    HIPCHK(hipMalloc(&d, sizeof *d));
+    hipError_t e = hipMemset(d, 0, sizeof *d);
+    if (e == hipSuccess) {
+        sph_launch_selftest_sqrt(nullptr, lo_bits, hi_bits, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipDeviceSynchronize();
+    if (e == hipSuccess)
+        e = hipMemcpy(mismatches, d, sizeof *d, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess)
+        return set_err(SPH_EHIP, "sph_selftest_sqrt: %s", hipGetErrorString(e));
+    return SPH_OK;
+}
+
 } // extern "C"
 
 int sph_internal_canonicalize(sph_ctx *c) { return canonicalize(c); }

@@ -89,7 +89,8 @@ __device__ __forceinline__ unsigned sph_cell_key(float x, float y, const sph_dev
 // x - (s - ulp)*s <= 0, s + ulp when x - (s + ulp)*s > 0 (each residual
 // formed with one rounding by an FMA), else s.  Smaller, zero and
 // non-finite inputs take the general sqrtf path.  Results equal sqrtf's
-// (IEEE round-to-nearest) for every input.
+// (IEEE round-to-nearest) for every input: checked exhaustively over all
+// floats (sph_selftest_sqrt).
 __device__ __forceinline__ float sph_sqrt(float x)
 {
     if (!(x >= 0x1p-96f && x < __builtin_inff()))

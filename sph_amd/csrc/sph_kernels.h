@@ -91,3 +91,7 @@ void sph_launch_load_state(hipStream_t st, unsigned slots, const sph_dev_params 
 // Time the next kernel launch of this thread with a start/stop event pair
 // (recorded by the launch itself: the kernel's own duration).
 void sph_set_next_launch_events(hipEvent_t start, hipEvent_t stop);
+
+// Exhaustive bit-for-bit check of sph_sqrt against sqrtf over float bit
+// patterns [lo, hi).
+void sph_launch_selftest_sqrt(hipStream_t st, unsigned lo, unsigned hi, unsigned long long *bad);

@@ -796,6 +796,34 @@ void launch(F kernel, dim3 grid, dim3 block, hipStream_t st, Args... args)
 
 } // namespace
 
+// Exhaustive check of sph_sqrt: every float with bit pattern in [lo, hi)
+// (grid-stride) against sqrtf, bit for bit (any NaN equals any NaN).
+__device__ __attribute__((noinline)) float ieee_sqrt(float x) { return sqrtf(x); }
+
+__global__ void __launch_bounds__(256)
+k_selftest_sqrt(unsigned lo, unsigned hi, unsigned long long *__restrict__ bad)
+{
+    unsigned long long nbad = 0;
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    for (unsigned long long b = lo + (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+         b < hi; b += stride) {
+        const float x = __uint_as_float((unsigned)b);
+        const float got = sph_sqrt(x), want = ieee_sqrt(x);
+        const bool same = __float_as_uint(got) == __float_as_uint(want) ||
+                          (got != got && want != want);
+        nbad += same ? 0u : 1u;
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        nbad += __shfl_xor(nbad, o, 64);
+    if ((threadIdx.x & 63u) == 0 && nbad)
+        atomicAdd(bad, nbad);
+}
+
+void sph_launch_selftest_sqrt(hipStream_t st, unsigned lo, unsigned hi, unsigned long long *bad)
+{
+    launch(k_selftest_sqrt, dim3(4096), dim3(256), st, lo, hi, bad);
+}
+
 void sph_set_next_launch_events(hipEvent_t start, hipEvent_t stop)
 {
     g_next_start = start;
