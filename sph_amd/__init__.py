@@ -178,7 +178,10 @@ def lib():
             "sph_trajectory_read": (ip, [vp, ip, vp]),
             "sph_trajectory_load": (ip, [vp, vp, ip, vp, vp]),
         }
+        lenient = bool(os.environ.get("SPH_AMD_LIB"))  # another build may predate some calls
         for name, (res, args) in sigs.items():
+            if lenient and not hasattr(L, name):
+                continue
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args

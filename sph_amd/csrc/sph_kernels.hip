@@ -156,6 +156,33 @@ __device__ void plan_tile(unsigned s0, unsigned s1, const unsigned *__restrict__
     T.ok = total <= (unsigned)kTileCap;
 }
 
+// Slot of this lane's particle within its cell `key` for the counting sort:
+// one atomic per distinct key among the wavefront's live lanes, which take
+// consecutive slots in lane order (the reorder ranks a cell by particle id,
+// so any assignment of distinct slots gives the same sorted order).  Sorted
+// particles mostly stay in their cell, so a wavefront holds few distinct keys
+// and the same-address atomics of a per-particle count are avoided.  Call
+// with every live lane of the wavefront.
+__device__ __forceinline__ unsigned wave_slot(unsigned *cnt, unsigned key)
+{
+    const unsigned lane = threadIdx.x & 63u;
+    unsigned long long pending = __ballot(1);
+    unsigned slot = 0;
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const unsigned k = __shfl(key, leader, 64);
+        const unsigned long long same = __ballot(key == k);
+        unsigned base = 0;
+        if ((int)lane == leader)
+            base = atomicAdd(&cnt[k], (unsigned)__popcll(same));
+        base = __shfl(base, leader, 64);
+        if (key == k)
+            slot = base + (unsigned)__popcll(same & ((1ull << lane) - 1ull));
+        pending &= ~same;
+    }
+    return slot;
+}
+
 // Copy the planned segments into LDS (all threads), then barrier.
 template <class Rec, class Load>
 __device__ __forceinline__ void stage_tile(const TilePlan &T, Rec *tile, Load load)
@@ -253,7 +280,7 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
             held_vel[s] = VEL[s];
             const unsigned key = sph_cell_key(a.x, a.y, P, &C->key_out_of_range);
             out_key[s] = key;
-            out_slot[s] = atomicAdd(&cnt[key], 1u);
+            out_slot[s] = wave_slot(cnt, key);
         }
         return;
     }
@@ -308,7 +335,7 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
             out_pos[s] = make_float2(x, y);
             const unsigned key = sph_cell_key(x, y, P, &C->key_out_of_range);
             out_key[s] = key;
-            out_slot[s] = atomicAdd(&cnt[key], 1u);
+            out_slot[s] = wave_slot(cnt, key);
         } else {
             out_pos[s] = make_float2(nvx, nvy);
         }
@@ -387,7 +414,7 @@ k_key_hist(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     const float2 p = POS[s];
     const unsigned key = sph_cell_key(p.x, p.y, P, &C->key_out_of_range);
     out_key[s] = key;
-    out_slot[s] = atomicAdd(&cnt[key], 1u);
+    out_slot[s] = wave_slot(cnt, key);
 }
 
 // ---------------------------------------------------------------------------
@@ -625,7 +652,7 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
             out_pos[s] = a;
             const unsigned nk = sph_cell_key(a.x, a.y, P, &C->key_out_of_range);
             out_key[s] = nk;
-            out_slot[s] = atomicAdd(&cnt[nk], 1u);
+            out_slot[s] = wave_slot(cnt, nk);
         }
         return;
     }
@@ -686,7 +713,7 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
             atomicAdd(&C->nonfinite, 1ull);
         const unsigned nk = sph_cell_key(nx, ny, P, &C->key_out_of_range);
         out_key[s] = nk;
-        out_slot[s] = atomicAdd(&cnt[nk], 1u);
+        out_slot[s] = wave_slot(cnt, nk);
     }
     out_pos[s] = make_float2(nx, ny);
 }
