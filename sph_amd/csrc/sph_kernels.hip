@@ -114,47 +114,8 @@ struct TilePlan {
     unsigned g0[6];   // segment sorted begin, segment = (row - jA) * 3 + dy + 1
     unsigned len[6];  // segment length
     int off[6];       // LDS index = k + off[segment]
+    unsigned sv[12];  // the segments' bounding cell starts, read by 12 threads
 };
-
-// Thread 0 plans the tile of sorted particles [s0, s1).
-__device__ void plan_tile(unsigned s0, unsigned s1, const unsigned *__restrict__ K,
-                          const unsigned *__restrict__ start, const sph_dev_params &P,
-                          TilePlan &T)
-{
-    const unsigned sx = P.sx, sy = P.sy;
-    const unsigned kf = K[s0] - P.cell_base, kl = K[s1 - 1] - P.cell_base;
-    const unsigned jA = kf / sx, jB = kl / sx;
-    T.jA = jA;
-    T.ok = 0;
-    for (int q = 0; q < 6; q++) {
-        T.g0[q] = 0;
-        T.len[q] = 0;
-        T.off[q] = 0;
-    }
-    if (jB > jA + 1)
-        return;
-    unsigned total = 0;
-    for (unsigned rho = 0; rho <= jB - jA; rho++) {
-        const unsigned j = jA + rho;
-        const unsigned lo = rho == 0 ? kf % sx : 0;
-        const unsigned hi = j == jB ? kl % sx : sx - 1;
-        const unsigned c0 = lo > 0 ? lo - 1 : 0;
-        const unsigned c1 = hi + 1 < sx ? hi + 1 : sx - 1;
-        for (int dy = -1; dy <= 1; dy++) {
-            const long r = (long)j + dy;
-            if (r < 0 || r >= (long)sy)
-                continue;
-            const int q = (int)rho * 3 + dy + 1;
-            const unsigned a = start[P.cell_base + (unsigned)r * sx + c0];
-            const unsigned b = start[P.cell_base + (unsigned)r * sx + c1 + 1];
-            T.g0[q] = a;
-            T.len[q] = b - a;
-            T.off[q] = (int)total - (int)a;
-            total += b - a;
-        }
-    }
-    T.ok = total <= (unsigned)kTileCap;
-}
 
 // Slot of this lane's particle within its cell `key` for the counting sort:
 // one atomic per distinct key among the wavefront's live lanes, which take
@@ -181,6 +142,61 @@ __device__ __forceinline__ unsigned wave_slot(unsigned *cnt, unsigned key)
         pending &= ~same;
     }
     return slot;
+}
+
+// The tile plan of the sorted particles [s0, s1) of a workgroup: the
+// segments of the candidate rows of its (at most two) cell rows, i.e. for
+// each row the sorted range of cells c0-1 .. c1+1.  Made by the whole
+// workgroup: every thread reads the two end keys, threads 0..11 read one
+// segment-bounding cell start each, and thread 0 assembles the plan from
+// those with no memory access of its own.  Ends with a barrier.
+__device__ void plan_tile_par(unsigned s0, unsigned s1, const unsigned *__restrict__ K,
+                              const unsigned *__restrict__ start, const sph_dev_params &P,
+                              TilePlan &T)
+{
+    const unsigned sx = P.sx, sy = P.sy;
+    const unsigned kf = K[s0] - P.cell_base, kl = K[s1 - 1] - P.cell_base;
+    const unsigned jA = kf / sx, jB = kl / sx;
+    const unsigned t = threadIdx.x;
+    if (t < 12u && jB <= jA + 1) {
+        const unsigned q = t / 2u, rho = q / 3u;
+        const long r = (long)(jA + rho) + (long)(q % 3u) - 1;
+        if (rho <= jB - jA && r >= 0 && r < (long)sy) {
+            const unsigned j = jA + rho;
+            const unsigned lo = rho == 0 ? kf % sx : 0;
+            const unsigned hi = j == jB ? kl % sx : sx - 1;
+            const unsigned c0 = lo > 0 ? lo - 1 : 0;
+            const unsigned c1 = hi + 1 < sx ? hi + 1 : sx - 1;
+            T.sv[t] = start[P.cell_base + (unsigned)r * sx + ((t & 1u) ? c1 + 1 : c0)];
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        T.jA = jA;
+        T.ok = 0;
+        for (int q = 0; q < 6; q++) {
+            T.g0[q] = 0;
+            T.len[q] = 0;
+            T.off[q] = 0;
+        }
+        if (jB <= jA + 1) {
+            unsigned total = 0;
+            for (unsigned rho = 0; rho <= jB - jA; rho++)
+                for (int dy = -1; dy <= 1; dy++) {
+                    const long r = (long)(jA + rho) + dy;
+                    if (r < 0 || r >= (long)sy)
+                        continue;
+                    const int q = (int)rho * 3 + dy + 1;
+                    const unsigned a = T.sv[2 * q], b = T.sv[2 * q + 1];
+                    T.g0[q] = a;
+                    T.len[q] = b - a;
+                    T.off[q] = (int)total - (int)a;
+                    total += b - a;
+                }
+            T.ok = total <= (unsigned)kTileCap;
+        }
+    }
+    __syncthreads();
 }
 
 // Copy the planned segments into LDS (all threads), then barrier.
@@ -287,10 +303,10 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
     // sloshing: horizontal box acceleration, applied like gravity
     const bool slosh = GRAV && P.slosh_amp != 0.0f;
     const float sdt = slosh ? sph_slosh_accel(P, *clock) * P.dt : 0.0f;
+    // the lane's own particle, loaded before the planning barriers
+    const float2 a0 = POS[s], av0 = VEL[s];
     if (VISC) {
-        if (threadIdx.x == 0)
-            plan_tile(s0, s1, K, start, P, T);
-        __syncthreads();
+        plan_tile_par(s0, s1, K, start, P, T);
         if (T.ok)
             stage_tile(T, tile, [&](unsigned g) {
                 const float2 p = POS[g], v = VEL[g];
@@ -299,8 +315,8 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
     }
     unsigned full = 0, half = 0;
     if (s < s1) {
-        const float2 a = POS[s];
-        const float2 av = VEL[s];
+        const float2 a = a0;
+        const float2 av = av0;
         const float avx = slosh ? av.x + sdt : av.x;
         const float avy = GRAV ? av.y + gdt : av.y;
         float nvx = avx, nvy = avy;
@@ -580,14 +596,13 @@ k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     const unsigned s = s0 + threadIdx.x;
     if (s0 >= s1 || P.frozen)
         return; // a padding workgroup, or a frozen scene
-    if (threadIdx.x == 0)
-        plan_tile(s0, s1, K, start, P, T);
-    __syncthreads();
+    const float2 a0 = POS[s]; // the lane's own particle, before the barriers
+    plan_tile_par(s0, s1, K, start, P, T);
     if (T.ok)
         stage_tile(T, tile, [&](unsigned g) { return POS[g]; });
     unsigned full = 0, half = 0;
     if (s < s1) {
-        const float2 a = POS[s];
+        const float2 a = a0;
         float rho = 0.0f, rhon = 0.0f;
         auto test = [&](unsigned k, float2 b) {
             const float dx = a.x - b.x, ddy = a.y - b.y;
@@ -656,9 +671,11 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
         }
         return;
     }
-    if (threadIdx.x == 0)
-        plan_tile(s0, s1, K, start, P, T);
-    __syncthreads();
+    // the lane's own particle, loaded before the planning barriers
+    const float2 a = POS[s];
+    const float2 ar = RHO[s];
+    const unsigned key = K[s];
+    plan_tile_par(s0, s1, K, start, P, T);
     auto load = [&](unsigned g) {
         const float2 p = POS[g], r = RHO[g];
         return make_float4(p.x, p.y, r.x, r.y);
@@ -667,8 +684,6 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
         stage_tile(T, tile, load);
     if (s >= s1)
         return;
-    const float2 a = POS[s];
-    const float2 ar = RHO[s];
     const float aP = P.k * (ar.x - P.rest_density);
     const float aPn = P.k_near * ar.y;
     float nx = a.x, ny = a.y;
@@ -697,7 +712,6 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
             ny -= Dy;
         }
     };
-    const unsigned key = K[s];
     if (T.ok)
         sph_walk_masked(key, s, P, start, lds_src(tile, T, key, P), test, body);
     else
