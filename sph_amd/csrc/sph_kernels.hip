@@ -117,31 +117,34 @@ struct TilePlan {
     unsigned sv[12];  // the segments' bounding cell starts, read by 12 threads
 };
 
-// Slot of this lane's particle within its cell `key` for the counting sort:
-// one atomic per distinct key among the wavefront's live lanes, which take
-// consecutive slots in lane order (the reorder ranks a cell by particle id,
-// so any assignment of distinct slots gives the same sorted order).  Sorted
-// particles mostly stay in their cell, so a wavefront holds few distinct keys
-// and the same-address atomics of a per-particle count are avoided.  Call
-// with every live lane of the wavefront.
+// Slot of this lane's particle within its cell `key` for the counting sort.
+// The wavefront's live lanes are grouped by key first (ballots, no memory),
+// then each group's first lane reserves the group's slots with one atomic,
+// all groups in the same instruction: one round trip per wavefront instead
+// of one per particle (or per key).  Lanes of a group take consecutive slots
+// in lane order; the reorder ranks a cell by particle id, so any assignment
+// of distinct slots gives the same sorted order.  Call with every live lane.
 __device__ __forceinline__ unsigned wave_slot(unsigned *cnt, unsigned key)
 {
     const unsigned lane = threadIdx.x & 63u;
     unsigned long long pending = __ballot(1);
-    unsigned slot = 0;
+    int leader = 0;
+    unsigned rank = 0, count = 0;
     while (pending) {
-        const int leader = __ffsll((long long)pending) - 1;
-        const unsigned k = __shfl(key, leader, 64);
+        const int first = __ffsll((long long)pending) - 1;
+        const unsigned k = __shfl(key, first, 64);
         const unsigned long long same = __ballot(key == k);
-        unsigned base = 0;
-        if ((int)lane == leader)
-            base = atomicAdd(&cnt[k], (unsigned)__popcll(same));
-        base = __shfl(base, leader, 64);
-        if (key == k)
-            slot = base + (unsigned)__popcll(same & ((1ull << lane) - 1ull));
+        if (key == k) {
+            leader = first;
+            rank = (unsigned)__popcll(same & ((1ull << lane) - 1ull));
+            count = (unsigned)__popcll(same);
+        }
         pending &= ~same;
     }
-    return slot;
+    unsigned base = 0;
+    if ((int)lane == leader)
+        base = atomicAdd(&cnt[key], count);
+    return __shfl(base, leader, 64) + rank;
 }
 
 // The tile plan of the sorted particles [s0, s1) of a workgroup: the
