@@ -248,6 +248,39 @@ __device__ __forceinline__ void sph_walk(unsigned s, unsigned key, const sph_dev
     }
 }
 
+// The candidate rows of a particle in global cell `key`: row dy = -1, 0, 1
+// holds its cells i-1 / i / i+1 (clipped to the grid) at sorted positions
+// [b0,b1) / [b1,b2) / [b2,b3); a row outside the grid is empty.  Kernels
+// load them as soon as the key is known, so that the walk itself reads no
+// global memory but its candidates.
+struct sph_rows {
+    unsigned b0[3], b1[3], b2[3], b3[3];
+};
+
+__device__ __forceinline__ sph_rows sph_row_bounds(unsigned key, const sph_dev_params &P,
+                                                   const unsigned *__restrict__ start)
+{
+    sph_rows R;
+    const unsigned sx = P.sx, sy = P.sy;
+    const unsigned local = key - P.cell_base;
+    const unsigned i = local % sx, j = local / sx;
+    const unsigned i0 = i > 0 ? i - 1 : 0;
+    const unsigned i1 = i + 1 < sx ? i + 1 : sx - 1;
+#pragma unroll
+    for (int dy = -1; dy <= 1; dy++) {
+        const long r = (long)j + dy;
+        const bool in = r >= 0 && r < (long)sy;
+        const unsigned base = P.cell_base + (in ? (unsigned)r : j) * sx;
+        const unsigned b0 = start[base + i0], b1 = start[base + i];
+        const unsigned b2 = start[base + i + 1], b3 = start[base + i1 + 1];
+        R.b0[dy + 1] = in ? b0 : 0;
+        R.b1[dy + 1] = in ? b1 : 0;
+        R.b2[dy + 1] = in ? b2 : 0;
+        R.b3[dy + 1] = in ? b3 : 0;
+    }
+    return R;
+}
+
 // The same walk in two passes per row, for SIMD efficiency: a branch-free
 // pass tests up to W (32 or 64) candidates at a time (test(k, rec) -> within
 // h) and records the accepted ones in a W-bit mask, from which the
@@ -279,27 +312,16 @@ template <> struct sph_mask_t<64> {
 };
 
 template <int W = 64, class Src, class Test, class Body>
-__device__ __forceinline__ unsigned sph_walk_masked(unsigned key, unsigned self,
-                                                    const sph_dev_params &P,
-                                                    const unsigned *__restrict__ start,
+__device__ __forceinline__ unsigned sph_walk_masked(unsigned self, const sph_rows &R,
                                                     const Src &src, Test &&test, Body &&body)
 {
     typedef sph_mask_t<W> M;
     typedef typename M::type mask_t;
-    const unsigned sx = P.sx, sy = P.sy;
-    const unsigned local = key - P.cell_base;
-    const unsigned i = local % sx, j = local / sx;
-    const unsigned i0 = i > 0 ? i - 1 : 0;
-    const unsigned i1 = i + 1 < sx ? i + 1 : sx - 1;
     unsigned accepted = 0;
 #pragma unroll
     for (int dy = -1; dy <= 1; dy++) {
-        const long r = (long)j + dy;
-        if (r < 0 || r >= (long)sy)
-            continue;
-        const unsigned base = P.cell_base + (unsigned)r * sx;
-        const unsigned b0 = start[base + i0], b1 = start[base + i];
-        const unsigned b2 = start[base + i + 1], b3 = start[base + i1 + 1];
+        const unsigned b0 = R.b0[dy + 1], b1 = R.b1[dy + 1];
+        const unsigned b2 = R.b2[dy + 1], b3 = R.b3[dy + 1];
         const int row = src.row(dy);
         for (unsigned c = b0; c < b3; c += W) {
             const unsigned m = b3 - c < (unsigned)W ? b3 - c : (unsigned)W;

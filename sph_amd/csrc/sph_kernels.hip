@@ -31,9 +31,12 @@
 // scene's parameters as uniform values (BS maps workgroup -> scene).
 //
 // Workgroups are numbered XCD-aware: the hardware deals workgroups to the
-// 8 XCDs round-robin, so logical tile b is remapped such that each XCD
-// processes one contiguous eighth of the sorted array and its neighbours
-// stay in that XCD's L2.
+// 8 XCDs round-robin, and logical tiles are remapped so that each XCD takes
+// runs of 16 consecutive tiles (which share candidate rows in its L2), the
+// runs dealt to the XCDs in turn.  Whole eighths per XCD would keep more rows
+// in one L2, but the work per tile follows the fluid's density, which varies
+// with height: in a settling column the densest eighth costs twice the mean,
+// and that XCD would finish last.
 #include "sph_kernels.h"
 
 #include <hip/hip_ext.h>
@@ -47,6 +50,10 @@ constexpr int kTPB = 256;
 #endif
 constexpr int kTileCap = SPH_TILE_CAP; // staged candidates per workgroup
 constexpr unsigned kXcds = 8;
+#ifndef SPH_XCD_CHUNK
+#define SPH_XCD_CHUNK 16
+#endif
+constexpr unsigned kXcdChunk = SPH_XCD_CHUNK; // consecutive tiles per XCD run
 
 __device__ __forceinline__ unsigned wave_max_u32(unsigned v)
 {
@@ -66,13 +73,16 @@ __device__ __forceinline__ unsigned wave_sum_u32(unsigned v)
 
 __device__ __forceinline__ bool lane0() { return (threadIdx.x & (kWave - 1)) == 0; }
 
-// Logical workgroup index with each XCD's share contiguous.
+// Logical workgroup index: runs of kXcdChunk consecutive tiles go to one
+// XCD, and consecutive runs to consecutive XCDs.
 __device__ __forceinline__ unsigned xcd_block()
 {
     const unsigned b = blockIdx.x, nb = gridDim.x;
+    const unsigned full = nb / (kXcds * kXcdChunk) * (kXcds * kXcdChunk);
+    if (b >= full)
+        return b; // the last, partial round keeps dispatch order
     const unsigned xcd = b % kXcds, local = b / kXcds;
-    const unsigned per = nb / kXcds, rem = nb % kXcds;
-    return xcd * per + min(xcd, rem) + local;
+    return ((local / kXcdChunk) * kXcds + xcd) * kXcdChunk + local % kXcdChunk;
 }
 
 // Raise a running maximum.  Every wave of a launch reports to the same
@@ -114,6 +124,8 @@ struct TilePlan {
     unsigned g0[6];   // segment sorted begin, segment = (row - jA) * 3 + dy + 1
     unsigned len[6];  // segment length
     int off[6];       // LDS index = k + off[segment]
+    unsigned tb[6];   // LDS index of the segment's first record (~0u: no segment)
+    unsigned total;   // staged records
     unsigned sv[12];  // the segments' bounding cell starts, read by 12 threads
 };
 
@@ -181,7 +193,9 @@ __device__ void plan_tile_par(unsigned s0, unsigned s1, const unsigned *__restri
             T.g0[q] = 0;
             T.len[q] = 0;
             T.off[q] = 0;
+            T.tb[q] = ~0u;
         }
+        T.total = 0;
         if (jB <= jA + 1) {
             unsigned total = 0;
             for (unsigned rho = 0; rho <= jB - jA; rho++)
@@ -194,24 +208,44 @@ __device__ void plan_tile_par(unsigned s0, unsigned s1, const unsigned *__restri
                     T.g0[q] = a;
                     T.len[q] = b - a;
                     T.off[q] = (int)total - (int)a;
+                    T.tb[q] = total;
                     total += b - a;
                 }
             T.ok = total <= (unsigned)kTileCap;
+            T.total = total;
         }
     }
     __syncthreads();
 }
 
-// Copy the planned segments into LDS (all threads), then barrier.
+// Copy the planned segments into LDS (all threads), then barrier.  The
+// segments lie back to back in the tile, so thread t stages tile records
+// t, t + 256, ... of the whole tile: it finds each record's segment (the
+// last one beginning at or before it), issues all of its loads, and only
+// then stores them - one memory round trip per workgroup, not one per
+// segment.
 template <class Rec, class Load>
 __device__ __forceinline__ void stage_tile(const TilePlan &T, Rec *tile, Load load)
 {
+    constexpr int kPer = (kTileCap + kTPB - 1) / kTPB;
+    const unsigned total = T.total;
+    Rec v[kPer];
 #pragma unroll
-    for (int q = 0; q < 6; q++) {
-        const unsigned g0 = T.g0[q], len = T.len[q];
-        const int off = T.off[q];
-        for (unsigned t = threadIdx.x; t < len; t += kTPB)
-            tile[(int)(g0 + t) + off] = load(g0 + t);
+    for (int u = 0; u < kPer; u++) {
+        const unsigned i = threadIdx.x + (unsigned)u * kTPB;
+        if (i < total) {
+            int off = T.off[0];
+#pragma unroll
+            for (int q = 1; q < 6; q++)
+                off = T.tb[q] <= i ? T.off[q] : off;
+            v[u] = load((unsigned)((int)i - off));
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+        const unsigned i = threadIdx.x + (unsigned)u * kTPB;
+        if (i < total)
+            tile[i] = v[u];
     }
     __syncthreads();
 }
@@ -306,8 +340,11 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
     // sloshing: horizontal box acceleration, applied like gravity
     const bool slosh = GRAV && P.slosh_amp != 0.0f;
     const float sdt = slosh ? sph_slosh_accel(P, *clock) * P.dt : 0.0f;
-    // the lane's own particle, loaded before the planning barriers
+    // the lane's own particle and candidate rows, loaded before the planning
+    // barriers
     const float2 a0 = POS[s], av0 = VEL[s];
+    const unsigned key = VISC && s < s1 ? K[s] : P.cell_base; // pad lanes: any valid cell
+    const sph_rows R = VISC ? sph_row_bounds(key, P, start) : sph_rows{};
     if (VISC) {
         plan_tile_par(s0, s1, K, start, P, T);
         if (T.ok)
@@ -338,11 +375,10 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
                     nvy -= iy * 0.5f;
                 }
             };
-            const unsigned key = K[s];
             if (T.ok)
-                full = sph_walk_masked(key, s, P, start, lds_src(tile, T, key, P), test, body);
+                full = sph_walk_masked(s, R, lds_src(tile, T, key, P), test, body);
             else
-                full = sph_walk_masked(key, s, P, start, global_src<float4>([&](unsigned g) {
+                full = sph_walk_masked(s, R, global_src<float4>([&](unsigned g) {
                                            const float2 p = POS[g], v = VEL[g];
                                            return make_float4(p.x, p.y, v.x, v.y);
                                        }), test, body);
@@ -599,7 +635,10 @@ k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     const unsigned s = s0 + threadIdx.x;
     if (s0 >= s1 || P.frozen)
         return; // a padding workgroup, or a frozen scene
-    const float2 a0 = POS[s]; // the lane's own particle, before the barriers
+    // the lane's own particle and candidate rows, before the barriers
+    const float2 a0 = POS[s];
+    const unsigned key = s < s1 ? K[s] : P.cell_base; // pad lanes: any valid cell
+    const sph_rows R = sph_row_bounds(key, P, start);
     plan_tile_par(s0, s1, K, start, P, T);
     if (T.ok)
         stage_tile(T, tile, [&](unsigned g) { return POS[g]; });
@@ -622,13 +661,11 @@ k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
                 rhon += OmR2 * (1.0f - ratio);
             }
         };
-        const unsigned key = K[s];
         if (T.ok)
-            full = sph_walk_masked<32>(key, s, P, start, lds_src(tile, T, key, P), test, body);
+            full = sph_walk_masked<32>(s, R, lds_src(tile, T, key, P), test, body);
         else
-            full = sph_walk_masked<32>(key, s, P, start,
-                                   global_src<float2>([&](unsigned g) { return POS[g]; }), test,
-                                   body);
+            full = sph_walk_masked<32>(s, R, global_src<float2>([&](unsigned g) { return POS[g]; }),
+                                       test, body);
         RHO[s] = make_float2(rho, rhon);
     }
     report_neighbors(full, half, C);
@@ -677,7 +714,8 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     // the lane's own particle, loaded before the planning barriers
     const float2 a = POS[s];
     const float2 ar = RHO[s];
-    const unsigned key = K[s];
+    const unsigned key = s < s1 ? K[s] : P.cell_base; // pad lanes: any valid cell
+    const sph_rows R = sph_row_bounds(key, P, start);
     plan_tile_par(s0, s1, K, start, P, T);
     auto load = [&](unsigned g) {
         const float2 p = POS[g], r = RHO[g];
@@ -716,9 +754,9 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
         }
     };
     if (T.ok)
-        sph_walk_masked(key, s, P, start, lds_src(tile, T, key, P), test, body);
+        sph_walk_masked(s, R, lds_src(tile, T, key, P), test, body);
     else
-        sph_walk_masked(key, s, P, start, global_src<float4>(load), test, body);
+        sph_walk_masked(s, R, global_src<float4>(load), test, body);
     if (FUSED) {
         sph_boundary(nx, ny, P);
         const float2 q = PREV[s];
