@@ -2,8 +2,10 @@
 
 The plain step's three gather kernels (viscosity + predict, density,
 relaxation) stage each workgroup's neighbour cells in LDS up to a tile
-capacity, and walk accepted pairs one or two at a time (pair ILP).  Both
-are compile-time shapes: sph_amd.build compiles one library per shape into
+capacity, and deal their workgroups to the 8 XCDs in runs of consecutive
+tiles (longer runs share more candidate rows in one XCD's L2, shorter runs
+spread dense regions of the fluid more evenly over the XCDs).  Both are
+compile-time shapes: sph_amd.build compiles one library per shape into
 sph_amd/variants/.  This tuner times every variant on a workload, each in a
 fresh process (SPH_AMD_LIB names the variant), and records the results and
 the fastest variant in configs/tuned.json.  sph_amd.lib() loads the variant
@@ -60,8 +62,8 @@ def _worker(workload, steps, warmup):
 def tune(workload=DEFAULT_WORKLOAD, steps=100, warmup=20, timeout=300):
     from sph_amd import build as bd
     results = {}
-    for cap, ilp in bd.VARIANTS:
-        path = os.path.join(bd.VARIANT_DIR, bd.variant_name(cap, ilp))
+    for cap, run in bd.VARIANTS:
+        path = os.path.join(bd.VARIANT_DIR, bd.variant_name(cap, run))
         if not os.path.exists(path):
             raise FileNotFoundError(f"{path}: build the variants first (python -m sph_amd.build)")
         env = dict(os.environ, SPH_AMD_LIB=path)
@@ -70,9 +72,9 @@ def tune(workload=DEFAULT_WORKLOAD, steps=100, warmup=20, timeout=300):
                            cwd=ROOT, env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
                            text=True, timeout=timeout)
         if r.returncode != 0:
-            raise RuntimeError(f"variant {bd.variant_name(cap, ilp)} failed: {r.stderr[-2000:]}")
-        results[bd.variant_name(cap, ilp)] = json.loads(r.stdout.strip().splitlines()[-1])
-        print(bd.variant_name(cap, ilp), results[bd.variant_name(cap, ilp)], flush=True)
+            raise RuntimeError(f"variant {bd.variant_name(cap, run)} failed: {r.stderr[-2000:]}")
+        results[bd.variant_name(cap, run)] = json.loads(r.stdout.strip().splitlines()[-1])
+        print(bd.variant_name(cap, run), results[bd.variant_name(cap, run)], flush=True)
     best = min(results, key=lambda k: results[k]["ms_per_step"])
     return {"best": best, "steps": steps, "warmup": warmup, "results": results}
 
@@ -98,6 +100,9 @@ def main(argv=None):
     res = tune(args.workload, args.steps, args.warmup)
     cfg = load_tuned() if args.out == TUNED else {}
     cfg.setdefault("default", DEFAULT_WORKLOAD)
+    cfg["comment"] = ("written by python -m sph_amd.autotune on one MI355X: ms per graph-replayed "
+                      "step of each library variant (LDS tile capacity t, tiles per XCD run x) "
+                      "after the warm-up steps")
     cfg.setdefault("workloads", {})[args.workload] = res
     with open(args.out, "w") as f:
         json.dump(cfg, f, indent=1, sort_keys=True)

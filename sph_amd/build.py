@@ -4,9 +4,9 @@
 
 * sph_amd/libsph_hip.so - HIP kernels + context + reference-named entry
   points + scene builder, for gfx950 only;
-* sph_amd/variants/libsph_hip_t<cap>_i<ilp>.so - the same library with other
+* sph_amd/variants/libsph_hip_t<cap>_x<run>.so - the same library with other
   launch shapes of the gather kernels (LDS tile capacity per workgroup x
-  accepted pairs per walk iteration), among which sph_amd.autotune picks;
+  consecutive tiles per XCD run), among which sph_amd.autotune picks;
 * sph_amd/sph_main - the C driver (a start_simulation-shaped loop);
 * oracle/*.so (test infrastructure) via oracle/Makefile, and oracle/_ref
   when the reference sources are present.
@@ -41,13 +41,17 @@ C_FLAGS = ["-O2", "-fPIC", "-std=c11", "-ffp-contract=off", "-fno-fast-math", "-
 HIP_SRCS = ["sph_kernels.hip", "sph_ctx.hip", "sph_mlp.hip"]
 C_SRCS = ["scene.c", "sph_shim.c"]
 VARIANT_DIR = os.path.join(PKG, "variants")
-# (LDS tile capacity, pair ILP) of the gather kernels; (1536, 1) is the default
-# library's shape.  Only sph_kernels.hip depends on them.
-VARIANTS = [(1024, 1), (1536, 1), (2048, 1), (1024, 2), (1536, 2), (2048, 2)]
+# (LDS tile capacity, tiles per XCD run) of the gather kernels; (1280, 16) is
+# the default library's shape.  Only sph_kernels.hip depends on them.
+VARIANTS = [(1024, 16), (1280, 16), (1536, 16), (2048, 16), (1536, 8), (1536, 32)]
+# Shapes built for the tests only: a 256-record tile sends nearly every
+# workgroup to the global-memory walk, and runs of one tile exercise the XCD
+# mapping's other extreme.
+CHECK_VARIANTS = [(256, 1)]
 
 
-def variant_name(cap, ilp):
-    return f"libsph_hip_t{cap}_i{ilp}.so"
+def variant_name(cap, run):
+    return f"libsph_hip_t{cap}_x{run}.so"
 
 
 def _run(cmd, cwd=None):
@@ -87,14 +91,14 @@ def build_library(verbose=False):
 
 
 def build_variants(objs, verbose=False):
-    """The library once per (tile capacity, pair ILP) shape."""
+    """The library once per (tile capacity, XCD run) shape."""
     os.makedirs(VARIANT_DIR, exist_ok=True)
-    for cap, ilp in VARIANTS:
-        kobj = os.path.join(BUILD, f"sph_kernels_t{cap}_i{ilp}.o")
-        _run([HIPCC, *HIP_FLAGS, f"-DSPH_TILE_CAP={cap}", f"-DSPH_PAIR_ILP={ilp}", "-c",
+    for cap, run in VARIANTS + CHECK_VARIANTS:
+        kobj = os.path.join(BUILD, f"sph_kernels_t{cap}_x{run}.o")
+        _run([HIPCC, *HIP_FLAGS, f"-DSPH_TILE_CAP={cap}", f"-DSPH_XCD_CHUNK={run}", "-c",
               os.path.join(CSRC, "sph_kernels.hip"), "-o", kobj])
         rest = [o for src, o in objs.items() if src != "sph_kernels.hip"]
-        out = os.path.join(VARIANT_DIR, variant_name(cap, ilp))
+        out = os.path.join(VARIANT_DIR, variant_name(cap, run))
         _link([kobj, *rest], out)
         if verbose:
             print("built", out)

@@ -10,12 +10,6 @@
 
 #include "../../include/sph_abi.h"
 
-// Accepted neighbour pairs evaluated per iteration of a particle's walk
-// (independent dependency chains for latency hiding).
-#ifndef SPH_PAIR_ILP
-#define SPH_PAIR_ILP 1
-#endif
-
 // Step parameters of one scene as the kernels read them, derived on the
 // host from tunable_parameters exactly as the reference derives them
 // (h_recip = 1.0f/h: src/fluid.c:427, 555; g = -g: src/fluid.c:403;
@@ -221,9 +215,10 @@ __device__ __forceinline__ bool sph_owns(unsigned k, unsigned s, int dy, unsigne
 // cell `key`) in canonical order: rows j-1, j, j+1; in each row the contiguous
 // sorted range of cells i-1..i+1 (clipped to the grid), ascending.  Self
 // (k == s) is visited too; every body rejects it together with its distance
-// test, so the wave takes one branch per candidate.  `src.row(dy)` prepares
-// a row (an LDS offset for staged tiles); `src.at(k, row)` returns candidate
-// k's staged record; body(k, dy, b1, b2, rec).
+// test, so the wave takes one branch per candidate.  `src.row(dy)` is the
+// offset from sorted position to source index in row dy (the LDS offset of
+// a staged tile, 0 for global memory); `src.at(idx)` returns the record at a
+// source index; body(k, dy, b1, b2, rec).
 template <class Src, class Body>
 __device__ __forceinline__ void sph_walk(unsigned s, unsigned key, const sph_dev_params &P,
                                          const unsigned *__restrict__ start, const Src &src,
@@ -242,9 +237,9 @@ __device__ __forceinline__ void sph_walk(unsigned s, unsigned key, const sph_dev
         const unsigned base = P.cell_base + (unsigned)r * sx;
         const unsigned b0 = start[base + i0], b1 = start[base + i];
         const unsigned b2 = start[base + i + 1], b3 = start[base + i1 + 1];
-        const int row = src.row(dy);
+        const unsigned off = (unsigned)src.row(dy);
         for (unsigned k = b0; k < b3; k++)
-            body(k, dy, b1, b2, src.at(k, row));
+            body(k, dy, b1, b2, src.at(k + off));
     }
 }
 
@@ -281,79 +276,102 @@ __device__ __forceinline__ sph_rows sph_row_bounds(unsigned key, const sph_dev_p
     return R;
 }
 
-// The same walk in two passes per row, for SIMD efficiency: a branch-free
-// pass tests up to W (32 or 64) candidates at a time (test(k, rec) -> within
-// h) and records the accepted ones in a W-bit mask, from which the
-// particle's own slot `self` is then cleared; a second pass runs body() on
-// the set bits only, lowest first.  The visiting order of accepted
-// candidates is exactly sph_walk's, so sums are bit-identical, but a wave no
-// longer executes the pair work for every candidate that any one of its
-// lanes accepts.  W = 32 makes the mask one register (cheapest per
-// candidate: the density pass, whose pair work is light); W = 64 halves the
-// number of second-pass loops (the heavy relaxation and viscosity pairs).
-// body(k, dy, b1, b2, rec, use) applies its updates only when `use` is
-// true.  Returns the number of accepted candidates.
-template <int W> struct sph_mask_t;
-template <> struct sph_mask_t<32> {
-    typedef unsigned type;
-    static __device__ __forceinline__ unsigned ctz(unsigned m) { return (unsigned)__builtin_ctz(m); }
-    static __device__ __forceinline__ unsigned popc(unsigned m) { return (unsigned)__popc(m); }
-};
-template <> struct sph_mask_t<64> {
-    typedef unsigned long long type;
-    static __device__ __forceinline__ unsigned ctz(unsigned long long m)
-    {
-        return (unsigned)__builtin_ctzll(m);
-    }
-    static __device__ __forceinline__ unsigned popc(unsigned long long m)
-    {
-        return (unsigned)__popcll(m);
-    }
-};
-
-template <int W = 64, class Src, class Test, class Body>
-__device__ __forceinline__ unsigned sph_walk_masked(unsigned self, const sph_rows &R,
-                                                    const Src &src, Test &&test, Body &&body)
+// The same walk in two passes, for SIMD efficiency.  Pass 1 tests the
+// candidates 32 at a time (test(rec) -> within h), branch-free, into a mask
+// of accepted ones, from which the particle's own slot `self` is cleared;
+// non-empty masks wait in a per-lane queue of up to three chunks (mask,
+// source index of bit 0, owner threshold).  Pass 2 drains the queue in the
+// order the chunks were found, lowest bit first, running body() on each
+// accepted candidate: exactly sph_walk's visiting order, so sums are
+// bit-identical.  The whole wave drains together, at the end and whenever
+// some lane's queue is full, so a lane runs the accepted pairs of up to
+// three chunks (usually all three rows) in one loop: a wave executes as many
+// pair iterations as its busiest lane has pairs, where a drain per row would
+// execute the sum over rows of each row's busiest lane.
+//
+// The half-list count of src/hash.c:176-237 is taken per chunk: the pair is
+// in self's half list when the candidate lies in a forward cell (0,+1),
+// (+1,-1), (+1,0), (+1,+1) or shares self's cell at a later sorted position
+// (cells are sorted by id), i.e. when its sorted position is at least b1 in
+// row +1, self + 1 in row 0 and b2 in row -1.  body(idx, own, rec) receives
+// that test for the candidate when OWN is set (false otherwise); idx is the
+// candidate's index in the source.  Returns (accepted, in the half list).
+template <bool OWN = false, class Src, class Test, class Body>
+__device__ __forceinline__ uint2 sph_walk_masked(unsigned self, const sph_rows &R, const Src &src,
+                                                 Test &&test, Body &&body)
 {
-    typedef sph_mask_t<W> M;
-    typedef typename M::type mask_t;
-    unsigned accepted = 0;
+    unsigned m0 = 0, m1 = 0, m2 = 0; // pending masks, m0 drained first
+    unsigned x0 = 0, x1 = 0, x2 = 0; // source index of each mask's bit 0
+    unsigned t0 = 0, t1 = 0, t2 = 0; // owner threshold, source index space
+    unsigned nq = 0;
+    unsigned full = 0, half = 0;
+    auto drain = [&]() {
+        while (m0) {
+            const unsigned t = (unsigned)__builtin_ctz(m0);
+            m0 &= m0 - 1u;
+            const unsigned idx = x0 + t;
+            body(idx, OWN && idx >= t0, src.at(idx));
+            if (m0 == 0) {
+                m0 = m1;
+                x0 = x1;
+                t0 = t1;
+                m1 = m2;
+                x1 = x2;
+                t1 = t2;
+                m2 = 0;
+            }
+        }
+        nq = 0;
+    };
 #pragma unroll
     for (int dy = -1; dy <= 1; dy++) {
-        const unsigned b0 = R.b0[dy + 1], b1 = R.b1[dy + 1];
-        const unsigned b2 = R.b2[dy + 1], b3 = R.b3[dy + 1];
-        const int row = src.row(dy);
-        for (unsigned c = b0; c < b3; c += W) {
-            const unsigned m = b3 - c < (unsigned)W ? b3 - c : (unsigned)W;
-            mask_t mask = 0;
-            for (unsigned t = 0; t < m; t++)
-                mask |= (mask_t)test(c + t, src.at(c + t, row)) << t;
-            if (self - c < m) // the particle itself is no neighbour
-                mask &= ~((mask_t)1 << (self - c));
-            accepted += M::popc(mask);
-#if SPH_PAIR_ILP >= 2
-            // two accepted candidates per iteration, as independent chains;
-            // the second is applied only if it exists (use = false turns
-            // its updates into no-ops), so the order of updates is unchanged
-            while (mask) {
-                const unsigned t0 = M::ctz(mask);
-                mask &= mask - 1;
-                const bool two = mask != 0;
-                const unsigned t1 = two ? M::ctz(mask) : t0;
-                mask &= two ? mask - 1 : mask;
-                const auto r0 = src.at(c + t0, row);
-                const auto r1 = src.at(c + t1, row);
-                body(c + t0, dy, b1, b2, r0, true);
-                body(c + t1, dy, b1, b2, r1, two);
+        // the row, its owner threshold and self in source index space (each
+        // a true index in the row's range where it matters: the arithmetic
+        // is modular, so a negative offset cancels)
+        const unsigned off = (unsigned)src.row(dy);
+        const unsigned thr = dy == 1 ? R.b1[dy + 1] : dy == 0 ? self + 1u : R.b2[dy + 1];
+        const unsigned tt = thr + off, me = self + off;
+        const unsigned b3 = R.b3[dy + 1] + off;
+        unsigned c = R.b0[dy + 1] + off;
+        // The chunk loop runs while any lane of the wave has candidates left
+        // in this row (lanes that are done contribute empty masks), so a
+        // drain is always taken by the whole wave at once: lanes' pair work
+        // overlaps instead of running lane after lane.
+        while (__any(c < b3)) {
+            unsigned mask = 0, x = 0;
+            if (c < b3) {
+                const unsigned m = b3 - c < 32u ? b3 - c : 32u;
+#pragma unroll 8
+                for (unsigned t = 0; t < m; t++)
+                    mask |= (unsigned)test(src.at(c + t)) << t;
+                if (me - c < m) // the particle itself is no neighbour
+                    mask &= ~(1u << (me - c));
+                full += (unsigned)__popc(mask);
+                const unsigned own_from = tt > c ? tt - c : 0u;
+                half += (unsigned)__popc(own_from < 32u ? mask & (~0u << own_from) : 0u);
+                x = c;
+                c += 32u;
             }
-#else
-            while (mask) {
-                const unsigned t = M::ctz(mask);
-                mask &= mask - 1;
-                body(c + t, dy, b1, b2, src.at(c + t, row), true);
+            if (__any(nq == 3u && mask != 0u))
+                drain();
+            if (mask) {
+                if (nq == 0u) {
+                    m0 = mask;
+                    x0 = x;
+                    t0 = tt;
+                } else if (nq == 1u) {
+                    m1 = mask;
+                    x1 = x;
+                    t1 = tt;
+                } else {
+                    m2 = mask;
+                    x2 = x;
+                    t2 = tt;
+                }
+                nq++;
             }
-#endif
         }
     }
-    return accepted;
+    drain();
+    return make_uint2(full, half);
 }

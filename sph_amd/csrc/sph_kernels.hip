@@ -46,7 +46,7 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kTPB = 256;
 #ifndef SPH_TILE_CAP
-#define SPH_TILE_CAP 1536
+#define SPH_TILE_CAP 1280
 #endif
 constexpr int kTileCap = SPH_TILE_CAP; // staged candidates per workgroup
 constexpr unsigned kXcds = 8;
@@ -220,10 +220,9 @@ __device__ void plan_tile_par(unsigned s0, unsigned s1, const unsigned *__restri
 
 // Copy the planned segments into LDS (all threads), then barrier.  The
 // segments lie back to back in the tile, so thread t stages tile records
-// t, t + 256, ... of the whole tile: it finds each record's segment (the
-// last one beginning at or before it), issues all of its loads, and only
-// then stores them - one memory round trip per workgroup, not one per
-// segment.
+// t, t + 256, ...: it finds each record's segment (the last one beginning at
+// or before it), issues all of its loads, and only then stores them - one
+// memory round trip per workgroup, not one per segment.
 template <class Rec, class Load>
 __device__ __forceinline__ void stage_tile(const TilePlan &T, Rec *tile, Load load)
 {
@@ -250,14 +249,15 @@ __device__ __forceinline__ void stage_tile(const TilePlan &T, Rec *tile, Load lo
     __syncthreads();
 }
 
-// Candidate source reading the LDS tile.
+// Candidate source reading the LDS tile: the tile index of sorted position
+// k of row dy is k + row(dy).
 template <class Rec>
 struct LdsSrc {
     const Rec *tile;
     int offA[3], offB[3];
     bool rowB;
     __device__ int row(int dy) const { return rowB ? offB[dy + 1] : offA[dy + 1]; }
-    __device__ Rec at(unsigned k, int off) const { return tile[(int)k + off]; }
+    __device__ Rec at(unsigned idx) const { return tile[idx]; }
 };
 
 template <class Rec>
@@ -279,13 +279,33 @@ template <class Rec, class Load>
 struct GlobalSrc {
     Load load;
     __device__ int row(int) const { return 0; }
-    __device__ Rec at(unsigned k, int) const { return load(k); }
+    __device__ Rec at(unsigned idx) const { return load(idx); }
 };
 
 template <class Rec, class Load>
 __device__ __forceinline__ GlobalSrc<Rec, Load> global_src(Load l)
 {
     return GlobalSrc<Rec, Load>{l};
+}
+
+// Walk a workgroup's candidates from its LDS tile if the plan staged one,
+// else from global memory (same order, same results).  Every thread of the
+// workgroup calls this (the staging and its barrier need all of them);
+// `live` lanes walk.
+template <bool OWN, class Rec, class Load, class Test, class Body>
+__device__ __forceinline__ uint2 walk_group(const TilePlan &T, Rec *tile, Load load, bool live,
+                                            unsigned self, const sph_rows &R, unsigned key,
+                                            const sph_dev_params &P, Test &test, Body &body)
+{
+    uint2 fh = make_uint2(0u, 0u);
+    if (T.ok) {
+        stage_tile(T, tile, load);
+        if (live)
+            fh = sph_walk_masked<OWN>(self, R, lds_src(tile, T, key, P), test, body);
+    } else if (live) {
+        fh = sph_walk_masked<OWN>(self, R, global_src<Rec>(load), test, body);
+    }
+    return fh;
 }
 
 } // namespace
@@ -342,55 +362,46 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
     const float sdt = slosh ? sph_slosh_accel(P, *clock) * P.dt : 0.0f;
     // the lane's own particle and candidate rows, loaded before the planning
     // barriers
-    const float2 a0 = POS[s], av0 = VEL[s];
-    const unsigned key = VISC && s < s1 ? K[s] : P.cell_base; // pad lanes: any valid cell
+    const float2 a = POS[s], av = VEL[s];
+    const bool live = s < s1;
+    const unsigned key = VISC && live ? K[s] : P.cell_base; // pad lanes: any valid cell
     const sph_rows R = VISC ? sph_row_bounds(key, P, start) : sph_rows{};
-    if (VISC) {
-        plan_tile_par(s0, s1, K, start, P, T);
-        if (T.ok)
-            stage_tile(T, tile, [&](unsigned g) {
-                const float2 p = POS[g], v = VEL[g];
-                return make_float4(p.x, p.y, v.x, v.y);
-            });
-    }
+    const float avx = slosh ? av.x + sdt : av.x;
+    const float avy = GRAV ? av.y + gdt : av.y;
+    float nvx = avx, nvy = avy;
     unsigned full = 0, half = 0;
-    if (s < s1) {
-        const float2 a = a0;
-        const float2 av = av0;
-        const float avx = slosh ? av.x + sdt : av.x;
-        const float avy = GRAV ? av.y + gdt : av.y;
-        float nvx = avx, nvy = avy;
-        if (VISC) {
-            auto test = [&](unsigned k, float4 b) {
-                const float dx = b.x - a.x, ddy = b.y - a.y;
-                return !(dx * dx + ddy * ddy > P.h2);
-            };
-            auto body = [&](unsigned k, int dy, unsigned b1, unsigned b2, float4 b, bool use) {
-                half += use && sph_owns(k, s, dy, b1, b2);
-                const float bvx = slosh ? b.z + sdt : b.z;
-                const float bvy = GRAV ? b.w + gdt : b.w;
-                float ix, iy;
-                if (sph_visc_impulse(a.x, a.y, avx, avy, b.x, b.y, bvx, bvy, P, ix, iy) && use) {
-                    nvx -= ix * 0.5f;
-                    nvy -= iy * 0.5f;
-                }
-            };
-            if (T.ok)
-                full = sph_walk_masked(s, R, lds_src(tile, T, key, P), test, body);
-            else
-                full = sph_walk_masked(s, R, global_src<float4>([&](unsigned g) {
-                                           const float2 p = POS[g], v = VEL[g];
-                                           return make_float4(p.x, p.y, v.x, v.y);
-                                       }), test, body);
-        }
+    if (VISC) {
+        auto load = [&](unsigned g) {
+            const float2 p = POS[g], v = VEL[g];
+            return make_float4(p.x, p.y, v.x, v.y);
+        };
+        auto test = [&](float4 b) {
+            const float dx = b.x - a.x, ddy = b.y - a.y;
+            return !(dx * dx + ddy * ddy > P.h2);
+        };
+        auto body = [&](unsigned, bool, float4 b) {
+            const float bvx = slosh ? b.z + sdt : b.z;
+            const float bvy = GRAV ? b.w + gdt : b.w;
+            float ix, iy;
+            if (sph_visc_impulse(a.x, a.y, avx, avy, b.x, b.y, bvx, bvy, P, ix, iy)) {
+                nvx -= ix * 0.5f;
+                nvy -= iy * 0.5f;
+            }
+        };
+        plan_tile_par(s0, s1, K, start, P, T);
+        const uint2 fh = walk_group<false>(T, tile, load, live, s, R, key, P, test, body);
+        full = fh.x;
+        half = fh.y;
+    }
+    if (live) {
         if (PRED) {
             float x = a.x + nvx * P.dt;
             float y = a.y + nvy * P.dt;
             sph_boundary(x, y, P);
             out_pos[s] = make_float2(x, y);
-            const unsigned key = sph_cell_key(x, y, P, &C->key_out_of_range);
-            out_key[s] = key;
-            out_slot[s] = wave_slot(cnt, key);
+            const unsigned nk = sph_cell_key(x, y, P, &C->key_out_of_range);
+            out_key[s] = nk;
+            out_slot[s] = wave_slot(cnt, nk);
         } else {
             out_pos[s] = make_float2(nvx, nvy);
         }
@@ -636,38 +647,31 @@ k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     if (s0 >= s1 || P.frozen)
         return; // a padding workgroup, or a frozen scene
     // the lane's own particle and candidate rows, before the barriers
-    const float2 a0 = POS[s];
-    const unsigned key = s < s1 ? K[s] : P.cell_base; // pad lanes: any valid cell
+    const float2 a = POS[s];
+    const bool live = s < s1;
+    const unsigned key = live ? K[s] : P.cell_base; // pad lanes: any valid cell
     const sph_rows R = sph_row_bounds(key, P, start);
     plan_tile_par(s0, s1, K, start, P, T);
-    if (T.ok)
-        stage_tile(T, tile, [&](unsigned g) { return POS[g]; });
-    unsigned full = 0, half = 0;
-    if (s < s1) {
-        const float2 a = a0;
-        float rho = 0.0f, rhon = 0.0f;
-        auto test = [&](unsigned k, float2 b) {
-            const float dx = a.x - b.x, ddy = a.y - b.y;
-            return !(dx * dx + ddy * ddy > P.h2);
-        };
-        auto body = [&](unsigned k, int dy, unsigned b1, unsigned b2, float2 b, bool use) {
-            half += use && sph_owns(k, s, dy, b1, b2);
-            const float dx = a.x - b.x, ddy = a.y - b.y;
-            const float r2 = dx * dx + ddy * ddy;
-            const float ratio = sph_sqrt(r2) * P.h_recip;
-            const float OmR2 = (1.0f - ratio) * (1.0f - ratio);
-            if (ratio < 1.0f && use) {
-                rho += OmR2;
-                rhon += OmR2 * (1.0f - ratio);
-            }
-        };
-        if (T.ok)
-            full = sph_walk_masked<32>(s, R, lds_src(tile, T, key, P), test, body);
-        else
-            full = sph_walk_masked<32>(s, R, global_src<float2>([&](unsigned g) { return POS[g]; }),
-                                       test, body);
+    float rho = 0.0f, rhon = 0.0f;
+    auto load = [&](unsigned g) { return POS[g]; };
+    auto test = [&](float2 b) {
+        const float dx = a.x - b.x, ddy = a.y - b.y;
+        return !(dx * dx + ddy * ddy > P.h2);
+    };
+    auto body = [&](unsigned, bool, float2 b) {
+        const float dx = a.x - b.x, ddy = a.y - b.y;
+        const float r2 = dx * dx + ddy * ddy;
+        const float ratio = sph_sqrt(r2) * P.h_recip;
+        const float OmR2 = (1.0f - ratio) * (1.0f - ratio);
+        if (ratio < 1.0f) {
+            rho += OmR2;
+            rhon += OmR2 * (1.0f - ratio);
+        }
+    };
+    const uint2 fh = walk_group<false>(T, tile, load, live, s, R, key, P, test, body);
+    if (live)
         RHO[s] = make_float2(rho, rhon);
-    }
+    const unsigned full = fh.x, half = fh.y;
     report_neighbors(full, half, C);
 }
 
@@ -711,39 +715,37 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
         }
         return;
     }
-    // the lane's own particle, loaded before the planning barriers
+    // the lane's own particle and candidate rows, loaded before the planning
+    // barriers
     const float2 a = POS[s];
     const float2 ar = RHO[s];
-    const unsigned key = s < s1 ? K[s] : P.cell_base; // pad lanes: any valid cell
+    const bool live = s < s1;
+    const unsigned key = live ? K[s] : P.cell_base; // pad lanes: any valid cell
     const sph_rows R = sph_row_bounds(key, P, start);
     plan_tile_par(s0, s1, K, start, P, T);
     auto load = [&](unsigned g) {
         const float2 p = POS[g], r = RHO[g];
         return make_float4(p.x, p.y, r.x, r.y);
     };
-    if (T.ok)
-        stage_tile(T, tile, load);
-    if (s >= s1)
-        return;
     const float aP = P.k * (ar.x - P.rest_density);
     const float aPn = P.k_near * ar.y;
     float nx = a.x, ny = a.y;
-    auto test = [&](unsigned k, float4 b) {
+    auto test = [&](float4 b) {
         const float dx = a.x - b.x, ddy = a.y - b.y;
         return !(dx * dx + ddy * ddy > P.h2);
     };
-    auto body = [&](unsigned k, int dy, unsigned b1, unsigned b2, float4 b, bool use) {
+    auto body = [&](unsigned, bool own, float4 b) {
         const float dx = a.x - b.x, ddy = a.y - b.y;
         const float r2 = dx * dx + ddy * ddy;
         const float rr = sph_sqrt(r2);
         const float r_recip = 1.0f / rr;
         const float ratio = rr * P.h_recip;
         const float OmR = 1.0f - ratio;
-        if (use && rr <= 0.000001f && sph_owns(k, s, dy, b1, b2)) {
+        if (rr <= 0.000001f && own) {
             nx += 0.000001f;
             ny += 0.000001f;
         }
-        if (use && ratio < 1.0f && rr > 0.0f) {
+        if (ratio < 1.0f && rr > 0.0f) {
             const float bP = P.k * (b.z - P.rest_density);
             const float bPn = P.k_near * b.w;
             const float D = sph_relax_D(P.dt, aP + bP, aPn + bPn, OmR, P.k_spring, P.h, rr);
@@ -753,10 +755,9 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
             ny -= Dy;
         }
     };
-    if (T.ok)
-        sph_walk_masked(s, R, lds_src(tile, T, key, P), test, body);
-    else
-        sph_walk_masked(s, R, global_src<float4>(load), test, body);
+    walk_group<true>(T, tile, load, live, s, R, key, P, test, body);
+    if (!live)
+        return;
     if (FUSED) {
         sph_boundary(nx, ny, P);
         const float2 q = PREV[s];

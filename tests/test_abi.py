@@ -73,8 +73,8 @@ def test_every_tuning_variant_exports_the_abi():
     kernels) are drop-in replacements: same exported entry points."""
     from sph_amd import build as bd
     declared = declared_functions()
-    for cap, ilp in bd.VARIANTS:
-        path = os.path.join(bd.VARIANT_DIR, bd.variant_name(cap, ilp))
+    for cap, run in bd.VARIANTS + bd.CHECK_VARIANTS:
+        path = os.path.join(bd.VARIANT_DIR, bd.variant_name(cap, run))
         assert os.path.exists(path), f"{path}: build the variants first"
         exported = exported_symbols(path)
         assert not [n for n in declared if n not in exported], path
@@ -86,8 +86,8 @@ def test_tuned_library_choice(tmp_path):
     t = tmp_path / "tuned.json"
     assert sa.tuned_library(str(t)) is None
     t.write_text('{"default": "dam_break_1m", "workloads": '
-                 '{"dam_break_1m": {"best": "libsph_hip_t2048_i1.so"}}}')
-    assert sa.tuned_library(str(t)).endswith("variants/libsph_hip_t2048_i1.so")
+                 '{"dam_break_1m": {"best": "libsph_hip_t2048_x16.so"}}}')
+    assert sa.tuned_library(str(t)).endswith("variants/libsph_hip_t2048_x16.so")
     t.write_text('{"default": "dam_break_1m", "workloads": {"dam_break_1m": {"best": "no_such.so"}}}')
     assert sa.tuned_library(str(t)) is None
 

@@ -1,0 +1,81 @@
+"""Dense tiles: the gather kernels' two candidate paths on one scene.
+
+A workgroup stages the candidate rows of its 256 sorted particles in LDS
+when they fit the library's tile (1,024-2,048 records by build); a workgroup
+whose tile would not fit, or that spans more than two cell rows, walks the
+same candidates in global memory in the same order.  A dense lattice
+(spacing 0.15h, about 44 particles per cell) makes tiles of 1,050-1,220
+records, and one workgroup spans several rows; it is compared bit for bit
+with the oracle.  The same scene then runs on the test build with
+256-record tiles (nearly every workgroup on the global-memory walk, XCD runs
+of one tile) in a child process, and must equal the default library bit for
+bit.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import sph_amd as sa
+from oracle import pyoracle as po
+from tests.common import assert_bitexact, dam_break_tunables
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SMALL = os.path.join(sa.PKG_DIR, "variants", "libsph_hip_t256_x1.so")
+
+CHILD = r"""
+import sys
+import numpy as np
+import sph_amd as sa
+d = np.load(sys.argv[1])
+parts = d["parts"]
+s = sa.Solver(sa.Tunables.from_buffer_copy(d["tun"].tobytes()), sa.make_box(60, 40), len(parts))
+s.upload(parts)
+s.step(int(sys.argv[3]))
+np.save(sys.argv[2], s.download())
+"""
+
+
+def dense_case():
+    parts = sa.scene_lattice(128, 128, 0.15, 0.01, seed=7)
+    return parts, dam_break_tunables(), po.make_box(60, 40)
+
+
+def test_dense_tiles_bitexact_against_oracle():
+    parts, tun, box = dense_case()
+    s = sa.Solver(sa.Tunables.from_buffer_copy(bytes(tun)), sa.AABB.from_buffer_copy(bytes(box)),
+                  len(parts))
+    s.upload(parts)
+    g = po.GatherSim(box, tun, len(parts))
+    g.set_state(parts.view(po.P32))
+    for step in range(1, 4):
+        s.step(1, graph=False)
+        g.step(1)
+        assert_bitexact(s.download(), g.get_state(), what=f"dense step {step}")
+    c, oc = s.counters(), g.counters()
+    assert c["max_neighbors"] == oc["max_neighbors"]
+    assert c["neighbor_overflow"] == oc["neighbor_overflow"]
+
+
+def test_small_window_build_equals_default(tmp_path):
+    assert os.path.exists(SMALL), "build the variants first (python -m sph_amd.build)"
+    parts, tun, box = dense_case()
+    steps = 6
+    s = sa.Solver(sa.Tunables.from_buffer_copy(bytes(tun)), sa.AABB.from_buffer_copy(bytes(box)),
+                  len(parts))
+    s.upload(parts)
+    s.step(steps)
+    want = s.download()
+    src, out = tmp_path / "in.npz", tmp_path / "out.npy"
+    np.savez(src, parts=parts, tun=np.frombuffer(bytes(tun), np.uint8))
+    env = dict(os.environ, SPH_AMD_LIB=SMALL)
+    print("child: 256-record tiles,", len(parts), "particles,", steps, "steps", flush=True)
+    r = subprocess.run([sys.executable, "-c", CHILD, str(src), str(out), str(steps)], cwd=ROOT,
+                       env=env, capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0, r.stderr[-2000:]
+    got = np.load(out)
+    assert_bitexact(got, want, what="256-record tiles vs default")
