@@ -334,6 +334,11 @@ int sph_scene_lattice(fluid_particle *out, int nx, int ny, float x0, float y0, f
  * whose sqrt differs from IEEE sqrtf (all NaNs count as equal). */
 int sph_selftest_sqrt(unsigned lo_bits, unsigned hi_bits, unsigned long long *mismatches);
 
+/* The same for the kernels' correctly rounded reciprocal against IEEE
+ * division 1.0f / x over its domain (+-0, +-inf, NaN and 2^-126 <= |x| <=
+ * 2^125; patterns outside it are skipped); no counterpart in the reference. */
+int sph_selftest_rcp(unsigned lo_bits, unsigned hi_bits, unsigned long long *mismatches);
+
 /* ------------------------------------------------------------------------
  * 3. Reference-named entry points (drop-in for src/fluid.h:112-126 and
  *    src/hash.h:50-52).  They drive one process-global context that is

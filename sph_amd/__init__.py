@@ -173,6 +173,8 @@ def lib():
             "sph_trajectory_create": (ip, [ctypes.POINTER(vp), ip, ip, ip]),
             "sph_selftest_sqrt": (ip, [ctypes.c_uint, ctypes.c_uint,
                                        ctypes.POINTER(ctypes.c_uint64)]),
+            "sph_selftest_rcp": (ip, [ctypes.c_uint, ctypes.c_uint,
+                                       ctypes.POINTER(ctypes.c_uint64)]),
             "sph_trajectory_destroy": (None, [vp]),
             "sph_trajectory_record": (ip, [vp, vp, ip]),
             "sph_trajectory_read": (ip, [vp, ip, vp]),
@@ -508,6 +510,14 @@ def selftest_sqrt(lo_bits: int, hi_bits: int) -> int:
     float bit patterns [lo_bits, hi_bits)."""
     bad = ctypes.c_uint64()
     _check(lib().sph_selftest_sqrt(lo_bits, hi_bits, ctypes.byref(bad)), "sph_selftest_sqrt")
+    return int(bad.value)
+
+
+def selftest_rcp(lo_bits: int, hi_bits: int) -> int:
+    """Mismatches of the kernels' reciprocal against IEEE 1.0f / x over the
+    float bit patterns [lo_bits, hi_bits)."""
+    bad = ctypes.c_uint64()
+    _check(lib().sph_selftest_rcp(lo_bits, hi_bits, ctypes.byref(bad)), "sph_selftest_rcp")
     return int(bad.value)
 
 

@@ -249,8 +249,8 @@ int dalloc(T **p, size_t count)
 int check_scene_desc(const sph_scene_desc &d, int idx)
 {
     const float h = d.tunables.smoothing_radius;
-    if (!(h > 0.0f))
-        return set_err(SPH_EINVAL, "scene %d: smoothing_radius must be > 0", idx);
+    if (!(h > 0.0f && h <= 0x1p60f)) // the kernels' reciprocal needs r <= h <= 2^60
+        return set_err(SPH_EINVAL, "scene %d: smoothing_radius must be in (0, 2^60]", idx);
     if (d.boundary.min_x != 0.0f || d.boundary.min_y != 0.0f)
         return set_err(SPH_EINVAL, "scene %d: the box must start at the origin", idx);
     if (!(d.boundary.max_x > 0.0f && d.boundary.max_y > 0.0f))
@@ -1122,10 +1122,11 @@ int sph_time_kernels(sph_ctx *c, int nsteps, int max_kinds, const char **names, 
     return SPH_OK;
 }
 
-int sph_selftest_sqrt(unsigned lo_bits, unsigned hi_bits, unsigned long long *mismatches)
+static int selftest_math(int fn, const char *name, unsigned lo_bits, unsigned hi_bits,
+                         unsigned long long *mismatches)
 {
     if (!mismatches || hi_bits < lo_bits)
-        return set_err(SPH_EINVAL, "sph_selftest_sqrt: bad argument");
+        return set_err(SPH_EINVAL, "%s: bad argument", name);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return set_err(SPH_ENODEV, "no HIP device available");
@@ -1133,7 +1134,7 @@ int sph_selftest_sqrt(unsigned lo_bits, unsigned hi_bits, unsigned long long *mi
     HIPCHK(hipMalloc(&d, sizeof *d));
     hipError_t e = hipMemset(d, 0, sizeof *d);
     if (e == hipSuccess) {
-        sph_launch_selftest_sqrt(nullptr, lo_bits, hi_bits, d);
+        sph_launch_selftest_math(nullptr, fn, lo_bits, hi_bits, d);
         e = hipGetLastError();
     }
     if (e == hipSuccess)
@@ -1142,8 +1143,18 @@ int sph_selftest_sqrt(unsigned lo_bits, unsigned hi_bits, unsigned long long *mi
         e = hipMemcpy(mismatches, d, sizeof *d, hipMemcpyDeviceToHost);
     (void)hipFree(d);
     if (e != hipSuccess)
-        return set_err(SPH_EHIP, "sph_selftest_sqrt: %s", hipGetErrorString(e));
+        return set_err(SPH_EHIP, "%s: %s", name, hipGetErrorString(e));
     return SPH_OK;
+}
+
+int sph_selftest_sqrt(unsigned lo_bits, unsigned hi_bits, unsigned long long *mismatches)
+{
+    return selftest_math(0, "sph_selftest_sqrt", lo_bits, hi_bits, mismatches);
+}
+
+int sph_selftest_rcp(unsigned lo_bits, unsigned hi_bits, unsigned long long *mismatches)
+{
+    return selftest_math(1, "sph_selftest_rcp", lo_bits, hi_bits, mismatches);
 }
 
 } // extern "C"

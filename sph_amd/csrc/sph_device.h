@@ -77,41 +77,66 @@ __device__ __forceinline__ unsigned sph_cell_key(float x, float y, const sph_dev
     return P.cell_base + (unsigned)k;
 }
 
-// Correctly rounded square root.  For 2^-96 <= x < inf, v_sqrt_f32 is
-// within one ulp of sqrt(x); the rounded result is the candidate among
-// s - ulp, s, s + ulp whose square brackets x: s - ulp when
+// Correctly rounded square root, branch-free.  For 2^-96 <= x < inf,
+// v_sqrt_f32 is within one ulp of sqrt(x); the rounded result is the
+// candidate among s - ulp, s, s + ulp whose square brackets x: s - ulp when
 // x - (s - ulp)*s <= 0, s + ulp when x - (s + ulp)*s > 0 (each residual
-// formed with one rounding by an FMA), else s.  Smaller, zero and
-// non-finite inputs take the general sqrtf path.  Results equal sqrtf's
-// (IEEE round-to-nearest) for every input: checked exhaustively over all
-// floats (sph_selftest_sqrt).
+// formed with one rounding by an FMA), else s.  Smaller inputs (denormals
+// included) are scaled by 2^64 into that range and the root by 2^-32, both
+// exactly; zero, negative numbers, infinity and NaN pass through the same
+// arithmetic with the right result (the residuals are NaN or zero there).
+// Results equal sqrtf's (IEEE round-to-nearest) for every input: checked
+// exhaustively over all floats (sph_selftest_sqrt).
 __device__ __forceinline__ float sph_sqrt(float x)
 {
-    if (!(x >= 0x1p-96f && x < __builtin_inff()))
-        return sqrtf(x);
-    const float s = __builtin_amdgcn_sqrtf(x);
+    const bool tiny = x < 0x1p-96f;
+    const float xs = tiny ? x * 0x1p64f : x;
+    const float s = __builtin_amdgcn_sqrtf(xs);
     const float sd = __int_as_float(__float_as_int(s) - 1);
     const float su = __int_as_float(__float_as_int(s) + 1);
     float r = s;
-    if (__builtin_fmaf(-sd, s, x) <= 0.0f)
-        r = sd;
-    if (__builtin_fmaf(-su, s, x) > 0.0f)
-        r = su;
-    return r;
+    r = __builtin_fmaf(-sd, s, xs) <= 0.0f ? sd : r;
+    r = __builtin_fmaf(-su, s, xs) > 0.0f ? su : r;
+    return tiny ? r * 0x1p-32f : r;
 }
 
-// src/fluid.c:613-625
-__device__ __forceinline__ void sph_check_velocity(float &vx, float &vy)
+// Correctly rounded 1/x, branch-free, for x = +-0, +-inf, NaN and
+// 2^-126 <= |x| <= 2^125.  There v_rcp_f32 gives y within one ulp of 1/x;
+// the residual e = 1 - x*y is formed with one rounding by an FMA and the
+// corrected y + e*y rounded once (a second FMA) is the correctly rounded
+// reciprocal.  e is NaN exactly for zero, infinite and NaN x, where y itself
+// is 1/x.  Every r the pair terms divide by lies in that domain: r =
+// sqrt(r2) is 0 or at least 2^-74.5 and at most h, which the host bounds
+// by 2^60.  Results equal 1.0f / x over the whole domain: checked
+// exhaustively (sph_selftest_rcp).
+__device__ __forceinline__ float sph_rcp(float x)
+{
+    const float y = __builtin_amdgcn_rcpf(x);
+    const float e = __builtin_fmaf(-x, y, 1.0f);
+    const float r = __builtin_fmaf(e, y, y);
+    return e == e ? r : y;
+}
+
+// The domain of sph_rcp.
+__device__ __forceinline__ bool sph_rcp_domain(float x)
+{
+    const float ax = __builtin_fabsf(x);
+    return !(ax > 0.0f && ax < 0x1p-126f) && !(ax > 0x1p125f && ax < __builtin_inff());
+}
+
+// src/fluid.c:613-625 without branches: the median of (v, -5, 5) is v
+// clamped to [-5, 5] (v_med3_f32); a NaN passes through unchanged, as in
+// the reference's if / else if.
+__device__ __forceinline__ float sph_clamp_v(float v)
 {
     const float v_max = 5.0f;
-    if (vx > v_max)
-        vx = v_max;
-    else if (vx < -v_max)
-        vx = -v_max;
-    if (vy > v_max)
-        vy = v_max;
-    else if (vy < -v_max)
-        vy = -v_max;
+    return v != v ? v : __builtin_amdgcn_fmed3f(v, -v_max, v_max);
+}
+
+__device__ __forceinline__ void sph_check_velocity(float &vx, float &vy)
+{
+    vx = sph_clamp_v(vx);
+    vy = sph_clamp_v(vy);
 }
 
 // src/fluid.c:656-744: mover push-out then clamp into the box.
@@ -174,6 +199,8 @@ __device__ __forceinline__ float sph_relax_D(float dt, float Psum, float Pnsum, 
 // src/fluid.c:442-459: impulse of pair (p, q) with p in the owner role.
 // Exactly antisymmetric in (p, q), so a particle in either role subtracts
 // half of the impulse computed with itself as p.
+// Evaluated unconditionally and returned with the test u > 0, so that the
+// caller applies it with selects (no branch in the pair loop).
 __device__ __forceinline__ bool sph_visc_impulse(float px, float py, float pvx, float pvy, float qx,
                                                  float qy, float qvx, float qvy,
                                                  const sph_dev_params &P, float &ix, float &iy)
@@ -181,17 +208,14 @@ __device__ __forceinline__ bool sph_visc_impulse(float px, float py, float pvx, 
     float QmP_x = qx - px;
     float QmP_y = qy - py;
     float r = sph_sqrt(QmP_x * QmP_x + QmP_y * QmP_y);
-    float r_recip = 1.0f / r;
+    float r_recip = sph_rcp(r); // 1.0f / r
     float ratio = r * P.h_recip;
     float u = ((pvx - qvx) * QmP_x + (pvy - qvy) * QmP_y) * r_recip;
-    if (u > 0.0f) {
-        float imp = P.dt * (1.0f - ratio) * (P.sigma * u + P.beta * u * u);
-        ix = imp * QmP_x * r_recip;
-        iy = imp * QmP_y * r_recip;
-        sph_check_velocity(ix, iy);
-        return true;
-    }
-    return false;
+    float imp = P.dt * (1.0f - ratio) * (P.sigma * u + P.beta * u * u);
+    ix = imp * QmP_x * r_recip;
+    iy = imp * QmP_y * r_recip;
+    sph_check_velocity(ix, iy);
+    return u > 0.0f;
 }
 
 // Half-list owner rule of src/hash.c:176-237 for the pair (self at sorted
@@ -306,21 +330,26 @@ __device__ __forceinline__ uint2 sph_walk_masked(unsigned self, const sph_rows &
     unsigned nq = 0;
     unsigned full = 0, half = 0;
     auto drain = [&]() {
-        while (m0) {
-            const unsigned t = (unsigned)__builtin_ctz(m0);
-            m0 &= m0 - 1u;
-            const unsigned idx = x0 + t;
-            body(idx, OWN && idx >= t0, src.at(idx));
-            if (m0 == 0) {
-                m0 = m1;
-                x0 = x1;
-                t0 = t1;
-                m1 = m2;
-                x1 = x2;
-                t1 = t2;
-                m2 = 0;
-            }
+        // the queue in locals: selects between captured variables can be
+        // folded into a load from a selected address, which puts the queue
+        // in scratch memory
+        unsigned q0 = m0, q1 = m1, q2 = m2, y0 = x0, y1 = x1, y2 = x2, u0 = t0, u1 = t1,
+                 u2 = t2;
+        while (q0) {
+            const unsigned t = (unsigned)__builtin_ctz(q0);
+            q0 &= q0 - 1u;
+            const unsigned idx = y0 + t;
+            body(idx, OWN && idx >= u0, src.at(idx));
+            const bool next = q0 == 0; // chunk done: the queue moves up
+            q0 = next ? q1 : q0;
+            y0 = next ? y1 : y0;
+            u0 = next ? u1 : u0;
+            q1 = next ? q2 : q1;
+            y1 = next ? y2 : y1;
+            u1 = next ? u2 : u1;
+            q2 = next ? 0u : q2;
         }
+        m0 = m1 = m2 = 0u;
         nq = 0;
     };
 #pragma unroll

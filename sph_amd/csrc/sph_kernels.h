@@ -94,4 +94,5 @@ void sph_set_next_launch_events(hipEvent_t start, hipEvent_t stop);
 
 // Exhaustive bit-for-bit check of sph_sqrt against sqrtf over float bit
 // patterns [lo, hi).
-void sph_launch_selftest_sqrt(hipStream_t st, unsigned lo, unsigned hi, unsigned long long *bad);
+void sph_launch_selftest_math(hipStream_t st, int fn, unsigned lo, unsigned hi,
+                              unsigned long long *bad);

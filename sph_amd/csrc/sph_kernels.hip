@@ -371,22 +371,22 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
     float nvx = avx, nvy = avy;
     unsigned full = 0, half = 0;
     if (VISC) {
+        // a neighbour's record carries its velocity after gravity and
+        // sloshing (the same expressions as the particle's own), applied
+        // once per record rather than once per pair
         auto load = [&](unsigned g) {
             const float2 p = POS[g], v = VEL[g];
-            return make_float4(p.x, p.y, v.x, v.y);
+            return make_float4(p.x, p.y, slosh ? v.x + sdt : v.x, GRAV ? v.y + gdt : v.y);
         };
         auto test = [&](float4 b) {
             const float dx = b.x - a.x, ddy = b.y - a.y;
             return !(dx * dx + ddy * ddy > P.h2);
         };
         auto body = [&](unsigned, bool, float4 b) {
-            const float bvx = slosh ? b.z + sdt : b.z;
-            const float bvy = GRAV ? b.w + gdt : b.w;
             float ix, iy;
-            if (sph_visc_impulse(a.x, a.y, avx, avy, b.x, b.y, bvx, bvy, P, ix, iy)) {
-                nvx -= ix * 0.5f;
-                nvy -= iy * 0.5f;
-            }
+            const bool push = sph_visc_impulse(a.x, a.y, avx, avy, b.x, b.y, b.z, b.w, P, ix, iy);
+            nvx = push ? nvx - ix * 0.5f : nvx;
+            nvy = push ? nvy - iy * 0.5f : nvy;
         };
         plan_tile_par(s0, s1, K, start, P, T);
         const uint2 fh = walk_group<false>(T, tile, load, live, s, R, key, P, test, body);
@@ -663,10 +663,9 @@ k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
         const float r2 = dx * dx + ddy * ddy;
         const float ratio = sph_sqrt(r2) * P.h_recip;
         const float OmR2 = (1.0f - ratio) * (1.0f - ratio);
-        if (ratio < 1.0f) {
-            rho += OmR2;
-            rhon += OmR2 * (1.0f - ratio);
-        }
+        const bool in = ratio < 1.0f;
+        rho = in ? rho + OmR2 : rho;
+        rhon = in ? rhon + OmR2 * (1.0f - ratio) : rhon;
     };
     const uint2 fh = walk_group<false>(T, tile, load, live, s, R, key, P, test, body);
     if (live)
@@ -723,9 +722,11 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     const unsigned key = live ? K[s] : P.cell_base; // pad lanes: any valid cell
     const sph_rows R = sph_row_bounds(key, P, start);
     plan_tile_par(s0, s1, K, start, P, T);
+    // a neighbour's record carries its pressure and near-pressure
+    // (src/fluid.c:563-564), evaluated once per record rather than per pair
     auto load = [&](unsigned g) {
         const float2 p = POS[g], r = RHO[g];
-        return make_float4(p.x, p.y, r.x, r.y);
+        return make_float4(p.x, p.y, P.k * (r.x - P.rest_density), P.k_near * r.y);
     };
     const float aP = P.k * (ar.x - P.rest_density);
     const float aPn = P.k_near * ar.y;
@@ -738,22 +739,20 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
         const float dx = a.x - b.x, ddy = a.y - b.y;
         const float r2 = dx * dx + ddy * ddy;
         const float rr = sph_sqrt(r2);
-        const float r_recip = 1.0f / rr;
+        const float r_recip = sph_rcp(rr); // 1.0f / rr
         const float ratio = rr * P.h_recip;
         const float OmR = 1.0f - ratio;
-        if (rr <= 0.000001f && own) {
-            nx += 0.000001f;
-            ny += 0.000001f;
-        }
-        if (ratio < 1.0f && rr > 0.0f) {
-            const float bP = P.k * (b.z - P.rest_density);
-            const float bPn = P.k_near * b.w;
-            const float D = sph_relax_D(P.dt, aP + bP, aPn + bPn, OmR, P.k_spring, P.h, rr);
-            const float Dx = D * (b.x - a.x) * r_recip;
-            const float Dy = D * (b.y - a.y) * r_recip;
-            nx -= Dx;
-            ny -= Dy;
-        }
+        // both updates as selects (no branch in the pair loop)
+        const bool nudge = rr <= 0.000001f && own;
+        nx = nudge ? nx + 0.000001f : nx;
+        ny = nudge ? ny + 0.000001f : ny;
+        const float D = sph_relax_D(P.dt, aP + b.z, aPn + b.w, OmR, P.k_spring, P.h, rr);
+        // b - a = -(a - b) exactly in IEEE arithmetic
+        const float Dx = D * -dx * r_recip;
+        const float Dy = D * -ddy * r_recip;
+        const bool push = ratio < 1.0f && rr > 0.0f;
+        nx = push ? nx - Dx : nx;
+        ny = push ? ny - Dy : ny;
     };
     walk_group<true>(T, tile, load, live, s, R, key, P, test, body);
     if (!live)
@@ -879,19 +878,26 @@ void launch(F kernel, dim3 grid, dim3 block, hipStream_t st, Args... args)
 
 } // namespace
 
-// Exhaustive check of sph_sqrt: every float with bit pattern in [lo, hi)
-// (grid-stride) against sqrtf, bit for bit (any NaN equals any NaN).
+// Exhaustive checks of sph_sqrt (FN 0) against sqrtf and sph_rcp (FN 1)
+// against IEEE division: every float with bit pattern in [lo, hi)
+// (grid-stride; for sph_rcp, every one in its domain), bit for bit (any NaN
+// equals any NaN).
 __device__ __attribute__((noinline)) float ieee_sqrt(float x) { return sqrtf(x); }
+__device__ __attribute__((noinline)) float ieee_rcp(float x) { return 1.0f / x; }
 
+template <int FN>
 __global__ void __launch_bounds__(256)
-k_selftest_sqrt(unsigned lo, unsigned hi, unsigned long long *__restrict__ bad)
+k_selftest_math(unsigned lo, unsigned hi, unsigned long long *__restrict__ bad)
 {
     unsigned long long nbad = 0;
     const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
     for (unsigned long long b = lo + (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
          b < hi; b += stride) {
         const float x = __uint_as_float((unsigned)b);
-        const float got = sph_sqrt(x), want = ieee_sqrt(x);
+        if (FN == 1 && !sph_rcp_domain(x))
+            continue;
+        const float got = FN == 0 ? sph_sqrt(x) : sph_rcp(x);
+        const float want = FN == 0 ? ieee_sqrt(x) : ieee_rcp(x);
         const bool same = __float_as_uint(got) == __float_as_uint(want) ||
                           (got != got && want != want);
         nbad += same ? 0u : 1u;
@@ -902,9 +908,13 @@ k_selftest_sqrt(unsigned lo, unsigned hi, unsigned long long *__restrict__ bad)
         atomicAdd(bad, nbad);
 }
 
-void sph_launch_selftest_sqrt(hipStream_t st, unsigned lo, unsigned hi, unsigned long long *bad)
+void sph_launch_selftest_math(hipStream_t st, int fn, unsigned lo, unsigned hi,
+                              unsigned long long *bad)
 {
-    launch(k_selftest_sqrt, dim3(4096), dim3(256), st, lo, hi, bad);
+    if (fn == 0)
+        launch(k_selftest_math<0>, dim3(4096), dim3(256), st, lo, hi, bad);
+    else
+        launch(k_selftest_math<1>, dim3(4096), dim3(256), st, lo, hi, bad);
 }
 
 void sph_set_next_launch_events(hipEvent_t start, hipEvent_t stop)
