@@ -65,8 +65,21 @@ def workload(n_side=1024, box=(1200.0, 800.0), seed=12345):
     return tun, sa.make_box(*box), parts
 
 
-def cpu_baseline(tun, box, parts, budget_s=15.0):
-    """The reference's own step (oracle/_ref, -O3 -ffast-math), 1 core."""
+def cpu_model():
+    """The host CPU's model name (Linux), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(tun, box, parts, budget_s=30.0, min_steps=50):
+    """The reference's own step (oracle/_ref, -O3 -ffast-math), 1 core, over
+    at least min_steps steady steps (more if they fit in budget_s)."""
     from oracle import pyoracle as po
     fast = po.have_compiled_ref(fast=True)
     if fast:
@@ -83,11 +96,12 @@ def cpu_baseline(tun, box, parts, budget_s=15.0):
     t0 = time.perf_counter()
     step(1)
     per = time.perf_counter() - t0
-    k = max(1, min(200, int(budget_s / max(per, 1e-6))))
+    k = max(min_steps, min(200, int(budget_s / max(per, 1e-6))))
     t0 = time.perf_counter()
     step(k)
     dt = time.perf_counter() - t0
     return {"value": len(parts) * k / dt, "unit": "particle-steps/s", "cores": 1, "kind": kind,
+            "cpu": cpu_model(),
             "sample": f"{k} steady steps (after 2 warm-up) of the same N={len(parts)} dam break, "
                       f"{dt:.1f} s on one host core ({os.cpu_count()} visible)"}
 
@@ -124,6 +138,7 @@ def cpu_baseline_batched(cores, budget_s=10.0):
     value = sum(n * k / el for n, k, el in res)
     parts = sum(n for n, _, _ in res)
     return {"value": value, "unit": "particle-steps/s", "cores": cores, "kind": "reference",
+            "cpu": cpu_model(),
             "sample": f"{cores} of the 256 scenes ({parts} particles, every {256 // cores}th), one "
                       f"per core, {budget_s:.0f} s each of steady steps after one warm-up step; "
                       f"throughput summed over the concurrent workers"}
