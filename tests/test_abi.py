@@ -214,3 +214,30 @@ def test_driver_builds():
     if exe is None:
         pytest.skip("driver source absent")
     assert os.access(exe, os.X_OK)
+
+
+def test_gather_kernels_use_no_scratch_memory():
+    """The gather kernels keep their walk state (the accepted-chunk queue,
+    the row bounds, the staged records) in registers and LDS.  A queue
+    written as selects between captured variables was once folded into
+    loads from a selected address: scratch memory, and twice the
+    relaxation kernel's time.  hipcc's resource report must show no scratch
+    for any instance of the three gather kernels."""
+    from sph_amd import build as bd
+    src = os.path.join(bd.CSRC, "sph_kernels.hip")
+    r = subprocess.run([bd.HIPCC, *bd.HIP_FLAGS, "--cuda-device-only", "-c", src, "-o", os.devnull,
+                        "-Rpass-analysis=kernel-resource-usage"],
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    name, scratch = None, {}
+    for line in r.stderr.splitlines():
+        m = re.search(r"Function Name: (\S+)", line)
+        if m:
+            name = m.group(1)
+        m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", line)
+        if m and name:
+            scratch[name] = int(m.group(1))
+    hot = {k: v for k, v in scratch.items()
+           if any(t in k for t in ("k_visc_predict", "k_density", "k_relax"))}
+    assert len(hot) >= 4, sorted(scratch)
+    assert all(v == 0 for v in hot.values()), hot
