@@ -170,12 +170,14 @@ void enqueue_fused_step(sph_ctx *c, bool visc)
     }
     {
         Tick t(c, KDE);
-        sph_launch_density(c->st, S, c->dP, c->BS, c->POS1, c->K1, c->start, c->RHO, c->dC);
+        sph_launch_density(c->st, S, c->dP, c->BS, c->POS1, c->K1, c->start, c->RHO, c->MASK,
+                           c->dC);
     }
     {
         Tick t(c, KRE);
         sph_launch_relax(c->st, S, true, c->dP, c->BS, c->POS1, c->PREV, c->RHO, c->K1, c->start,
-                         c->UPOS, c->UVEL, c->ukey, c->uslot, c->cnt, c->dC, c->dclock);
+                         c->UPOS, c->UVEL, c->ukey, c->uslot, c->cnt, c->MASK, c->dC,
+                         c->dclock);
     }
     enqueue_sort_tail(c, c->ID1);
     {
@@ -330,7 +332,7 @@ int sph_create_batch(sph_ctx **out, const sph_scene_desc *scenes, int nscenes)
     A(ID0, N); A(ID1, N); A(perm_src, N); A(perm_id, N); A(pos_of_id, N);
     A(K0, N); A(K1, N); A(ukey, N); A(uslot, N);
     A(cnt, M); A(start, M); A(bsum, sph_scan_blocks((unsigned)M));
-    A(coords, 2 * N); A(dP, (size_t)nscenes); A(BS, N / kSphTPB); A(dC, 1); A(dclock, 1);
+    A(coords, 2 * N); A(MASK, N); A(dP, (size_t)nscenes); A(BS, N / kSphTPB); A(dC, 1); A(dclock, 1);
 #undef A
     if (!rc) {
         hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
@@ -389,7 +391,7 @@ void sph_destroy(sph_ctx *c)
                     c->perm_id, c->pos_of_id, c->K0, c->K1,    c->ukey,      c->uslot,
                     c->cnt,  c->start, c->bsum,    c->coords,  c->dP,        c->BS,
                     c->dC,   c->dclock, c->SPOS,    c->SVEL,    c->SPREV,     c->SRHO,
-                    c->SID,  c->SK,    c->Sstart,  c->Sclock,  c->dflags};
+                    c->SID,  c->SK,    c->Sstart,  c->Sclock,  c->dflags, c->MASK};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -724,7 +726,8 @@ int sph_phase_hash(sph_ctx *c, int compute_density)
     HIPCHK(hipMemcpyAsync(c->K0, c->K1, sizeof(unsigned) * (size_t)S, hipMemcpyDeviceToDevice,
                           c->st));
     if (compute_density)
-        sph_launch_density(c->st, S, c->dP, c->BS, c->POS0, c->K0, c->start, c->RHO, c->dC);
+        sph_launch_density(c->st, S, c->dP, c->BS, c->POS0, c->K0, c->start, c->RHO, c->MASK,
+                           c->dC);
     HIPCHK(hipGetLastError());
     c->lists_valid = true;
     c->hashes++;
@@ -742,7 +745,8 @@ int sph_phase_relax(sph_ctx *c)
     if (!c->lists_valid)
         return set_err(SPH_ESTATE, "sph_phase_relax: no hash since upload");
     sph_launch_relax(c->st, c->slots, false, c->dP, c->BS, c->POS0, c->PREV, c->RHO, c->K0,
-                     c->start, c->UPOS, nullptr, nullptr, nullptr, nullptr, c->dC, c->dclock);
+                     c->start, c->UPOS, nullptr, nullptr, nullptr, nullptr, nullptr, c->dC,
+                     c->dclock);
     HIPCHK(hipMemcpyAsync(c->POS0, c->UPOS, sizeof(float2) * (size_t)c->slots,
                           hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipGetLastError());

@@ -46,6 +46,7 @@ struct sph_ctx {
     unsigned *K0 = nullptr, *K1 = nullptr, *ukey = nullptr, *uslot = nullptr, *cnt = nullptr,
              *start = nullptr, *bsum = nullptr;
     short *coords = nullptr;
+    uint4 *MASK = nullptr; // per slot: the density pass's first accepted-candidate masks
     std::vector<float> ax_by_id, ay_by_id; // a_x/a_y: carried, never changed (src/fluid.c)
     bool lists_valid = false; // a hash ran since upload: K0/start describe POS0
     int aux_order = 0;        // PREV, RHO ordered like ID0 (0) or ID1 (1)

@@ -320,10 +320,47 @@ __device__ __forceinline__ sph_rows sph_row_bounds(unsigned key, const sph_dev_p
 // row +1, self + 1 in row 0 and b2 in row -1.  body(idx, own, rec) receives
 // that test for the candidate when OWN is set (false otherwise); idx is the
 // candidate's index in the source.  Returns (accepted, in the half list).
-template <bool OWN = false, class Src, class Test, class Body>
+// Accepted-candidate masks carried from one walk to a later walk over the
+// same neighbour set: the density pass and the relaxation after it walk the
+// same sorted positions, so the relaxation replays the density pass's masks
+// of a particle's first kMaskMemo chunks (in walk order) instead of testing
+// those candidates again.  A policy supplies (have/get) or keeps (put) them.
+constexpr unsigned kMaskMemo = 4;
+
+struct sph_masks_none {
+    __device__ bool have(unsigned) const { return false; }
+    __device__ unsigned get(unsigned) const { return 0u; }
+    __device__ void put(unsigned, unsigned) {}
+};
+
+struct sph_masks_record {
+    uint4 m = make_uint4(0u, 0u, 0u, 0u);
+    __device__ bool have(unsigned) const { return false; }
+    __device__ unsigned get(unsigned) const { return 0u; }
+    __device__ void put(unsigned ci, unsigned mask)
+    {
+        m.x = ci == 0u ? mask : m.x;
+        m.y = ci == 1u ? mask : m.y;
+        m.z = ci == 2u ? mask : m.z;
+        m.w = ci == 3u ? mask : m.w;
+    }
+};
+
+struct sph_masks_replay {
+    uint4 m;
+    __device__ bool have(unsigned ci) const { return ci < kMaskMemo; }
+    __device__ unsigned get(unsigned ci) const
+    {
+        return ci == 0u ? m.x : ci == 1u ? m.y : ci == 2u ? m.z : m.w;
+    }
+    __device__ void put(unsigned, unsigned) {}
+};
+
+template <bool OWN = false, class Src, class Test, class Body, class Masks>
 __device__ __forceinline__ uint2 sph_walk_masked(unsigned self, const sph_rows &R, const Src &src,
-                                                 Test &&test, Body &&body)
+                                                 Test &&test, Body &&body, Masks &mk)
 {
+    unsigned ci = 0; // chunks of this lane so far (all rows, walk order)
     unsigned m0 = 0, m1 = 0, m2 = 0; // pending masks, m0 drained first
     unsigned x0 = 0, x1 = 0, x2 = 0; // source index of each mask's bit 0
     unsigned t0 = 0, t1 = 0, t2 = 0; // owner threshold, source index space
@@ -370,11 +407,17 @@ __device__ __forceinline__ uint2 sph_walk_masked(unsigned self, const sph_rows &
             unsigned mask = 0, x = 0;
             if (c < b3) {
                 const unsigned m = b3 - c < 32u ? b3 - c : 32u;
+                if (mk.have(ci)) {
+                    mask = mk.get(ci);
+                } else {
 #pragma unroll 8
-                for (unsigned t = 0; t < m; t++)
-                    mask |= (unsigned)test(src.at(c + t)) << t;
-                if (me - c < m) // the particle itself is no neighbour
-                    mask &= ~(1u << (me - c));
+                    for (unsigned t = 0; t < m; t++)
+                        mask |= (unsigned)test(src.at(c + t)) << t;
+                    if (me - c < m) // the particle itself is no neighbour
+                        mask &= ~(1u << (me - c));
+                }
+                mk.put(ci, mask);
+                ci++;
                 full += (unsigned)__popc(mask);
                 const unsigned own_from = tt > c ? tt - c : 0u;
                 half += (unsigned)__popc(own_from < 32u ? mask & (~0u << own_from) : 0u);

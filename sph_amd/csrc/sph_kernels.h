@@ -46,12 +46,12 @@ void sph_launch_reorder4(hipStream_t st, unsigned slots, const sph_dev_params *P
                          unsigned *key_dst, sph_dev_counters *C);
 void sph_launch_density(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
                         const float2 *POS, const unsigned *K, const unsigned *start, float2 *RHO,
-                        sph_dev_counters *C);
+                        uint4 *masks, sph_dev_counters *C);
 void sph_launch_relax(hipStream_t st, unsigned slots, bool fused, const sph_dev_params *P,
                       const int *BS, const float2 *POS, const float2 *PREV, const float2 *RHO,
                       const unsigned *K, const unsigned *start, float2 *opos, float2 *ovel,
-                      unsigned *okey, unsigned *oslot, unsigned *cnt, sph_dev_counters *C,
-                      unsigned long long *clock);
+                      unsigned *okey, unsigned *oslot, unsigned *cnt, const uint4 *masks,
+                      sph_dev_counters *C, unsigned long long *clock);
 void sph_launch_pack_coords(hipStream_t st, unsigned slots, const sph_dev_params *P,
                             const int *BS, const float2 *POS, const int *ID, short *out);
 void sph_launch_neighbor_lists(hipStream_t st, unsigned slots, const sph_dev_params *P,

@@ -292,18 +292,19 @@ __device__ __forceinline__ GlobalSrc<Rec, Load> global_src(Load l)
 // else from global memory (same order, same results).  Every thread of the
 // workgroup calls this (the staging and its barrier need all of them);
 // `live` lanes walk.
-template <bool OWN, class Rec, class Load, class Test, class Body>
+template <bool OWN, class Rec, class Load, class Test, class Body, class Masks>
 __device__ __forceinline__ uint2 walk_group(const TilePlan &T, Rec *tile, Load load, bool live,
                                             unsigned self, const sph_rows &R, unsigned key,
-                                            const sph_dev_params &P, Test &test, Body &body)
+                                            const sph_dev_params &P, Test &test, Body &body,
+                                            Masks &mk)
 {
     uint2 fh = make_uint2(0u, 0u);
     if (T.ok) {
         stage_tile(T, tile, load);
         if (live)
-            fh = sph_walk_masked<OWN>(self, R, lds_src(tile, T, key, P), test, body);
+            fh = sph_walk_masked<OWN>(self, R, lds_src(tile, T, key, P), test, body, mk);
     } else if (live) {
-        fh = sph_walk_masked<OWN>(self, R, global_src<Rec>(load), test, body);
+        fh = sph_walk_masked<OWN>(self, R, global_src<Rec>(load), test, body, mk);
     }
     return fh;
 }
@@ -389,7 +390,8 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
             nvy = push ? nvy - iy * 0.5f : nvy;
         };
         plan_tile_par(s0, s1, K, start, P, T);
-        const uint2 fh = walk_group<false>(T, tile, load, live, s, R, key, P, test, body);
+        sph_masks_none mk;
+        const uint2 fh = walk_group<false>(T, tile, load, live, s, R, key, P, test, body, mk);
         full = fh.x;
         half = fh.y;
     }
@@ -635,7 +637,7 @@ __global__ void __launch_bounds__(kTPB)
 k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
           const float2 *__restrict__ POS, const unsigned *__restrict__ K,
           const unsigned *__restrict__ start, float2 *__restrict__ RHO,
-          sph_dev_counters *__restrict__ C)
+          uint4 *__restrict__ masks, sph_dev_counters *__restrict__ C)
 {
     __shared__ TilePlan T;
     __shared__ float2 tile[kTileCap];
@@ -667,9 +669,12 @@ k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
         rho = in ? rho + OmR2 : rho;
         rhon = in ? rhon + OmR2 * (1.0f - ratio) : rhon;
     };
-    const uint2 fh = walk_group<false>(T, tile, load, live, s, R, key, P, test, body);
-    if (live)
+    sph_masks_record mk;
+    const uint2 fh = walk_group<false>(T, tile, load, live, s, R, key, P, test, body, mk);
+    if (live) {
         RHO[s] = make_float2(rho, rhon);
+        masks[s] = mk.m; // for the relaxation of the same step
+    }
     const unsigned full = fh.x, half = fh.y;
     report_neighbors(full, half, C);
 }
@@ -686,7 +691,8 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
         const unsigned *__restrict__ start, float2 *__restrict__ out_pos,
         float2 *__restrict__ out_vel, unsigned *__restrict__ out_key,
         unsigned *__restrict__ out_slot, unsigned *__restrict__ cnt,
-        sph_dev_counters *__restrict__ C, unsigned long long *__restrict__ clock)
+        const uint4 *__restrict__ masks, sph_dev_counters *__restrict__ C,
+        unsigned long long *__restrict__ clock)
 {
     __shared__ TilePlan T;
     __shared__ float4 tile[kTileCap];
@@ -754,7 +760,13 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
         nx = push ? nx - Dx : nx;
         ny = push ? ny - Dy : ny;
     };
-    walk_group<true>(T, tile, load, live, s, R, key, P, test, body);
+    if (FUSED) { // the density pass of this step walked the same neighbour set
+        sph_masks_replay mk{live ? masks[s] : make_uint4(0u, 0u, 0u, 0u)};
+        walk_group<true>(T, tile, load, live, s, R, key, P, test, body, mk);
+    } else {
+        sph_masks_none mk;
+        walk_group<true>(T, tile, load, live, s, R, key, P, test, body, mk);
+    }
     if (!live)
         return;
     if (FUSED) {
@@ -1066,23 +1078,23 @@ void sph_launch_reorder4(hipStream_t st, unsigned slots, const sph_dev_params *P
 
 void sph_launch_density(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
                         const float2 *POS, const unsigned *K, const unsigned *start, float2 *RHO,
-                        sph_dev_counters *C)
+                        uint4 *masks, sph_dev_counters *C)
 {
-    launch(k_density, GRID(slots), kTPB, st, P, BS, POS, K, start, RHO, C);
+    launch(k_density, GRID(slots), kTPB, st, P, BS, POS, K, start, RHO, masks, C);
 }
 
 void sph_launch_relax(hipStream_t st, unsigned slots, bool fused, const sph_dev_params *P,
                       const int *BS, const float2 *POS, const float2 *PREV, const float2 *RHO,
                       const unsigned *K, const unsigned *start, float2 *opos, float2 *ovel,
-                      unsigned *okey, unsigned *oslot, unsigned *cnt, sph_dev_counters *C,
-                      unsigned long long *clock)
+                      unsigned *okey, unsigned *oslot, unsigned *cnt, const uint4 *masks,
+                      sph_dev_counters *C, unsigned long long *clock)
 {
     if (fused)
         launch(k_relax<true>, GRID(slots), kTPB, st, P, BS, POS, PREV, RHO, K, start, opos, ovel,
-                                                    okey, oslot, cnt, C, clock);
+                                                    okey, oslot, cnt, masks, C, clock);
     else
         launch(k_relax<false>, GRID(slots), kTPB, st, P, BS, POS, PREV, RHO, K, start, opos, ovel,
-                                                     okey, oslot, cnt, C, clock);
+                                                     okey, oslot, cnt, masks, C, clock);
 }
 
 void sph_launch_pack_coords(hipStream_t st, unsigned slots, const sph_dev_params *P,
