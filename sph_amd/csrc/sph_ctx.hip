@@ -324,6 +324,7 @@ int sph_create_batch(sph_ctx **out, const sph_scene_desc *scenes, int nscenes)
     c->slots = (unsigned)slots;
     int rc = SPH_OK;
     const size_t N = c->slots, M = (size_t)c->cells + 1;
+    const size_t MP = sph_scan_padded((unsigned)M); // the scan works on whole tiles
 #define A(p, count)                                                                              \
     if (!rc)                                                                                     \
     rc = dalloc(&c->p, count)
@@ -331,7 +332,7 @@ int sph_create_batch(sph_ctx **out, const sph_scene_desc *scenes, int nscenes)
     A(POS1, N); A(UPOS, N); A(UVEL, N); A(T2a, N); A(T2b, N);
     A(ID0, N); A(ID1, N); A(perm_src, N); A(perm_id, N); A(pos_of_id, N);
     A(K0, N); A(K1, N); A(ukey, N); A(uslot, N);
-    A(cnt, M); A(start, M); A(bsum, sph_scan_blocks((unsigned)M));
+    A(cnt, MP); A(start, MP); A(bsum, sph_scan_blocks((unsigned)M));
     A(coords, 2 * N); A(MASK, N); A(dP, (size_t)nscenes); A(BS, N / kSphTPB); A(dC, 1); A(dclock, 1);
 #undef A
     if (!rc) {
@@ -346,9 +347,9 @@ int sph_create_batch(sph_ctx **out, const sph_scene_desc *scenes, int nscenes)
                 bs[c->sc[s].poff / kSphTPB + b] = s;
         hipError_t e = hipMemcpy(c->BS, bs.data(), sizeof(int) * bs.size(), hipMemcpyHostToDevice);
         if (e == hipSuccess)
-            e = hipMemset(c->cnt, 0, sizeof(unsigned) * M);
+            e = hipMemset(c->cnt, 0, sizeof(unsigned) * MP);
         if (e == hipSuccess)
-            e = hipMemset(c->start, 0, sizeof(unsigned) * M);
+            e = hipMemset(c->start, 0, sizeof(unsigned) * MP);
         if (e == hipSuccess)
             e = hipMemset(c->dC, 0, sizeof(sph_dev_counters));
         if (e == hipSuccess)

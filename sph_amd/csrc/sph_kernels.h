@@ -29,7 +29,10 @@ void sph_launch_update_velocities(hipStream_t st, unsigned slots, const sph_dev_
 void sph_launch_key_hist(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
                          const float2 *POS, unsigned *okey, unsigned *oslot, unsigned *cnt,
                          sph_dev_counters *C);
+// The scan over m counts reads and writes whole tiles: cnt and start must hold
+// sph_scan_padded(m) entries (16-byte aligned), cnt's beyond m zero.
 unsigned sph_scan_blocks(unsigned m);
+size_t sph_scan_padded(unsigned m);
 void sph_launch_scan_reduce(hipStream_t st, const unsigned *cnt, unsigned m, unsigned *bsum);
 void sph_launch_scan_apply(hipStream_t st, unsigned *cnt, unsigned m, const unsigned *bsum,
                            unsigned *start);

@@ -487,90 +487,81 @@ k_key_hist(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
 
 // ---------------------------------------------------------------------------
 // Cell-count scan: start[c] = sum of cnt[0..c), over m = cells + 1 entries
-// (cnt[cells] is always 0, so start[cells] = n).  Reduce-then-scan: tiles of
-// kScanTile counts; the apply kernel adds the sum of all earlier tiles,
-// writes the starts and clears the counts for the next sort.
+// (cnt[cells] is always 0, so start[cells] = n).  Reduce-then-scan over tiles
+// of kScanTile counts; the apply kernel adds the sum of all earlier tiles,
+// writes the starts and clears the counts for the next sort.  Both arrays hold
+// whole tiles (sph_scan_padded(m) entries, the padding's counts zero), so
+// nothing is bounds-checked: a tile is two halves of 1024 counts, thread t
+// holding counts [4t, 4t+4) of each half, and every wave's load or store
+// covers 1 KiB of consecutive memory.
 constexpr int kScanThreads = 256;
-constexpr int kScanPerThread = 8;
-constexpr int kScanTile = kScanThreads * kScanPerThread;
+constexpr int kScanTile = 2 * 4 * kScanThreads;
+
+__device__ __forceinline__ unsigned sum4(uint4 v) { return v.x + v.y + v.z + v.w; }
 
 __global__ void __launch_bounds__(kScanThreads)
-k_scan_reduce(const unsigned *__restrict__ cnt, unsigned m, unsigned *__restrict__ bsum)
+k_scan_reduce(const uint4 *__restrict__ cnt, unsigned *__restrict__ bsum)
 {
     __shared__ unsigned red[kScanThreads / kWave];
-    const unsigned base = blockIdx.x * kScanTile + threadIdx.x * kScanPerThread;
-    unsigned sum = 0;
-#pragma unroll
-    for (int e = 0; e < kScanPerThread; e++)
-        sum += base + e < m ? cnt[base + e] : 0u;
+    const uint4 *t = cnt + blockIdx.x * (kScanTile / 4);
+    unsigned sum = sum4(t[threadIdx.x]) + sum4(t[threadIdx.x + kScanThreads]);
     sum = wave_sum_u32(sum);
     if (lane0())
         red[threadIdx.x / kWave] = sum;
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned t = 0;
+        unsigned a = 0;
         for (int i = 0; i < kScanThreads / kWave; i++)
-            t += red[i];
-        bsum[blockIdx.x] = t;
+            a += red[i];
+        bsum[blockIdx.x] = a;
     }
 }
 
 __global__ void __launch_bounds__(kScanThreads)
-k_scan_apply(unsigned *__restrict__ cnt, unsigned m, const unsigned *__restrict__ bsum,
-             unsigned *__restrict__ start)
+k_scan_apply(uint4 *__restrict__ cnt, const unsigned *__restrict__ bsum,
+             uint4 *__restrict__ start)
 {
-    __shared__ unsigned wsum[kScanThreads / kWave];
-    __shared__ unsigned wpre[kScanThreads / kWave];
+    constexpr int kW = kScanThreads / kWave;
+    __shared__ unsigned wpre[kW], wa[kW], wb[kW];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    uint4 *t = cnt + blockIdx.x * (kScanTile / 4);
+    const uint4 a = t[threadIdx.x], b = t[threadIdx.x + kScanThreads];
     // prefix of all earlier tiles
     unsigned pre = 0;
-    for (unsigned b = threadIdx.x; b < blockIdx.x; b += kScanThreads)
-        pre += bsum[b];
+    for (unsigned i = threadIdx.x; i < blockIdx.x; i += kScanThreads)
+        pre += bsum[i];
     pre = wave_sum_u32(pre);
-    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    if (lane == 0)
-        wsum[w] = pre;
-    __syncthreads();
-    unsigned tile_pre = 0;
-    for (int i = 0; i < kScanThreads / kWave; i++)
-        tile_pre += wsum[i];
-    __syncthreads();
-
-    const unsigned base = blockIdx.x * kScanTile + threadIdx.x * kScanPerThread;
-    unsigned v[kScanPerThread];
-    unsigned t = 0;
-#pragma unroll
-    for (int e = 0; e < kScanPerThread; e++) {
-        v[e] = base + e < m ? cnt[base + e] : 0u;
-        t += v[e];
-    }
-    // inclusive wave scan of per-thread totals
-    unsigned incl = t;
+    // inclusive wave scans of both halves' per-thread totals
+    const unsigned ta = sum4(a), tb = sum4(b);
+    unsigned ia = ta, ib = tb;
 #pragma unroll
     for (int o = 1; o < kWave; o <<= 1) {
-        unsigned y = (unsigned)__shfl_up((int)incl, o, kWave);
-        if (lane >= o)
-            incl += y;
+        const unsigned ya = (unsigned)__shfl_up((int)ia, o, kWave);
+        const unsigned yb = (unsigned)__shfl_up((int)ib, o, kWave);
+        ia += lane >= o ? ya : 0u;
+        ib += lane >= o ? yb : 0u;
     }
-    if (lane == kWave - 1)
-        wsum[w] = incl;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned a = 0;
-        for (int i = 0; i < kScanThreads / kWave; i++) {
-            wpre[i] = a;
-            a += wsum[i];
-        }
+    if (lane == kWave - 1) {
+        wpre[w] = pre;
+        wa[w] = ia;
+        wb[w] = ib;
     }
     __syncthreads();
-    unsigned run = tile_pre + wpre[w] + (incl - t);
+    unsigned tile_pre = 0, half_a = 0, pa = 0, pb = 0;
 #pragma unroll
-    for (int e = 0; e < kScanPerThread; e++) {
-        if (base + e < m) {
-            start[base + e] = run;
-            cnt[base + e] = 0u;
-        }
-        run += v[e];
+    for (int i = 0; i < kW; i++) {
+        tile_pre += wpre[i];
+        half_a += wa[i];
+        pa += i < w ? wa[i] : 0u;
+        pb += i < w ? wb[i] : 0u;
     }
+    const unsigned ra = tile_pre + pa + (ia - ta);
+    const unsigned rb = tile_pre + half_a + pb + (ib - tb);
+    uint4 *o = start + blockIdx.x * (kScanTile / 4);
+    o[threadIdx.x] = make_uint4(ra, ra + a.x, ra + a.x + a.y, ra + a.x + a.y + a.z);
+    o[threadIdx.x + kScanThreads] = make_uint4(rb, rb + b.x, rb + b.x + b.y, rb + b.x + b.y + b.z);
+    t[threadIdx.x] = make_uint4(0u, 0u, 0u, 0u);
+    t[threadIdx.x + kScanThreads] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 // Drop (id, source index) into the cell ranges in slot order.
@@ -1040,15 +1031,19 @@ void sph_launch_key_hist(hipStream_t st, unsigned slots, const sph_dev_params *P
 
 unsigned sph_scan_blocks(unsigned m) { return (m + kScanTile - 1) / kScanTile; }
 
+size_t sph_scan_padded(unsigned m) { return (size_t)sph_scan_blocks(m) * kScanTile; }
+
 void sph_launch_scan_reduce(hipStream_t st, const unsigned *cnt, unsigned m, unsigned *bsum)
 {
-    launch(k_scan_reduce, dim3(sph_scan_blocks(m)), kScanThreads, st, cnt, m, bsum);
+    launch(k_scan_reduce, dim3(sph_scan_blocks(m)), kScanThreads, st,
+           reinterpret_cast<const uint4 *>(cnt), bsum);
 }
 
 void sph_launch_scan_apply(hipStream_t st, unsigned *cnt, unsigned m, const unsigned *bsum,
                            unsigned *start)
 {
-    launch(k_scan_apply, dim3(sph_scan_blocks(m)), kScanThreads, st, cnt, m, bsum, start);
+    launch(k_scan_apply, dim3(sph_scan_blocks(m)), kScanThreads, st,
+           reinterpret_cast<uint4 *>(cnt), bsum, reinterpret_cast<uint4 *>(start));
 }
 
 void sph_launch_scatter(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
