@@ -44,9 +44,11 @@ KERNEL_BYTES = {
     "scan_reduce": (0, 4),       # read count
     "scan_apply": (0, 12),       # read count, write start, clear count
     "scatter": (20, 0),          # read key, slot, id (12); write src, id (8)
-    "reorder": (44, 0),          # read src, id, key (12), 4 fields (16); write 4 fields, id, key (24)... see DESIGN
+    "reorder": (52, 0),          # read src, id, key (12), 2 float2 fields (16); write 2 fields, id, key (24)
     "density": (20, 4),          # read x, y, key (12); write rho, rho_near (8); cell starts
+    #                              (plus 16 B of candidate masks written for the relaxation)
     "relax": (52, 12),           # read x, y, xp, yp, rho, rhon, key (28); write x, y, vx, vy, key, slot (24)
+    #                              (plus the 16 B of masks read back)
 }
 # profiler names of the kernels (template instances of the fused step)
 KERNEL_SYMBOLS = {"visc_predict": "k_visc_predict<true,", "relax": "k_relax<true>",
