@@ -586,6 +586,8 @@ k_scatter(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
 // ranking restores the reference's bucket order (ascending id).  The rank
 // also feeds the occupancy counters: a particle of rank 100 means its cell
 // holds more than the reference's 100 bucket slots (src/fluid.c:174).
+constexpr unsigned kRankBatch = 8;
+
 template <int NF>
 __global__ void __launch_bounds__(kTPB)
 k_reorder(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
@@ -601,15 +603,30 @@ k_reorder(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     if (k < P.poff + (unsigned)P.n) {
         const int i = perm_src[k];
         const int myid = perm_id[k];
+        // the fields are in flight while the rank is found
+        float2 v[NF];
+#pragma unroll
+        for (int q = 0; q < NF; q++)
+            v[q] = f.src[q][i];
         const unsigned c = key[i];
         const unsigned b = start[c], e = start[c + 1];
+        // rank = members of the cell with a smaller id, kRankBatch loads in
+        // flight at a time (indices clamped into the cell, so every load is
+        // in bounds and none is predicated)
         unsigned rank = 0;
-        for (unsigned m = b; m < e; m++)
-            rank += perm_id[m] < myid;
+        for (unsigned m = b; m < e; m += kRankBatch) {
+            int idv[kRankBatch];
+#pragma unroll
+            for (int t = 0; t < kRankBatch; t++)
+                idv[t] = perm_id[min(m + t, e - 1u)];
+#pragma unroll
+            for (int t = 0; t < kRankBatch; t++)
+                rank += (m + t < e && idv[t] < myid) ? 1u : 0u;
+        }
         const unsigned d = b + rank;
 #pragma unroll
         for (int q = 0; q < NF; q++)
-            f.dst[q][d] = f.src[q][i];
+            f.dst[q][d] = v[q];
         id_dst[d] = myid;
         key_dst[d] = c;
         if (rank == SPH_REF_MAX_BUCKET)
