@@ -128,7 +128,7 @@ struct Tick {
     }
 };
 
-void enqueue_sort_tail(sph_ctx *c, const int *id_src)
+void enqueue_sort_tail(sph_ctx *c, const int *id_src, unsigned *key_dst)
 {
     const unsigned m = c->cells + 1;
     {
@@ -142,7 +142,7 @@ void enqueue_sort_tail(sph_ctx *c, const int *id_src)
     {
         Tick t(c, KSC);
         sph_launch_scatter(c->st, c->slots, c->dP, c->BS, c->ukey, c->uslot, c->start, id_src,
-                           c->perm_src, c->perm_id);
+                           c->perm_src, c->perm_id, key_dst);
     }
 }
 
@@ -161,12 +161,12 @@ void enqueue_fused_step(sph_ctx *c, bool visc)
                                 c->start, c->UPOS, c->ukey, c->uslot, c->cnt, c->dC, c->dclock,
                                 c->UVEL);
     }
-    enqueue_sort_tail(c, c->ID0);
+    enqueue_sort_tail(c, c->ID0, c->K1);
     {
         Tick t(c, KRO);
         sph_reorder_fields<2> f{{c->UPOS, c->POS0}, {c->POS1, c->PREV}};
-        sph_launch_reorder2(c->st, S, c->dP, c->BS, c->start, c->perm_src, c->perm_id, c->ukey,
-                            f, c->ID1, c->K1, c->dC);
+        sph_launch_reorder2(c->st, S, c->dP, c->BS, c->start, c->perm_src, c->perm_id, c->K1,
+                            f, c->ID1, c->dC);
     }
     {
         Tick t(c, KDE);
@@ -179,12 +179,12 @@ void enqueue_fused_step(sph_ctx *c, bool visc)
                          c->UPOS, c->UVEL, c->ukey, c->uslot, c->cnt, c->MASK, c->dC,
                          c->dclock);
     }
-    enqueue_sort_tail(c, c->ID1);
+    enqueue_sort_tail(c, c->ID1, c->K0);
     {
         Tick t(c, KRO);
         sph_reorder_fields<2> f{{c->UPOS, c->UVEL}, {c->POS0, c->VEL}};
-        sph_launch_reorder2(c->st, S, c->dP, c->BS, c->start, c->perm_src, c->perm_id, c->ukey,
-                            f, c->ID0, c->K0, c->dC);
+        sph_launch_reorder2(c->st, S, c->dP, c->BS, c->start, c->perm_src, c->perm_id, c->K0,
+                            f, c->ID0, c->dC);
     }
 }
 
@@ -715,11 +715,11 @@ int sph_phase_hash(sph_ctx *c, int compute_density)
         return SPH_OK;
     const unsigned S = c->slots;
     sph_launch_key_hist(c->st, S, c->dP, c->BS, c->POS0, c->ukey, c->uslot, c->cnt, c->dC);
-    enqueue_sort_tail(c, c->ID0);
+    enqueue_sort_tail(c, c->ID0, c->K1);
     sph_reorder_fields<4> f{{c->POS0, c->VEL, c->PREV, c->RHO},
                             {c->POS1, c->UVEL, c->T2a, c->T2b}};
-    sph_launch_reorder4(c->st, S, c->dP, c->BS, c->start, c->perm_src, c->perm_id, c->ukey, f,
-                        c->ID1, c->K1, c->dC);
+    sph_launch_reorder4(c->st, S, c->dP, c->BS, c->start, c->perm_src, c->perm_id, c->K1, f,
+                        c->ID1, c->dC);
     const size_t b = sizeof(float2) * (size_t)S;
     for (int k = 0; k < 4; k++)
         HIPCHK(hipMemcpyAsync((void *)f.src[k], f.dst[k], b, hipMemcpyDeviceToDevice, c->st));

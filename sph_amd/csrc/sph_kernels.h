@@ -38,15 +38,16 @@ void sph_launch_scan_apply(hipStream_t st, unsigned *cnt, unsigned m, const unsi
                            unsigned *start);
 void sph_launch_scatter(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
                         const unsigned *key, const unsigned *slot, const unsigned *start,
-                        const int *id_src, int *perm_src, int *perm_id);
+                        const int *id_src, int *perm_src, int *perm_id, unsigned *key_dst);
+// skey: the sorted keys the scatter wrote (its key_dst)
 void sph_launch_reorder2(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
                          const unsigned *start, const int *perm_src, const int *perm_id,
-                         const unsigned *key, sph_reorder_fields<2> f, int *id_dst,
-                         unsigned *key_dst, sph_dev_counters *C);
+                         const unsigned *skey, sph_reorder_fields<2> f, int *id_dst,
+                         sph_dev_counters *C);
 void sph_launch_reorder4(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
                          const unsigned *start, const int *perm_src, const int *perm_id,
-                         const unsigned *key, sph_reorder_fields<4> f, int *id_dst,
-                         unsigned *key_dst, sph_dev_counters *C);
+                         const unsigned *skey, sph_reorder_fields<4> f, int *id_dst,
+                         sph_dev_counters *C);
 void sph_launch_density(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
                         const float2 *POS, const unsigned *K, const unsigned *start, float2 *RHO,
                         uint4 *masks, sph_dev_counters *C);

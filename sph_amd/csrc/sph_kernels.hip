@@ -569,16 +569,18 @@ __global__ void __launch_bounds__(kTPB)
 k_scatter(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
           const unsigned *__restrict__ key, const unsigned *__restrict__ slot,
           const unsigned *__restrict__ start, const int *__restrict__ id_src,
-          int *__restrict__ perm_src, int *__restrict__ perm_id)
+          int *__restrict__ perm_src, int *__restrict__ perm_id, unsigned *__restrict__ key_dst)
 {
     const unsigned blk = xcd_block();
     const sph_dev_params &P = Pp[BS[blk]];
     const unsigned i = blk * kTPB + threadIdx.x;
     if (i >= P.poff + (unsigned)P.n)
         return;
-    const unsigned pos = start[key[i]] + slot[i];
+    const unsigned k = key[i];
+    const unsigned pos = start[k] + slot[i];
     perm_src[pos] = (int)i;
     perm_id[pos] = id_src[i];
+    key_dst[pos] = k; // every position of a cell holds its key, whatever the order inside
 }
 
 // Rank each particle by id inside its cell and gather its fields to
@@ -592,9 +594,8 @@ template <int NF>
 __global__ void __launch_bounds__(kTPB)
 k_reorder(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
           const unsigned *__restrict__ start, const int *__restrict__ perm_src,
-          const int *__restrict__ perm_id, const unsigned *__restrict__ key,
-          sph_reorder_fields<NF> f, int *__restrict__ id_dst, unsigned *__restrict__ key_dst,
-          sph_dev_counters *__restrict__ C)
+          const int *__restrict__ perm_id, const unsigned *__restrict__ skey,
+          sph_reorder_fields<NF> f, int *__restrict__ id_dst, sph_dev_counters *__restrict__ C)
 {
     const unsigned blk = xcd_block();
     const sph_dev_params &P = Pp[BS[blk]];
@@ -608,7 +609,7 @@ k_reorder(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
 #pragma unroll
         for (int q = 0; q < NF; q++)
             v[q] = f.src[q][i];
-        const unsigned c = key[i];
+        const unsigned c = skey[k]; // the sorted keys, written by the scatter
         const unsigned b = start[c], e = start[c + 1];
         // rank = members of the cell with a smaller id, kRankBatch loads in
         // flight at a time (indices clamped into the cell, so every load is
@@ -628,7 +629,6 @@ k_reorder(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
         for (int q = 0; q < NF; q++)
             f.dst[q][d] = v[q];
         id_dst[d] = myid;
-        key_dst[d] = c;
         if (rank == SPH_REF_MAX_BUCKET)
             atomicAdd(&C->bucket_overflow, 1ull);
         occupancy = rank + 1;
@@ -1065,27 +1065,28 @@ void sph_launch_scan_apply(hipStream_t st, unsigned *cnt, unsigned m, const unsi
 
 void sph_launch_scatter(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
                         const unsigned *key, const unsigned *slot, const unsigned *start,
-                        const int *id_src, int *perm_src, int *perm_id)
+                        const int *id_src, int *perm_src, int *perm_id, unsigned *key_dst)
 {
-    launch(k_scatter, GRID(slots), kTPB, st, P, BS, key, slot, start, id_src, perm_src, perm_id);
+    launch(k_scatter, GRID(slots), kTPB, st, P, BS, key, slot, start, id_src, perm_src, perm_id,
+           key_dst);
 }
 
 void sph_launch_reorder2(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
                          const unsigned *start, const int *perm_src, const int *perm_id,
-                         const unsigned *key, sph_reorder_fields<2> f, int *id_dst,
-                         unsigned *key_dst, sph_dev_counters *C)
+                         const unsigned *skey, sph_reorder_fields<2> f, int *id_dst,
+                         sph_dev_counters *C)
 {
-    launch(k_reorder<2>, GRID(slots), kTPB, st, P, BS, start, perm_src, perm_id, key, f, id_dst,
-                                               key_dst, C);
+    launch(k_reorder<2>, GRID(slots), kTPB, st, P, BS, start, perm_src, perm_id, skey, f, id_dst,
+                                               C);
 }
 
 void sph_launch_reorder4(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
                          const unsigned *start, const int *perm_src, const int *perm_id,
-                         const unsigned *key, sph_reorder_fields<4> f, int *id_dst,
-                         unsigned *key_dst, sph_dev_counters *C)
+                         const unsigned *skey, sph_reorder_fields<4> f, int *id_dst,
+                         sph_dev_counters *C)
 {
-    launch(k_reorder<4>, GRID(slots), kTPB, st, P, BS, start, perm_src, perm_id, key, f, id_dst,
-                                               key_dst, C);
+    launch(k_reorder<4>, GRID(slots), kTPB, st, P, BS, start, perm_src, perm_id, skey, f, id_dst,
+                                               C);
 }
 
 void sph_launch_density(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
