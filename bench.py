@@ -16,6 +16,7 @@ traffic, and the reference's own C step (compiled from its sources, -O3
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -43,15 +44,15 @@ KERNEL_BYTES = {
     "visc_predict": (36, 8),
     "scan_reduce": (0, 4),       # read count
     "scan_apply": (0, 12),       # read count, write start, clear count
-    "scatter": (20, 0),          # read key, slot, id (12); write src, id (8)
-    "reorder": (52, 0),          # read src, id, key (12), 2 float2 fields (16); write 2 fields, id, key (24)
+    "scatter": (24, 0),          # read key, slot, id (12); write src, id, sorted key (12)
+    "reorder": (48, 0),          # read src, id, sorted key (12), 2 float2 fields (16); write 2 fields, id (20)
     "density": (20, 4),          # read x, y, key (12); write rho, rho_near (8); cell starts
     #                              (plus 16 B of candidate masks written for the relaxation)
     "relax": (52, 12),           # read x, y, xp, yp, rho, rhon, key (28); write x, y, vx, vy, key, slot (24)
     #                              (plus the 16 B of masks read back)
 }
 # profiler names of the kernels (template instances of the fused step)
-KERNEL_SYMBOLS = {"visc_predict": "k_visc_predict<true,", "relax": "k_relax<true>",
+KERNEL_SYMBOLS = {"visc_predict": "k_visc_predict<true, true, true>", "relax": "k_relax<true>",
                   "density": "k_density", "reorder": "k_reorder<2>", "scatter": "k_scatter",
                   "scan_apply": "k_scan_apply", "scan_reduce": "k_scan_reduce"}
 # launches of each kind per step
@@ -354,7 +355,8 @@ def traffic_of(kernel_name):
         return None
     with open(TRAFFIC_FILE) as f:
         for line in f:
-            parts = line.split()
+            # columns are separated by two or more spaces; kernel names hold single ones
+            parts = re.split(r"\s{2,}", line.strip())
             if parts and parts[0] == kernel_name and len(parts) >= 5:
                 try:
                     return float(parts[-1]) * 1e6
