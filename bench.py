@@ -6,7 +6,9 @@ g = 9.8, dt = 1/60, rho0 = 10, k = 0.004, k_near = 0.01, k_spring = 10,
 sigma = 0, beta = 0.3, no mover.  A step is one full reference step
 (src/fluid.c:273-341): gravity, viscosity, predict, hash + density,
 relaxation, velocity update, hash.  The state stays in HBM; the timed region
-replays the captured HIP graph of the fused step.
+replays the captured HIP graph of the fused step.  The value is the median of
+--repeats timed runs (default 3), each of exactly --steps steps from the same
+uploaded initial state after --warmup untimed steps.
 
 Prints one JSON line: particle-steps/s, the HBM roofline of the dominant
 kernel (HIP-event timed on the solver's stream), the whole-step algorithmic
@@ -147,22 +149,34 @@ def cpu_baseline_batched(cores, budget_s=10.0):
                       f"throughput summed over the concurrent workers"}
 
 
-def batched(steps, warmup):
+def timed_repeats(s, upload, steps, warmup, repeats):
+    """Median wall time of `repeats` runs of exactly `steps` graph-replayed
+    steps, each from the same uploaded initial state after `warmup` untimed
+    steps, bracketed by stream synchronisation.  Returns (median, all)."""
+    runs = []
+    for _ in range(repeats):
+        upload()
+        s.reset_counters()
+        s.step(warmup)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.sync()
+        runs.append(time.perf_counter() - t0)
+    return float(np.median(runs)), runs
+
+
+def batched(steps, warmup, repeats=3):
     """The batched configuration: 256 scenes (2k-64k particles each, four
     parameter presets) stepped by the same launches through one graph."""
     scenes = sc.batched_scenes()
     parts, counts = sc.concat(scenes)
     s = sa.Solver.batch([(t, b, len(p)) for t, b, p, _ in scenes])
-    s.upload_batch(parts, counts)
-    s.step(warmup)
-    s.sync()
-    t0 = time.perf_counter()
-    s.step(steps)
-    s.sync()
-    el = time.perf_counter() - t0
+    el, runs = timed_repeats(s, lambda: s.upload_batch(parts, counts), steps, warmup, repeats)
     n = int(counts.sum())
     return {"value": n * steps / el, "unit": "particle-steps/s", "ms_per_step": el * 1e3 / steps,
             "steps": steps, "warmup": warmup, "scenes": len(scenes), "particles": n,
+            "repeats_ms_per_step": [r * 1e3 / steps for r in runs],
             "counters": s.counters()}
 
 
@@ -370,6 +384,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--repeats", type=int, default=3,
+                    help="timed runs (each from the same initial state); value is the median")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched workload line")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate line")
@@ -408,13 +424,7 @@ def main():
     tun, box, parts = workload()
     n = len(parts)
     s = sa.Solver(tun, box, n)
-    s.upload(parts)
-    s.step(args.warmup)
-    s.sync()
-    t0 = time.perf_counter()
-    s.step(args.steps)
-    s.sync()
-    el = time.perf_counter() - t0
+    el, runs = timed_repeats(s, lambda: s.upload(parts), args.steps, args.warmup, args.repeats)
     ms_per_step = el * 1e3 / args.steps
     value = n * args.steps / el
     counters = s.counters()
@@ -442,6 +452,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_per_step,
+        "repeats_ms_per_step": [r * 1e3 / args.steps for r in runs],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -464,7 +475,8 @@ def main():
         "counters": counters,
     }
     if not args.no_batched:
-        out["batched"] = batched(max(10, args.steps // 2), max(2, args.warmup // 2))
+        out["batched"] = batched(max(10, args.steps // 2), max(2, args.warmup // 2),
+                                 args.repeats)
         if cpu_batched:
             out["batched"]["cpu_baseline"] = cpu_batched
     if not args.no_surrogate:
