@@ -1,0 +1,49 @@
+"""bench.py's bookkeeping, checked without a GPU.
+
+* the HBM-traffic lookup takes a kernel by its whole name: profiler names
+  hold single spaces ("k_visc_predict<true, true, true>"), and two template
+  instances of one kernel must not be confused;
+* every kernel kind the benchmark prices has its algorithmic bytes, its
+  launches per step and, where the profiler names it differently, a symbol;
+* the step's algorithmic bytes are the survey's 192 + 56 c per particle.
+"""
+import bench
+
+
+def test_traffic_lookup_matches_whole_kernel_names(tmp_path, monkeypatch):
+    f = tmp_path / "traffic.txt"
+    f.write_text(
+        "calibration: 256 MB device copy; scale FETCH_SIZE x1.9998, WRITE_SIZE x1.0000\n"
+        "kernel                                     dispatches    FETCH_KiB    WRITE_KiB  "
+        "HBM_MB_per_launch\n"
+        "k_relax<true>                                     230      32018.9      26032.2"
+        "             92.224\n"
+        "k_visc_predict<true, false, true>                   1       8259.0      19691.9"
+        "             37.077\n"
+        "k_visc_predict<true, true, true>                  229      19730.9      18081.1"
+        "             58.919\n")
+    monkeypatch.setattr(bench, "TRAFFIC_FILE", str(f))
+    assert bench.traffic_of("k_visc_predict<true, true, true>") == 58.919e6
+    assert bench.traffic_of("k_visc_predict<true, false, true>") == 37.077e6
+    assert bench.traffic_of("k_relax<true>") == 92.224e6
+    assert bench.traffic_of("k_visc_predict<true,") is None   # a prefix is no name
+    assert bench.traffic_of("k_density") is None
+
+
+def test_committed_traffic_summary_covers_the_gather_kernels():
+    for kind in ("visc_predict", "relax"):
+        assert bench.traffic_of(bench.KERNEL_SYMBOLS[kind]) is not None, kind
+
+
+def test_kernel_tables_agree():
+    assert set(bench.KERNEL_LAUNCHES) == set(bench.KERNEL_BYTES)
+    assert set(bench.KERNEL_SYMBOLS) <= set(bench.KERNEL_BYTES)
+    # two sorts per step, one launch each of the gathers
+    assert bench.KERNEL_LAUNCHES["scatter"] == bench.KERNEL_LAUNCHES["reorder"] == 2
+    assert bench.KERNEL_LAUNCHES["density"] == bench.KERNEL_LAUNCHES["relax"] == 1
+
+
+def test_step_bytes_are_the_survey_count():
+    assert bench.B_ALG == (192, 56)
+    c = 960000 / 1048576
+    assert abs(bench.B_ALG[0] + bench.B_ALG[1] * c - 243.27) < 0.01
