@@ -300,26 +300,6 @@ __device__ __forceinline__ sph_rows sph_row_bounds(unsigned key, const sph_dev_p
     return R;
 }
 
-// The same walk in two passes, for SIMD efficiency.  Pass 1 tests the
-// candidates 32 at a time (test(rec) -> within h), branch-free, into a mask
-// of accepted ones, from which the particle's own slot `self` is cleared;
-// non-empty masks wait in a per-lane queue of up to three chunks (mask,
-// source index of bit 0, owner threshold).  Pass 2 drains the queue in the
-// order the chunks were found, lowest bit first, running body() on each
-// accepted candidate: exactly sph_walk's visiting order, so sums are
-// bit-identical.  The whole wave drains together, at the end and whenever
-// some lane's queue is full, so a lane runs the accepted pairs of up to
-// three chunks (usually all three rows) in one loop: a wave executes as many
-// pair iterations as its busiest lane has pairs, where a drain per row would
-// execute the sum over rows of each row's busiest lane.
-//
-// The half-list count of src/hash.c:176-237 is taken per chunk: the pair is
-// in self's half list when the candidate lies in a forward cell (0,+1),
-// (+1,-1), (+1,0), (+1,+1) or shares self's cell at a later sorted position
-// (cells are sorted by id), i.e. when its sorted position is at least b1 in
-// row +1, self + 1 in row 0 and b2 in row -1.  body(idx, own, rec) receives
-// that test for the candidate when OWN is set (false otherwise); idx is the
-// candidate's index in the source.  Returns (accepted, in the half list).
 // Accepted-candidate masks carried from one walk to a later walk over the
 // same neighbour set: the density pass and the relaxation after it walk the
 // same sorted positions, so the relaxation replays the density pass's masks
@@ -356,6 +336,27 @@ struct sph_masks_replay {
     __device__ void put(unsigned, unsigned) {}
 };
 
+// The same walk in two passes, for SIMD efficiency.  Pass 1 tests the
+// candidates 32 at a time (test(rec) -> within h), branch-free, into a mask
+// of accepted ones, from which the particle's own slot `self` is cleared;
+// non-empty masks wait in a per-lane queue of up to three chunks (mask,
+// source index of bit 0, owner threshold).  Pass 2 drains the queue in the
+// order the chunks were found, lowest bit first, running body() on each
+// accepted candidate: exactly sph_walk's visiting order, so sums are
+// bit-identical.  The whole wave drains together, at the end and whenever
+// some lane's queue is full, so a lane runs the accepted pairs of up to
+// three chunks (usually all three rows) in one loop: a wave executes as many
+// pair iterations as its busiest lane has pairs, where a drain per row would
+// execute the sum over rows of each row's busiest lane.
+//
+// The half-list count of src/hash.c:176-237 is taken per chunk: the pair is
+// in self's half list when the candidate lies in a forward cell (0,+1),
+// (+1,-1), (+1,0), (+1,+1) or shares self's cell at a later sorted position
+// (cells are sorted by id), i.e. when its sorted position is at least b1 in
+// row +1, self + 1 in row 0 and b2 in row -1.  body(idx, own, rec) receives
+// that test for the candidate when OWN is set (false otherwise); idx is the
+// candidate's index in the source.  mk supplies or keeps the masks of the
+// lane's first chunks (sph_masks_*).  Returns (accepted, in the half list).
 template <bool OWN = false, class Src, class Test, class Body, class Masks>
 __device__ __forceinline__ uint2 sph_walk_masked(unsigned self, const sph_rows &R, const Src &src,
                                                  Test &&test, Body &&body, Masks &mk)
