@@ -589,6 +589,7 @@ k_scatter(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
 // also feeds the occupancy counters: a particle of rank 100 means its cell
 // holds more than the reference's 100 bucket slots (src/fluid.c:174).
 constexpr unsigned kRankBatch = 8;
+constexpr unsigned kRankHalo = 64; // staged ids either side of a workgroup's positions
 
 template <int NF>
 __global__ void __launch_bounds__(kTPB)
@@ -597,32 +598,72 @@ k_reorder(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
           const int *__restrict__ perm_id, const unsigned *__restrict__ skey,
           sph_reorder_fields<NF> f, int *__restrict__ id_dst, sph_dev_counters *__restrict__ C)
 {
+    __shared__ int wid[kTPB + 2 * kRankHalo];
     const unsigned blk = xcd_block();
     const sph_dev_params &P = Pp[BS[blk]];
-    const unsigned k = blk * kTPB + threadIdx.x;
-    unsigned occupancy = 0;
-    if (k < P.poff + (unsigned)P.n) {
-        const int i = perm_src[k];
-        const int myid = perm_id[k];
-        // the fields are in flight while the rank is found
-        float2 v[NF];
+    const unsigned s0 = blk * kTPB;
+    const unsigned end = P.poff + (unsigned)P.n;
+    if (s0 >= end)
+        return; // a padding workgroup
+    const unsigned k = s0 + threadIdx.x;
+    const bool live = k < end;
+    // the ids of sorted positions [w0, w1): this workgroup's and up to
+    // kRankHalo either side, inside the scene, staged for the ranking
+    static_assert(2 * kRankHalo <= kTPB, "two staged ids per thread");
+    const unsigned w0 = s0 >= P.poff + kRankHalo ? s0 - kRankHalo : P.poff;
+    const unsigned w1 = min(s0 + kTPB + kRankHalo, end);
+    const unsigned g0 = w0 + threadIdx.x, g1 = g0 + kTPB;
+    const int id0 = g0 < w1 ? perm_id[g0] : 0;
+    const int id1 = g1 < w1 ? perm_id[g1] : 0;
+    // the lane's own record, its fields and its cell's range, all in flight
+    // with the staging loads, before the barrier
+    int i = 0, myid = 0;
+    unsigned b = 0, e = 0;
+    float2 v[NF];
+#pragma unroll
+    for (int q = 0; q < NF; q++)
+        v[q] = make_float2(0.0f, 0.0f);
+    if (live) {
+        i = perm_src[k];
+        myid = perm_id[k];
+        const unsigned c = skey[k]; // the sorted keys, written by the scatter
 #pragma unroll
         for (int q = 0; q < NF; q++)
             v[q] = f.src[q][i];
-        const unsigned c = skey[k]; // the sorted keys, written by the scatter
-        const unsigned b = start[c], e = start[c + 1];
-        // rank = members of the cell with a smaller id, kRankBatch loads in
-        // flight at a time (indices clamped into the cell, so every load is
-        // in bounds and none is predicated)
+        b = start[c];
+        e = start[c + 1];
+    }
+    if (g0 < w1)
+        wid[g0 - w0] = id0;
+    if (g1 < w1)
+        wid[g1 - w0] = id1;
+    __syncthreads();
+    unsigned occupancy = 0;
+    if (live) {
+        // rank = members of the cell with a smaller id, kRankBatch at a time
+        // (indices clamped into the cell, so every read is in bounds and none
+        // is predicated): from the staged ids when the cell lies inside them
         unsigned rank = 0;
-        for (unsigned m = b; m < e; m += kRankBatch) {
-            int idv[kRankBatch];
+        if (b >= w0 && e <= w1) {
+            for (unsigned m = b; m < e; m += kRankBatch) {
+                int idv[kRankBatch];
 #pragma unroll
-            for (int t = 0; t < kRankBatch; t++)
-                idv[t] = perm_id[min(m + t, e - 1u)];
+                for (int t = 0; t < kRankBatch; t++)
+                    idv[t] = wid[min(m + t, e - 1u) - w0];
 #pragma unroll
-            for (int t = 0; t < kRankBatch; t++)
-                rank += (m + t < e && idv[t] < myid) ? 1u : 0u;
+                for (int t = 0; t < kRankBatch; t++)
+                    rank += (m + t < e && idv[t] < myid) ? 1u : 0u;
+            }
+        } else {
+            for (unsigned m = b; m < e; m += kRankBatch) {
+                int idv[kRankBatch];
+#pragma unroll
+                for (int t = 0; t < kRankBatch; t++)
+                    idv[t] = perm_id[min(m + t, e - 1u)];
+#pragma unroll
+                for (int t = 0; t < kRankBatch; t++)
+                    rank += (m + t < e && idv[t] < myid) ? 1u : 0u;
+            }
         }
         const unsigned d = b + rank;
 #pragma unroll
