@@ -183,7 +183,11 @@ SPH_STATIC_ASSERT(sizeof(neighbor_grid_t) == 40, "neighbor_grid_t must be 40 byt
 
 typedef struct sph_ctx sph_ctx;
 
-/* Device-side health counters, accumulated since the last reset. */
+/* Device-side health counters, accumulated since the last reset.  The
+ * neighbour counts (neighbor_overflow, max_neighbors) of a step's second hash,
+ * the lists the reference's next viscosity pass uses (fluid.c:341), are taken
+ * by this library's next viscosity pass, which walks exactly that set: after
+ * T steps they cover both hashes of steps 1..T-1 and the first of step T. */
 typedef struct {
     uint64_t steps;             /* full steps taken                               */
     uint64_t hashes;            /* cell sorts performed                           */

@@ -9,7 +9,10 @@ records, and one workgroup spans several rows; it is compared bit for bit
 with the oracle.  The same scene then runs on the test build with
 256-record tiles (nearly every workgroup on the global-memory walk, XCD runs
 of one tile) in a child process, and must equal the default library bit for
-bit.
+bit.  An overfull lattice (spacing 0.06h, about 280 particles per cell) takes
+every workgroup off LDS and the reorder's ranking past its staged ids, and
+exceeds both of the reference's caps; it too is compared bit for bit, with the
+overflow counters.
 """
 import os
 import subprocess
@@ -59,6 +62,42 @@ def test_dense_tiles_bitexact_against_oracle():
     c, oc = s.counters(), g.counters()
     assert c["max_neighbors"] == oc["max_neighbors"]
     assert c["neighbor_overflow"] == oc["neighbor_overflow"]
+
+
+def test_overfull_cells_bitexact_against_oracle():
+    """Spacing 0.06h: about 280 particles per cell (past the reference's
+    100-slot buckets and 400-entry neighbour lists, both counted).  Cells this
+    large reach past the ids the reorder stages around each workgroup (its
+    positions +-64), so the ranking's global-memory path runs, and every tile
+    is too large for LDS."""
+    parts = sa.scene_lattice(48, 48, 0.06, 0.01, seed=5)
+    tun, box = dam_break_tunables(), po.make_box(60, 40)
+    s = sa.Solver(sa.Tunables.from_buffer_copy(bytes(tun)), sa.AABB.from_buffer_copy(bytes(box)),
+                  len(parts))
+    s.upload(parts)
+    g = po.GatherSim(box, tun, len(parts))
+    g.set_state(parts.view(po.P32))
+    for step in range(1, 4):
+        s.step(1, graph=False)
+        g.step(1)
+        assert_bitexact(s.download(), g.get_state(), what=f"overfull step {step}")
+    s.step(2)  # the captured fused step
+    g.step(1)
+    # the device counts a step's post-relaxation lists in the next step's
+    # viscosity pass, which walks exactly that set: the neighbour counts are
+    # the oracle's before its last hash; the cell counts include every sort
+    for ph in range(po.PH_GRAVITY, po.PH_HASH):
+        g.phase(ph)
+    before_last_hash = g.counters()
+    g.phase(po.PH_HASH)
+    oc = g.counters()
+    assert_bitexact(s.download(), g.get_state(), what="overfull, graph steps 4-5")
+    c = s.counters()
+    assert c["max_cell_count"] == oc["max_cell_count"] > 64 + 64
+    assert c["bucket_overflow"] == oc["bucket_overflow"] > 0
+    for k in ("neighbor_overflow", "max_neighbors"):
+        assert c[k] == before_last_hash[k], k
+    assert before_last_hash["neighbor_overflow"] > 0
 
 
 def test_small_window_build_equals_default(tmp_path):
