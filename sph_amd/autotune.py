@@ -8,7 +8,9 @@ spread dense regions of the fluid more evenly over the XCDs).  Both are
 compile-time shapes: sph_amd.build compiles one library per shape into
 sph_amd/variants/.  This tuner times every variant on a workload, each in a
 fresh process (SPH_AMD_LIB names the variant), and records the results and
-the fastest variant in configs/tuned.json.  sph_amd.lib() loads the variant
+the fastest variant in configs/tuned.json; the variant recorded before stays
+unless another beats it by more than --keep-within (0.5%, the spread of
+repeated timings), so that noise does not flip the choice.  sph_amd.lib() loads the variant
 recorded there for the default workload unless SPH_AMD_LIB names a library.
 Every variant computes the same bit-exact step; only speed differs.
 
@@ -59,7 +61,19 @@ def _worker(workload, steps, warmup):
     print(json.dumps({"ms_per_step": el * 1e3 / steps, "particle_steps_per_s": n * steps / el}))
 
 
-def tune(workload=DEFAULT_WORKLOAD, steps=100, warmup=20, timeout=300):
+def pick_best(results, incumbent=None, keep_within=0.005):
+    """The fastest variant, unless the incumbent (the variant recorded before)
+    is within keep_within (a fraction) of it: a challenger must beat the
+    incumbent by more than the run-to-run spread of one timing to replace it."""
+    best = min(results, key=lambda k: results[k]["ms_per_step"])
+    if incumbent in results and (results[incumbent]["ms_per_step"]
+                                 <= results[best]["ms_per_step"] * (1.0 + keep_within)):
+        return incumbent
+    return best
+
+
+def tune(workload=DEFAULT_WORKLOAD, steps=100, warmup=20, timeout=300, incumbent=None,
+         keep_within=0.005):
     from sph_amd import build as bd
     results = {}
     for cap, run in bd.VARIANTS:
@@ -75,8 +89,9 @@ def tune(workload=DEFAULT_WORKLOAD, steps=100, warmup=20, timeout=300):
             raise RuntimeError(f"variant {bd.variant_name(cap, run)} failed: {r.stderr[-2000:]}")
         results[bd.variant_name(cap, run)] = json.loads(r.stdout.strip().splitlines()[-1])
         print(bd.variant_name(cap, run), results[bd.variant_name(cap, run)], flush=True)
-    best = min(results, key=lambda k: results[k]["ms_per_step"])
-    return {"best": best, "steps": steps, "warmup": warmup, "results": results}
+    best = pick_best(results, incumbent, keep_within)
+    return {"best": best, "steps": steps, "warmup": warmup, "results": results,
+            "keep_within": keep_within}
 
 
 def load_tuned():
@@ -93,12 +108,17 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--worker", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--out", default=TUNED)
+    ap.add_argument("--keep-within", type=float, default=0.005,
+                    help="keep the recorded variant unless another is this fraction faster")
     args = ap.parse_args(argv)
     if args.worker:
         _worker(args.workload, args.steps, args.warmup)
         return
-    res = tune(args.workload, args.steps, args.warmup)
-    cfg = load_tuned() if args.out == TUNED else {}
+    prev = load_tuned()
+    incumbent = prev.get("workloads", {}).get(args.workload, {}).get("best")
+    res = tune(args.workload, args.steps, args.warmup, incumbent=incumbent,
+               keep_within=args.keep_within)
+    cfg = prev if args.out == TUNED else {}
     cfg.setdefault("default", DEFAULT_WORKLOAD)
     cfg["comment"] = ("written by python -m sph_amd.autotune on one MI355X: ms per graph-replayed "
                       "step of each library variant (LDS tile capacity t, tiles per XCD run x) "

@@ -5,7 +5,9 @@
   instances of one kernel must not be confused;
 * every kernel kind the benchmark prices has its algorithmic bytes, its
   launches per step and, where the profiler names it differently, a symbol;
-* the step's algorithmic bytes are the survey's 192 + 56 c per particle.
+* the step's algorithmic bytes are the survey's 192 + 56 c per particle;
+* the autotuner keeps its recorded variant unless another is faster by more
+  than the timing spread.
 """
 import bench
 
@@ -47,3 +49,14 @@ def test_step_bytes_are_the_survey_count():
     assert bench.B_ALG == (192, 56)
     c = 960000 / 1048576
     assert abs(bench.B_ALG[0] + bench.B_ALG[1] * c - 243.27) < 0.01
+
+
+def test_autotuner_keeps_incumbent_within_noise():
+    from sph_amd import autotune as at
+    res = {"a.so": {"ms_per_step": 1.000}, "b.so": {"ms_per_step": 0.998},
+           "c.so": {"ms_per_step": 1.100}}
+    assert at.pick_best(res) == "b.so"
+    assert at.pick_best(res, incumbent="a.so") == "a.so"        # 0.2% slower: kept
+    assert at.pick_best(res, incumbent="c.so") == "b.so"        # 10% slower: replaced
+    assert at.pick_best(res, incumbent="gone.so") == "b.so"     # not measured: fastest
+    assert at.pick_best(res, incumbent="a.so", keep_within=0.001) == "b.so"
