@@ -136,26 +136,26 @@ struct TilePlan {
 // of one per particle (or per key).  Lanes of a group take consecutive slots
 // in lane order; the reorder ranks a cell by particle id, so any assignment
 // of distinct slots gives the same sorted order.  Call with every live lane.
+// The grouping loop reads each group's key with a scalar lane read (the lane
+// index is wave-uniform) and keeps only the lane's own group mask, from which
+// rank, count and leader follow after the loop.
 __device__ __forceinline__ unsigned wave_slot(unsigned *cnt, unsigned key)
 {
     const unsigned lane = threadIdx.x & 63u;
     unsigned long long pending = __ballot(1);
-    int leader = 0;
-    unsigned rank = 0, count = 0;
+    unsigned long long mine = 0; // live lanes with this lane's key
     while (pending) {
         const int first = __ffsll((long long)pending) - 1;
-        const unsigned k = __shfl(key, first, 64);
+        const unsigned k = (unsigned)__builtin_amdgcn_readlane((int)key, first);
         const unsigned long long same = __ballot(key == k);
-        if (key == k) {
-            leader = first;
-            rank = (unsigned)__popcll(same & ((1ull << lane) - 1ull));
-            count = (unsigned)__popcll(same);
-        }
+        mine = key == k ? same : mine;
         pending &= ~same;
     }
+    const int leader = __ffsll((long long)mine) - 1; // the group's lowest lane
+    const unsigned rank = (unsigned)__popcll(mine & ((1ull << lane) - 1ull));
     unsigned base = 0;
     if ((int)lane == leader)
-        base = atomicAdd(&cnt[key], count);
+        base = atomicAdd(&cnt[key], (unsigned)__popcll(mine));
     return __shfl(base, leader, 64) + rank;
 }
 
