@@ -1087,7 +1087,10 @@ void sph_launch_key_hist(hipStream_t st, unsigned slots, const sph_dev_params *P
     launch(k_key_hist, GRID(slots), kTPB, st, P, BS, POS, okey, oslot, cnt, C);
 }
 
-unsigned sph_scan_blocks(unsigned m) { return (m + kScanTile - 1) / kScanTile; }
+unsigned sph_scan_blocks(unsigned m)
+{
+    return (unsigned)(((unsigned long long)m + kScanTile - 1) / kScanTile); // m near 2^32: no wrap
+}
 
 size_t sph_scan_padded(unsigned m) { return (size_t)sph_scan_blocks(m) * kScanTile; }
 
