@@ -332,6 +332,46 @@ def _counter_means(db, counter):
     return out
 
 
+def counter_summary(db, last=0, simds=1024, xcds=8):
+    """Mean per dispatch of every counter of a rocprofv3 --pmc run, per kernel
+    (the last `last` dispatches when given), and the number of records each
+    counter has per dispatch.  rocprofv3 reports a counter summed over the
+    device: GRBM_GUI_ACTIVE over the `xcds` XCDs, so one XCD's busy cycles are
+    GRBM_GUI_ACTIVE / xcds.  Derived where the counters are present:
+    VALU issue = SQ_INSTS_VALU x 4 cycles (a wave64 instruction on a 16-lane
+    SIMD) / (busy cycles x simds); resident waves per SIMD = SQ_WAVE_CYCLES x 4
+    (it counts quad-cycles) / (busy cycles x simds)."""
+    import sqlite3
+    con = sqlite3.connect(db)
+    rows = con.execute("select dispatch_id, kernel_name, counter_name, value "
+                       "from counters_collection").fetchall()
+    ids = sorted({r[0] for r in rows})
+    keep = set(ids[-last:] if last else ids)
+    agg, disp, recs = {}, {}, {}
+    for d, k, cn, v in rows:
+        if d not in keep:
+            continue
+        name = re.sub(r"\(.*", "", k).replace("void ", "")
+        agg.setdefault(name, {}).setdefault(cn, 0.0)
+        agg[name][cn] += v
+        disp.setdefault(name, set()).add(d)
+        recs[(name, cn, d)] = recs.get((name, cn, d), 0) + 1
+    lines = []
+    for k in sorted(agg):
+        n = len(disp[k])
+        m = {cn: v / n for cn, v in agg[k].items()}
+        nrec = max(c for (kk, _, _), c in recs.items() if kk == k)
+        busy = m.get("GRBM_GUI_ACTIVE", 0.0) / xcds
+        line = (f"{k}  dispatches={n}  records/dispatch/counter={nrec}  "
+                + "  ".join(f"{cn}={v:.6g}" for cn, v in sorted(m.items())))
+        if busy and "SQ_INSTS_VALU" in m:
+            line += f"  | VALU issue {m['SQ_INSTS_VALU'] * 4 / (busy * simds):.3f}"
+        if busy and "SQ_WAVE_CYCLES" in m:
+            line += f"  waves/SIMD {m['SQ_WAVE_CYCLES'] * 4 / (busy * simds):.2f}"
+        lines.append(line)
+    return "\n".join(lines)
+
+
 def traffic_summary(fetch_db, write_db, copy_mb):
     """HBM bytes per launch of every kernel from a FETCH_SIZE and a WRITE_SIZE
     counter run (KiB per dispatch), each scaled by the factor that makes the
@@ -399,12 +439,18 @@ def main():
                     help="print a summary of a rocprofv3 results database and exit")
     ap.add_argument("--last", type=int, default=0,
                     help="with --profile-summary: keep the last N trace dispatches")
+    ap.add_argument("--counter-summary", metavar="DB",
+                    help="print per-kernel counter means of a rocprofv3 --pmc run "
+                         "(with --last N: its last N dispatches) and exit")
     ap.add_argument("--traffic-summary", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
                     help="print calibrated HBM traffic per kernel from two rocprofv3 --pmc "
                          "databases and exit (uses --calib-copy-mb)")
     args = ap.parse_args()
     if args.profile_summary:
         print(profile_summary(args.profile_summary, args.last))
+        return
+    if args.counter_summary:
+        print(counter_summary(args.counter_summary, args.last))
         return
     if args.traffic_summary:
         print(traffic_summary(*args.traffic_summary, args.calib_copy_mb or 256.0))

@@ -60,3 +60,26 @@ def test_autotuner_keeps_incumbent_within_noise():
     assert at.pick_best(res, incumbent="c.so") == "b.so"        # 10% slower: replaced
     assert at.pick_best(res, incumbent="gone.so") == "b.so"     # not measured: fastest
     assert at.pick_best(res, incumbent="a.so", keep_within=0.001) == "b.so"
+
+
+def test_counter_summary_means_and_derived_figures(tmp_path):
+    import sqlite3
+    db = tmp_path / "run_results.db"
+    con = sqlite3.connect(db)
+    con.execute("create table counters_collection "
+                "(dispatch_id int, kernel_name text, counter_name text, value real)")
+    rows = []
+    for d, k in ((1, "k_a(int)"), (2, "k_a(int)"), (3, "void k_b<2>(float*)")):
+        rows += [(d, k, "GRBM_GUI_ACTIVE", 1000.0), (d, k, "SQ_INSTS_VALU", 128000.0 * d),
+                 (d, k, "SQ_WAVE_CYCLES", 4096000.0)]
+    con.executemany("insert into counters_collection values (?,?,?,?)", rows)
+    con.commit()
+    out = bench.counter_summary(str(db), simds=1024, xcds=8).splitlines()
+    a = next(l for l in out if l.startswith("k_a "))
+    assert "dispatches=2" in a and "SQ_INSTS_VALU=192000" in a
+    # busy cycles per XCD 1000/8; VALU 192000*4/(125*1024); waves 4096000*4/(125*1024)
+    assert "VALU issue 6.000" in a and "waves/SIMD 128.00" in a
+    b = next(l for l in out if l.startswith("k_b<2> "))
+    assert "dispatches=1" in b
+    last = bench.counter_summary(str(db), last=1).splitlines()
+    assert len(last) == 1 and last[0].startswith("k_b<2> ")
