@@ -57,6 +57,27 @@ def test_c1_multistep_graph_bitexact():
     assert_bitexact(s.download(), g.get_state(), what="after 100 steps")
 
 
+def test_settled_dam_break_65k_bitexact_over_220_steps():
+    """The bench's regime at 1/16 of its size: a 256 x 256 lattice dam break
+    (65,536 particles, box 300 x 200) through 220 graph-replayed steps, as far
+    as the bench's timed window reaches, and compared at steps 110 and 220.
+    By then the column has collapsed into dense bottom rows: rows of several
+    32-candidate chunks (more than the four the relaxation replays), full LDS
+    tiles and cells of several tens of particles."""
+    tun = dam_break_tunables()
+    box = po.make_box(300, 200)
+    parts = sa.scene_lattice(256, 256, 0.5, 0.01, seed=3)
+    s, g = make_pair(parts, tun, box)
+    for upto in (110, 220):
+        print("steps to", upto, flush=True)
+        s.step(110)
+        g.step(110)
+        assert_bitexact(s.download(), g.get_state(), what=f"65k dam break, step {upto}")
+    c, oc = s.counters(), g.counters()
+    assert c["max_cell_count"] == oc["max_cell_count"]
+    assert c["nonfinite"] == 0 and c["bucket_overflow"] == oc["bucket_overflow"]
+
+
 def test_phases_bitexact():
     """Each reference function on its own (phase API) vs the oracle."""
     d = golden("phases64.npz")
