@@ -322,7 +322,6 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_c2_dam_break.txt")
 
 def _counter_means(db, counter):
     """Per kernel: list of per-dispatch values of one counter."""
-    import re
     import sqlite3
     con = sqlite3.connect(db)
     out = {}
