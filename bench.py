@@ -279,7 +279,6 @@ def profile_summary(db, last=0):
     trace, and the mean per dispatch of every collected counter (FETCH_SIZE
     and WRITE_SIZE are in KiB).  `last` keeps only the last N dispatches of
     the trace (the steady state)."""
-    import re
     import sqlite3
     con = sqlite3.connect(db)
     names = {r[0] for r in con.execute("select name from sqlite_master")}
