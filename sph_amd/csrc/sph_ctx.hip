@@ -94,6 +94,8 @@ void derive_params(sph_ctx *c)
         p.slosh_omega = S.slosh_omega;
         p.slosh_t0 = S.slosh_t0;
         p.frozen = S.frozen;
+        int e = 0;
+        p.h_pow2 = std::frexp(p.h, &e) == 0.5f && std::fpclassify(p.h_recip) == FP_NORMAL;
     }
 }
 

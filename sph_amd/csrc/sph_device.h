@@ -35,6 +35,8 @@ struct sph_dev_params {
     float slosh_omega; //   applied, like gravity, as v_x += -A sin(omega t) dt
     unsigned slosh_t0; //   with t = (steps since upload + slosh_t0) * dt
     int frozen;        // 1: the fused step leaves this scene's state unchanged
+    int h_pow2;        // 1: h is a power of two with a normal reciprocal, so that
+                       //    x / h == x * h_recip for every x (both round x 2^-k once)
 };
 
 // Horizontal box acceleration of a sloshing scene at the step counted by
@@ -63,7 +65,8 @@ struct sph_dev_counters {
 __device__ __forceinline__ unsigned sph_cell_key(float x, float y, const sph_dev_params &P,
                                                  unsigned long long *oor)
 {
-    float qx = x / P.h, qy = y / P.h;
+    const float qx = P.h_pow2 ? x * P.h_recip : x / P.h;
+    const float qy = P.h_pow2 ? y * P.h_recip : y / P.h;
     if (!(qx >= 0.0f && qy >= 0.0f && qx < 4294967296.0f && qy < 4294967296.0f)) {
         atomicAdd(oor, 1ull);
         return P.cell_base + P.ncells - 1;

@@ -78,6 +78,35 @@ def test_settled_dam_break_65k_bitexact_over_220_steps():
     assert c["nonfinite"] == 0 and c["bucket_overflow"] == oc["bucket_overflow"]
 
 
+@pytest.mark.parametrize("h", [0.75, 1.3])
+def test_smoothing_radius_other_than_one_bitexact(h):
+    """Cell keys divide positions by h.  For a power of two (h = 1 elsewhere in
+    these tests, 0.75 is not one) the kernels multiply by the exact reciprocal,
+    which rounds the same; any other h takes the division.  Both against the
+    oracle, 30 steps of a dam break at spacing h/2 in a box scaled by h."""
+    tun = dam_break_tunables(smoothing_radius=h)
+    box = po.make_box(60 * h, 40 * h)
+    parts = sa.scene_lattice(64, 64, 0.5 * h, 0.01 * h, seed=9)
+    s, g = make_pair(parts, tun, box)
+    s.step(30)
+    g.step(30)
+    assert_bitexact(s.download(), g.get_state(), what=f"h={h}, 30 steps")
+    keys = s.read_sorted("key")
+    assert np.all(np.diff(keys.astype(np.int64)) >= 0)
+
+
+def test_smoothing_radius_power_of_two_keys_match_division():
+    """h = 0.5 and h = 2: the multiplied keys against the oracle's divided ones."""
+    for h in (0.5, 2.0):
+        tun = dam_break_tunables(smoothing_radius=h)
+        box = po.make_box(60 * h, 40 * h)
+        parts = sa.scene_lattice(64, 64, 0.5 * h, 0.01 * h, seed=4)
+        s, g = make_pair(parts, tun, box)
+        s.step(10)
+        g.step(10)
+        assert_bitexact(s.download(), g.get_state(), what=f"h={h}, 10 steps")
+
+
 def test_phases_bitexact():
     """Each reference function on its own (phase API) vs the oracle."""
     d = golden("phases64.npz")
