@@ -5,7 +5,8 @@
   instances of one kernel must not be confused;
 * every kernel kind the benchmark prices has its algorithmic bytes, its
   launches per step and, where the profiler names it differently, a symbol;
-* the step's algorithmic bytes are the survey's 192 + 56 c per particle;
+* the step's algorithmic bytes are 192 + 56 c per particle-step (one read of
+  each phase's inputs and one write of its outputs, c = cells per particle);
 * the autotuner keeps its recorded variant unless another is faster by more
   than the timing spread.
 """
@@ -45,7 +46,7 @@ def test_kernel_tables_agree():
     assert bench.KERNEL_LAUNCHES["density"] == bench.KERNEL_LAUNCHES["relax"] == 1
 
 
-def test_step_bytes_are_the_survey_count():
+def test_step_bytes_per_particle_step():
     assert bench.B_ALG == (192, 56)
     c = 960000 / 1048576
     assert abs(bench.B_ALG[0] + bench.B_ALG[1] * c - 243.27) < 0.01
