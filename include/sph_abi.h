@@ -272,6 +272,15 @@ int sph_set_tunables(sph_ctx *ctx, const tunable_parameters *tunables);
 /* Run nsteps full steps (src/fluid.c:273-348 with MPI removed) through the
  * captured HIP graph.  Returns once the work is enqueued; see sph_sync. */
 int sph_step(sph_ctx *ctx, int nsteps);
+/* One full step whose first half (gravity, viscosity, predict:
+ * src/fluid.c:273-279) uses `first` and whose second half (hash, density,
+ * relaxation, velocity update, hash: src/fluid.c:310-341) uses `second`,
+ * applied to every scene.  This is the step on which the reference loop
+ * receives new tunables from its render rank (src/fluid.c:293-294, between
+ * predict_positions and hash_fluid).  Afterwards the context's tunables are
+ * `second`.  smoothing_radius cannot change. */
+int sph_step_tunables2(sph_ctx *ctx, const tunable_parameters *first,
+                       const tunable_parameters *second);
 /* Same, launching kernels one by one instead of replaying the graph. */
 int sph_step_eager(sph_ctx *ctx, int nsteps);
 int sph_sync(sph_ctx *ctx);
@@ -280,6 +289,11 @@ int sph_sync(sph_ctx *ctx);
  * out[id] for id in 0..n-1.  Fills every field the reference's loop leaves
  * behind after a step (x_prev/density/pressure from that step). */
 int sph_download(sph_ctx *ctx, fluid_particle *out);
+/* Positions only: xy_by_id[2*id] = x, xy_by_id[2*id + 1] = y (caller order
+ * for batches: scene offset + id).  One 8-byte record per particle crosses
+ * to the host, the fields the reference's loop reads between steps
+ * (src/fluid.c:356-363). */
+int sph_download_positions(sph_ctx *ctx, float *xy_by_id);
 /* Pack coordinates as the reference does for its renderer
  * (src/fluid.c:356-363): coords[2*id] = (2x/max_x - 1)*SHRT_MAX, same for y. */
 int sph_pack_coords_i16(sph_ctx *ctx, short *coords);
@@ -317,6 +331,16 @@ int sph_read_sorted(sph_ctx *ctx, int field, void *out);
 /* Cell table of the last hash: start[c], count[c] for c < cells. */
 int sph_read_cells(sph_ctx *ctx, unsigned int *start, unsigned int *count);
 int sph_num_cells(const sph_ctx *ctx, unsigned int *size_x, unsigned int *size_y);
+/* Neighbour sets of the last hash, by caller index (scene offset + id), for
+ * tests and tools: full_by_id[i] = number of particles within h of i (self
+ * excluded; the reference's pair test r*r <= h*h, src/hash.c:99, 185, 221);
+ * half_by_id[i] = length of i's half list in the reference
+ * (src/hash.c:168-240: same-cell partners of higher id and partners in the
+ * forward cells); ids_by_id[i*stride + k] = id of i's k-th neighbour in the
+ * canonical walk order, for k < min(full, stride).  The context must have
+ * been hashed since the last upload. */
+int sph_read_neighbors(sph_ctx *ctx, int stride, int *full_by_id, int *half_by_id,
+                       int *ids_by_id);
 
 /* Per-kernel timing: enqueue nsteps full steps kernel by kernel, with
  * pre-created HIP events recorded on the context stream before every kernel
@@ -381,9 +405,17 @@ void hash_halo(fluid_particle **fluid_particle_pointers, neighbor_grid_t *grid, 
 
 /* Shim controls (no reference counterpart). */
 int sph_shim_set_sync_interval(int steps); /* 0 = never copy back automatically */
+/* Which fields the copy back writes into the caller's particles:
+ * SPH_SYNC_ALL (default) or SPH_SYNC_POSITIONS (x and y only, the fields the
+ * reference's loop reads between steps, src/fluid.c:356-363). */
+#define SPH_SYNC_ALL 0
+#define SPH_SYNC_POSITIONS 1
+int sph_shim_set_sync_fields(int fields);
 int sph_shim_sync(void);                   /* copy device state to the caller now */
-int sph_shim_mark_host_dirty(void);        /* re-upload from the caller on next call */
-void sph_shim_release(void);               /* destroy the process-global context */
+/* The caller changed its particles (or their number, up to any size): the
+ * next call re-uploads them and hashes them, keeping viscosity on. */
+int sph_shim_mark_host_dirty(void);
+void sph_shim_release(void); /* destroy the process-global context, reset the controls */
 sph_ctx *sph_shim_context(void);
 
 /* ------------------------------------------------------------------------

@@ -223,11 +223,11 @@ class GatherSim:
         return keys, order, start
 
     def counters(self):
-        out = np.zeros(5, np.uint64)
+        out = np.zeros(6, np.uint64)
         self.L.gather_counters(self.h, _ptr(out))
         return dict(bucket_overflow=int(out[0]), neighbor_overflow=int(out[1]),
                     key_out_of_range=int(out[2]), max_cell_count=int(out[3]),
-                    max_neighbors=int(out[4]))
+                    max_neighbors=int(out[4]), impulse_clamps=int(out[5]))
 
     def neighbor_sets(self, stride=128):
         counts = np.zeros(self.n, np.int32)

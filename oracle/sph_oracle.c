@@ -193,9 +193,11 @@ static REAL relax_D(REAL dt, REAL Psum, REAL Pnsum, REAL OmR, REAL k_spring, REA
 }
 
 /* Viscosity impulse of pair (p,q) evaluated with p in the owner role,
- * src/fluid.c:442-459.  Returns 0 when no impulse applies (u <= 0). */
+ * src/fluid.c:442-459.  Returns 0 when no impulse applies (u <= 0).  When
+ * `clamps` is given, it counts the impulses whose clamp to +-5
+ * (src/fluid.c:459) changed a component. */
 static int visc_impulse(REAL px, REAL py, REAL pvx, REAL pvy, REAL qx, REAL qy, REAL qvx, REAL qvy,
-                        const o_tun *t, REAL h_recip, REAL *imp_x, REAL *imp_y)
+                        const o_tun *t, REAL h_recip, REAL *imp_x, REAL *imp_y, uint64_t *clamps)
 {
     REAL QmP_x = qx - px;
     REAL QmP_y = qy - py;
@@ -207,7 +209,10 @@ static int visc_impulse(REAL px, REAL py, REAL pvx, REAL pvy, REAL qx, REAL qy, 
         REAL imp = t->dt * (RC(1.0) - ratio) * (t->sigma * u + t->beta * u * u);
         *imp_x = imp * QmP_x * r_recip;
         *imp_y = imp * QmP_y * r_recip;
+        REAL ux = *imp_x, uy = *imp_y;
         check_velocity(imp_x, imp_y);
+        if (clamps && (ux != *imp_x || uy != *imp_y))
+            ++*clamps;
         return 1;
     }
     return 0;
@@ -299,7 +304,7 @@ static void ref_viscosity(oref *s, o_particle *P)
             o_particle *q = &P[s->nb[(size_t)i * s->max_neighbors + j]];
             REAL ix, iy;
             if (visc_impulse(px, py, p->v_x, p->v_y, q->x, q->y, q->v_x, q->v_y, &s->t, h_recip,
-                             &ix, &iy)) {
+                             &ix, &iy, NULL)) {
                 p->v_x -= ix * RC(0.5);
                 p->v_y -= iy * RC(0.5);
                 q->v_x += ix * RC(0.5);
@@ -536,6 +541,7 @@ typedef struct {
     int lists_valid;
     uint64_t bucket_overflow, neighbor_overflow, key_oor;
     unsigned max_cell, max_full_neighbors;
+    uint64_t impulse_clamps; /* pair impulses clamped to +-5 (each end counts) */
 } ogat;
 
 ogat *OFN(gather_create)(const AABB_t *box, const tunable_parameters *tp, int n)
@@ -708,7 +714,7 @@ static void gather_viscosity(ogat *s)
                     continue;
                 REAL ix, iy;
                 if (visc_impulse(ax, ay, avx, avy, s->x[b], s->y[b], s->vx[b], s->vy[b], &s->t,
-                                 h_recip, &ix, &iy)) {
+                                 h_recip, &ix, &iy, &s->impulse_clamps)) {
                     nvx -= ix * RC(0.5);
                     nvy -= iy * RC(0.5);
                 }
@@ -873,13 +879,14 @@ void OFN(gather_cells)(const ogat *s, unsigned *key_by_id, int *sorted_ids, unsi
         memcpy(start, s->start, sizeof(unsigned) * (s->ncells + 1));
 }
 
-void OFN(gather_counters)(const ogat *s, uint64_t out[5])
+void OFN(gather_counters)(const ogat *s, uint64_t out[6])
 {
     out[0] = s->bucket_overflow;
     out[1] = s->neighbor_overflow;
     out[2] = s->key_oor;
     out[3] = s->max_cell;
     out[4] = s->max_full_neighbors;
+    out[5] = s->impulse_clamps;
 }
 
 /* Full (symmetric) neighbour sets at the current positions, from the last

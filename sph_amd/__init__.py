@@ -127,6 +127,14 @@ def lib():
             "sph_set_tunables": (ip, [vp, ctypes.POINTER(Tunables)]),
             "sph_step": (ip, [vp, ip]),
             "sph_step_eager": (ip, [vp, ip]),
+            "sph_step_tunables2": (ip, [vp, ctypes.POINTER(Tunables), ctypes.POINTER(Tunables)]),
+            "sph_download_positions": (ip, [vp, vp]),
+            "sph_shim_set_sync_interval": (ip, [ip]),
+            "sph_shim_set_sync_fields": (ip, [ip]),
+            "sph_shim_sync": (ip, []),
+            "sph_shim_mark_host_dirty": (ip, []),
+            "sph_shim_release": (None, []),
+            "sph_shim_context": (vp, []),
             "sph_sync": (ip, [vp]),
             "sph_download": (ip, [vp, vp]),
             "sph_pack_coords_i16": (ip, [vp, vp]),
@@ -322,6 +330,18 @@ class Solver:
 
     def sync(self):
         _check(lib().sph_sync(self._h), "sph_sync")
+
+    def step_tunables2(self, first: Tunables, second: Tunables):
+        """One step with `first` for gravity / viscosity / predict and
+        `second` for the rest (sph_step_tunables2)."""
+        _check(lib().sph_step_tunables2(self._h, ctypes.byref(first), ctypes.byref(second)),
+               "sph_step_tunables2")
+        self.tunables = second
+
+    def download_positions(self) -> np.ndarray:
+        out = np.zeros((self.n, 2), np.float32)
+        _check(lib().sph_download_positions(self._h, out.ctypes.data), "sph_download_positions")
+        return out
 
     def download(self) -> np.ndarray:
         out = np.zeros(self.n, PARTICLE)

@@ -99,11 +99,30 @@ void derive_params(sph_ctx *c)
     }
 }
 
+// Derive and copy the tunables of every scene to both device parameter
+// arrays.  The caller has synchronised the context stream (the copy runs on
+// the null stream, which does not wait for the non-blocking context stream).
 int push_params(sph_ctx *c)
 {
     derive_params(c);
-    HIPCHK(hipMemcpy(c->dP, c->hp.data(), sizeof(sph_dev_params) * c->hp.size(),
+    const size_t b = sizeof(sph_dev_params) * c->hp.size();
+    HIPCHK(hipMemcpy(c->dP, c->hp.data(), b, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->dPA, c->hp.data(), b, hipMemcpyHostToDevice));
+    c->dPA_stale = false;
+    return SPH_OK;
+}
+
+// After a step with split tunables the first-half array still holds the
+// first half's values: bring it back to the current ones before the next
+// fused step.
+int refresh_first_half_params(sph_ctx *c)
+{
+    if (!c->dPA_stale)
+        return SPH_OK;
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipMemcpy(c->dPA, c->hp.data(), sizeof(sph_dev_params) * c->hp.size(),
                      hipMemcpyHostToDevice));
+    c->dPA_stale = false;
     return SPH_OK;
 }
 
@@ -159,7 +178,7 @@ void enqueue_fused_step(sph_ctx *c, bool visc)
     const unsigned S = c->slots;
     {
         Tick t(c, KV);
-        sph_launch_visc_predict(c->st, S, true, visc, true, c->dP, c->BS, c->POS0, c->VEL, c->K0,
+        sph_launch_visc_predict(c->st, S, true, visc, true, c->dPA, c->BS, c->POS0, c->VEL, c->K0,
                                 c->start, c->UPOS, c->ukey, c->uslot, c->cnt, c->dC, c->dclock,
                                 c->UVEL);
     }
@@ -335,7 +354,8 @@ int sph_create_batch(sph_ctx **out, const sph_scene_desc *scenes, int nscenes)
     A(ID0, N); A(ID1, N); A(perm_src, N); A(perm_id, N); A(pos_of_id, N);
     A(K0, N); A(K1, N); A(ukey, N); A(uslot, N);
     A(cnt, MP); A(start, MP); A(bsum, sph_scan_blocks((unsigned)M));
-    A(coords, 2 * N); A(MASK, N); A(dP, (size_t)nscenes); A(BS, N / kSphTPB); A(dC, 1); A(dclock, 1);
+    A(coords, 2 * N); A(MASK, N); A(dP, (size_t)nscenes); A(dPA, (size_t)nscenes);
+    A(BS, N / kSphTPB); A(dC, 1); A(dclock, 1);
 #undef A
     if (!rc) {
         hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
@@ -394,7 +414,8 @@ void sph_destroy(sph_ctx *c)
                     c->perm_id, c->pos_of_id, c->K0, c->K1,    c->ukey,      c->uslot,
                     c->cnt,  c->start, c->bsum,    c->coords,  c->dP,        c->BS,
                     c->dC,   c->dclock, c->SPOS,    c->SVEL,    c->SPREV,     c->SRHO,
-                    c->SID,  c->SK,    c->Sstart,  c->Sclock,  c->dflags, c->MASK};
+                    c->SID,  c->SK,    c->Sstart,  c->Sclock,  c->dflags, c->MASK,
+                    c->dPA,  c->XY};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -612,6 +633,8 @@ int sph_step_eager(sph_ctx *c, int nsteps)
         return set_err(SPH_EINVAL, "nsteps < 0");
     if (c->n_total == 0)
         return SPH_OK;
+    if (int rc = refresh_first_half_params(c))
+        return rc;
     for (int i = 0; i < nsteps; i++) {
         enqueue_fused_step(c, c->lists_valid);
         c->lists_valid = true;
@@ -639,6 +662,8 @@ int sph_step(sph_ctx *c, int nsteps)
     }
     if (nsteps == 0)
         return SPH_OK;
+    if (int rc = refresh_first_half_params(c))
+        return rc;
     if (int rc = ensure_graphs(c))
         return rc;
     while (nsteps >= kGraphSteps) {
@@ -662,6 +687,60 @@ int sph_sync(sph_ctx *c)
         return rc;
     HIPCHK(hipStreamSynchronize(c->st));
     return SPH_OK;
+}
+
+// One fused step whose first kernel (gravity, viscosity, predict) reads
+// `first` and the rest (hash, density, relaxation, velocity update) reads
+// `second`: the first-half array is loaded with `first`, the second with
+// `second`, one step is enqueued, and the context keeps `second` (the
+// first-half array is refreshed before the next fused step).
+int sph_step_tunables2(sph_ctx *c, const tunable_parameters *first,
+                       const tunable_parameters *second)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (!first || !second)
+        return set_err(SPH_EINVAL, "sph_step_tunables2: null tunables");
+    for (const Scene &S : c->sc)
+        if (first->smoothing_radius != S.tun.smoothing_radius ||
+            second->smoothing_radius != S.tun.smoothing_radius)
+            return set_err(SPH_EINVAL,
+                           "sph_step_tunables2: smoothing_radius fixes the grid and cannot change");
+    if (memcmp(first, second, sizeof *first) == 0) {
+        if (int rc = set_tunables_impl(c, -1, second))
+            return rc;
+        return sph_step(c, 1);
+    }
+    if (c->n_total == 0) {
+        return set_tunables_impl(c, -1, second);
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (Scene &S : c->sc)
+        S.tun = *first;
+    derive_params(c);
+    const size_t b = sizeof(sph_dev_params) * c->hp.size();
+    HIPCHK(hipMemcpy(c->dPA, c->hp.data(), b, hipMemcpyHostToDevice));
+    for (Scene &S : c->sc)
+        S.tun = *second;
+    derive_params(c);
+    HIPCHK(hipMemcpy(c->dP, c->hp.data(), b, hipMemcpyHostToDevice));
+    c->dPA_stale = false; // dPA holds `first` on purpose for this one step
+    int rc;
+    if (!c->lists_valid) {
+        rc = sph_step_eager(c, 1);
+    } else {
+        rc = ensure_graphs(c);
+        if (!rc) {
+            hipError_t e = hipGraphLaunch(c->g1, c->st);
+            if (e != hipSuccess)
+                rc = set_err(SPH_EHIP, "hipGraphLaunch failed: %s", hipGetErrorString(e));
+            c->steps++;
+            c->hashes += 2;
+            c->aux_order = 1;
+        }
+    }
+    c->dPA_stale = true; // hp now holds `second`; dPA must follow before the next step
+    return rc;
 }
 
 // ---- phase-level calls (reference functions one by one) -------------------
@@ -923,6 +1002,25 @@ int sph_download(sph_ctx *c, fluid_particle *out)
     return SPH_OK;
 }
 
+int sph_download_positions(sph_ctx *c, float *xy_by_id)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (c->n_total > 0 && !xy_by_id)
+        return set_err(SPH_EINVAL, "sph_download_positions: null output");
+    if (c->n_total == 0)
+        return SPH_OK;
+    if (!c->XY)
+        if (int rc = dalloc(&c->XY, (size_t)c->slots))
+            return rc;
+    sph_launch_gather_xy(c->st, c->slots, c->dP, c->BS, c->POS0, c->ID0, c->XY);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(xy_by_id, c->XY, sizeof(float2) * (size_t)c->n_total,
+                          hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return SPH_OK;
+}
+
 int sph_pack_coords_i16(sph_ctx *c, short *coords)
 {
     if (int rc = check_ctx(c))
@@ -1076,6 +1174,8 @@ int sph_time_kernels(sph_ctx *c, int nsteps, int max_kinds, const char **names, 
                        (int)NKIND);
     if (c->n_total == 0)
         return set_err(SPH_ESTATE, "sph_time_kernels: no particles");
+    if (int rc = refresh_first_half_params(c))
+        return rc;
     if (!c->lists_valid) {
         int rc = sph_step_eager(c, 1);
         if (rc)

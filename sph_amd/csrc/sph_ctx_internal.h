@@ -32,6 +32,13 @@ struct sph_ctx {
     int n_total = 0;
     std::vector<sph_dev_params> hp;
     sph_dev_params *dP = nullptr;
+    // parameters of the fused step's first kernel (gravity, viscosity,
+    // predict); equal to dP except after sph_step_tunables2, which gives the
+    // two halves of a step different tunables (dPA_stale: dPA still holds
+    // the first half's and must be refreshed before the next fused step)
+    sph_dev_params *dPA = nullptr;
+    bool dPA_stale = false;
+    float2 *XY = nullptr; // positions by id for sph_download_positions (allocated on first use)
     int *BS = nullptr; // workgroup -> scene
     sph_dev_counters *dC = nullptr;
     unsigned long long *dclock = nullptr; // fused steps since upload (sloshing time)

@@ -91,6 +91,9 @@ void sph_launch_gather_state(hipStream_t st, unsigned slots, const sph_dev_param
 void sph_launch_load_state(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
                            const float4 *in, float2 *POS, float2 *VEL, float2 *PREV, float2 *RHO,
                            int *ID);
+// Positions (x, y) by caller index uoff + id.
+void sph_launch_gather_xy(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                          const float2 *POS, const int *ID, float2 *out);
 
 // Time the next kernel launch of this thread with a start/stop event pair
 // (recorded by the launch itself: the kernel's own duration).

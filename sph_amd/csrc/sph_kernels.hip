@@ -1018,6 +1018,19 @@ k_gather_state(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
     out[P.uoff + (unsigned)ID[s]] = make_float4(p.x, p.y, v.x, v.y);
 }
 
+// Positions by caller index (scene offset + id), for position-only read-back.
+__global__ void __launch_bounds__(kTPB)
+k_gather_xy(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+            const float2 *__restrict__ POS, const int *__restrict__ ID, float2 *__restrict__ out)
+{
+    const unsigned blk = xcd_block();
+    const sph_dev_params &P = Pp[BS[blk]];
+    const unsigned s = blk * kTPB + threadIdx.x;
+    if (s >= P.poff + (unsigned)P.n)
+        return;
+    out[P.uoff + (unsigned)ID[s]] = POS[s];
+}
+
 __global__ void __launch_bounds__(kTPB)
 k_load_state(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
              const float4 *__restrict__ in, float2 *__restrict__ POS, float2 *__restrict__ VEL,
@@ -1193,6 +1206,12 @@ void sph_launch_gather_state(hipStream_t st, unsigned slots, const sph_dev_param
                              float4 *out)
 {
     launch(k_gather_state, GRID(slots), kTPB, st, P, BS, POS, VEL, ID, out);
+}
+
+void sph_launch_gather_xy(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                          const float2 *POS, const int *ID, float2 *out)
+{
+    launch(k_gather_xy, GRID(slots), kTPB, st, P, BS, POS, ID, out);
 }
 
 void sph_launch_load_state(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,

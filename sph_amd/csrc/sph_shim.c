@@ -6,15 +6,37 @@
  *
  * One process-global context serves the calls, as the reference keeps one
  * simulation per process.  It is created by the first call that carries the
- * box (predict_positions), uploading the caller's particles through the
- * pointer array; calls before that (the first apply_gravity of a run) run
- * on the host with the reference's arithmetic, and the first viscosity pass
- * is a no-op exactly as in the reference, whose lists start empty.
- * hash_fluid(..., false) ends a step (src/fluid.c:341); every
- * sync_interval steps it copies the device state back into the caller's
- * particles, so code that reads p->x afterwards (src/fluid.c:356-363) sees
- * the current state.  neighbor / bucket storage is left untouched: the
- * neighbour sets live on the device.
+ * box (predict_positions), with room for the max_fluid_particles_local that
+ * initParticles was given (the reference sizes it at twice the global count,
+ * src/fluid.c:156), uploading the caller's particles through the pointer
+ * array; calls before that (the first apply_gravity of a run) run on the
+ * host with the reference's arithmetic, and the first viscosity pass is a
+ * no-op exactly as in the reference, whose lists start empty.
+ *
+ * The step's calls are recorded, not run one by one: when a step's calls
+ * arrive in the reference's order (apply_gravity, viscosity_impluses,
+ * predict_positions, hash_fluid(true), double_density_relaxation,
+ * updateVelocities, hash_fluid(false); identify_oob_particles and hash_halo
+ * are no-ops on one domain), the last one runs the whole step as one fused,
+ * graph-replayed step - with the tunables of the first three calls for the
+ * first half and those of the last four for the second, because the
+ * reference receives new tunables between predict_positions and hash_fluid
+ * (src/fluid.c:293-294).  Any other order, or a call that needs the device
+ * state mid-step, replays the recorded calls phase by phase (the same
+ * values) and runs the rest of that step the same way.
+ *
+ * hash_fluid(..., false) ends a step (src/fluid.c:341); every sync_interval
+ * steps it copies the device state back into the caller's particles, so
+ * code that reads p->x afterwards (src/fluid.c:356-363) sees the current
+ * state: all fields, or only x and y (sph_shim_set_sync_fields), which is
+ * all the reference's loop reads.  neighbor / bucket storage is left
+ * untouched: the neighbour sets live on the device.
+ *
+ * After sph_shim_mark_host_dirty() the next call re-uploads the caller's
+ * particles (growing the context when their number exceeds its capacity)
+ * and hashes them, so the next viscosity pass has neighbour lists, as the
+ * reference keeps its lists across such a change.  initParticles starts a
+ * new run: its upload leaves the lists empty.
  *
  * Errors keep the reference's void returns; the message is available from
  * sph_last_error() and is also printed once to stderr.
@@ -26,13 +48,25 @@
 
 #include "../../include/sph_abi.h"
 
+enum { DIRTY_NONE = 0, DIRTY_KEEP_LISTS = 1, DIRTY_NEW_RUN = 2 };
+/* The recorded calls of a step, in the reference's order. */
+enum { CALL_GRAVITY = 1, CALL_VISCOSITY, CALL_PREDICT, CALL_HASH_DENSITY, CALL_RELAX,
+       CALL_UPDATE_VELOCITIES, CALL_HASH };
+
 static sph_ctx *g_ctx;
 static fluid_particle **g_ptrs;
 static int g_n;
-static int g_dirty = 1;
+static int g_capacity; /* max_fluid_particles_local given to initParticles */
+static AABB_t g_box;   /* the box the context was created with */
+static int g_dirty = DIRTY_NEW_RUN;
 static int g_sync_interval = 1;
+static int g_sync_fields = SPH_SYNC_ALL;
 static int g_steps_since_sync;
 static int g_reported;
+/* calls of the current step recorded so far (0: none; -1: this step runs
+ * phase by phase), and the tunables of its two halves */
+static int g_seq;
+static tunable_parameters g_first, g_second;
 
 static void shim_fail(const char *where, int rc)
 {
@@ -42,47 +76,156 @@ static void shim_fail(const char *where, int rc)
     }
 }
 
-/* Make the context match the caller's particles and tunables. */
-static int shim_ready(fluid_particle **ptrs, param *params, AABB_t *box)
+/* Make the context hold the caller's particles: create it (needs the box),
+ * grow it, or re-upload, as needed. */
+static int shim_upload(fluid_particle **ptrs, param *params, AABB_t *box)
 {
     int n = params->number_fluid_particles_local;
     int rc = SPH_OK;
+    if (g_ctx && n > g_capacity) { /* more particles than the context holds: recreate it */
+        sph_destroy(g_ctx);
+        g_ctx = NULL;
+        if (!box)
+            box = &g_box;
+    }
     if (!g_ctx) {
         if (!box)
             return SPH_ESTATE;
-        rc = sph_create(&g_ctx, params, box, n);
+        int cap = g_capacity > n ? g_capacity : n;
+        AABB_t b = *box;
+        rc = sph_create(&g_ctx, params, &b, cap);
         if (rc)
             return rc;
-        g_dirty = 1;
+        g_capacity = cap;
+        g_box = b;
+        if (g_dirty == DIRTY_NONE)
+            g_dirty = DIRTY_KEEP_LISTS;
     }
     if (n != g_n || ptrs != g_ptrs)
-        g_dirty = 1;
-    if (g_dirty) {
-        rc = sph_upload_pointers(g_ctx, ptrs, n);
+        if (g_dirty == DIRTY_NONE)
+            g_dirty = DIRTY_KEEP_LISTS;
+    if (g_dirty != DIRTY_NONE) {
+        int keep = g_dirty == DIRTY_KEEP_LISTS;
+        rc = sph_set_tunables(g_ctx, &params->tunable_params);
+        if (!rc)
+            rc = sph_upload_pointers(g_ctx, ptrs, n);
+        if (!rc && keep) /* lists of the uploaded positions for the next viscosity pass */
+            rc = sph_phase_hash(g_ctx, 0);
         if (rc)
             return rc;
         g_ptrs = ptrs;
         g_n = n;
-        g_dirty = 0;
+        g_dirty = DIRTY_NONE;
         g_steps_since_sync = 0;
     }
-    return sph_set_tunables(g_ctx, &params->tunable_params);
+    return SPH_OK;
+}
+
+/* Run recorded call `call` as its phase, with tunables t. */
+static int run_phase(int call, const tunable_parameters *t)
+{
+    int rc = sph_set_tunables(g_ctx, t);
+    if (rc)
+        return rc;
+    switch (call) {
+    case CALL_GRAVITY: return sph_phase_gravity(g_ctx);
+    case CALL_VISCOSITY: return sph_phase_viscosity(g_ctx);
+    case CALL_PREDICT: return sph_phase_predict(g_ctx);
+    case CALL_HASH_DENSITY: return sph_phase_hash(g_ctx, 1);
+    case CALL_RELAX: return sph_phase_relax(g_ctx);
+    case CALL_UPDATE_VELOCITIES: return sph_phase_update_velocities(g_ctx);
+    case CALL_HASH: return sph_phase_hash(g_ctx, 0);
+    default: return SPH_EINVAL;
+    }
+}
+
+/* Run the calls recorded so far phase by phase; the rest of the step
+ * follows the same way. */
+static int shim_flush(void)
+{
+    int rc = SPH_OK;
+    for (int call = CALL_GRAVITY; call <= g_seq && !rc; call++)
+        rc = run_phase(call, call <= CALL_PREDICT ? &g_first : &g_second);
+    g_seq = -1;
+    return rc;
+}
+
+/* A step function's entry: make the context current, then either record
+ * the call (returns 1 in *deferred) or leave it to run as a phase. */
+static int shim_enter(int call, fluid_particle **ptrs, param *params, AABB_t *box, int *deferred)
+{
+    *deferred = 0;
+    if (g_seq > 0 && g_dirty != DIRTY_NONE) { /* the host changed data mid-step */
+        int rc = shim_flush();
+        if (rc)
+            return rc;
+    }
+    int rc = shim_upload(ptrs, params, box);
+    if (rc)
+        return rc;
+    const tunable_parameters *t = &params->tunable_params;
+    if (call == CALL_GRAVITY) {
+        if (g_seq > 0 && (rc = shim_flush()))
+            return rc;
+        g_seq = 0;
+    }
+    if (g_seq < 0)
+        return sph_set_tunables(g_ctx, t);
+    int same = call <= CALL_PREDICT ? call == CALL_GRAVITY || !memcmp(t, &g_first, sizeof *t)
+                                    : call == CALL_HASH_DENSITY || !memcmp(t, &g_second, sizeof *t);
+    if (call != g_seq + 1 || !same) {
+        if ((rc = shim_flush()))
+            return rc;
+        return sph_set_tunables(g_ctx, t);
+    }
+    if (call == CALL_GRAVITY)
+        g_first = *t;
+    if (call == CALL_HASH_DENSITY)
+        g_second = *t;
+    g_seq = call;
+    *deferred = 1;
+    return SPH_OK;
+}
+
+static int shim_sync_now(void)
+{
+    if (!g_ctx || !g_ptrs)
+        return SPH_OK;
+    int rc = SPH_OK;
+    size_t n = (size_t)(g_n ? g_n : 1);
+    if (g_sync_fields == SPH_SYNC_POSITIONS) {
+        float *xy = (float *)malloc(sizeof(float) * 2 * n);
+        if (!xy)
+            return SPH_ENOMEM;
+        rc = sph_download_positions(g_ctx, xy);
+        if (!rc)
+            for (int i = 0; i < g_n; i++) { /* pointer index == id on one domain */
+                g_ptrs[i]->x = xy[2 * i];
+                g_ptrs[i]->y = xy[2 * i + 1];
+            }
+        free(xy);
+    } else {
+        fluid_particle *tmp = (fluid_particle *)malloc(sizeof(fluid_particle) * n);
+        if (!tmp)
+            return SPH_ENOMEM;
+        rc = sph_download(g_ctx, tmp);
+        if (!rc)
+            for (int i = 0; i < g_n; i++)
+                *g_ptrs[i] = tmp[i];
+        free(tmp);
+    }
+    g_steps_since_sync = 0;
+    return rc;
 }
 
 int sph_shim_sync(void)
 {
-    if (!g_ctx || !g_ptrs)
-        return SPH_OK;
-    fluid_particle *tmp = (fluid_particle *)malloc(sizeof(fluid_particle) * (size_t)(g_n ? g_n : 1));
-    if (!tmp)
-        return SPH_ENOMEM;
-    int rc = sph_download(g_ctx, tmp);
-    if (!rc)
-        for (int i = 0; i < g_n; i++)
-            *g_ptrs[i] = tmp[i]; /* pointer index == id on one domain */
-    free(tmp);
-    g_steps_since_sync = 0;
-    return rc;
+    if (g_ctx && g_seq > 0) { /* recorded calls first: the host sees the state they leave */
+        int rc = shim_flush();
+        if (rc)
+            return rc;
+    }
+    return shim_sync_now();
 }
 
 int sph_shim_set_sync_interval(int steps)
@@ -93,9 +236,18 @@ int sph_shim_set_sync_interval(int steps)
     return SPH_OK;
 }
 
+int sph_shim_set_sync_fields(int fields)
+{
+    if (fields != SPH_SYNC_ALL && fields != SPH_SYNC_POSITIONS)
+        return SPH_EINVAL;
+    g_sync_fields = fields;
+    return SPH_OK;
+}
+
 int sph_shim_mark_host_dirty(void)
 {
-    g_dirty = 1;
+    if (g_dirty == DIRTY_NONE)
+        g_dirty = DIRTY_KEEP_LISTS;
     return SPH_OK;
 }
 
@@ -105,7 +257,12 @@ void sph_shim_release(void)
     g_ctx = NULL;
     g_ptrs = NULL;
     g_n = 0;
-    g_dirty = 1;
+    g_capacity = 0;
+    g_dirty = DIRTY_NEW_RUN;
+    g_seq = 0;
+    g_steps_since_sync = 0;
+    g_sync_interval = 1;
+    g_sync_fields = SPH_SYNC_ALL;
     g_reported = 0;
 }
 
@@ -227,7 +384,14 @@ void initParticles(fluid_particle **fluid_particle_pointers, fluid_particle *flu
         fluid_particle_pointers[k]->v_x = 0.0f;
         fluid_particle_pointers[k]->v_y = 0.0f;
     }
-    g_dirty = 1;
+    /* a new run: room for max_fluid_particles_local, lists empty */
+    if (max_fluid_particles_local > g_capacity) {
+        sph_destroy(g_ctx);
+        g_ctx = NULL;
+        g_capacity = max_fluid_particles_local;
+    }
+    g_dirty = DIRTY_NEW_RUN;
+    g_seq = 0;
 }
 
 /* ---- step functions on the device -------------------------------------- */
@@ -243,10 +407,12 @@ void apply_gravity(fluid_particle **fluid_particle_pointers, param *params)
             p->density = 0.0f;
             p->density_near = 0.0f;
         }
+        g_seq = -1;
         return;
     }
-    int rc = shim_ready(fluid_particle_pointers, params, NULL);
-    if (!rc)
+    int deferred;
+    int rc = shim_enter(CALL_GRAVITY, fluid_particle_pointers, params, NULL, &deferred);
+    if (!rc && !deferred)
         rc = sph_phase_gravity(g_ctx);
     shim_fail("apply_gravity", rc);
 }
@@ -258,8 +424,9 @@ void viscosity_impluses(fluid_particle **fluid_particle_pointers, neighbor *neig
     (void)neighbors;
     if (!g_ctx)
         return; /* no hash yet: the reference's lists are empty */
-    int rc = shim_ready(fluid_particle_pointers, params, NULL);
-    if (!rc)
+    int deferred;
+    int rc = shim_enter(CALL_VISCOSITY, fluid_particle_pointers, params, NULL, &deferred);
+    if (!rc && !deferred)
         rc = sph_phase_viscosity(g_ctx);
     shim_fail("viscosity_impluses", rc);
 }
@@ -268,8 +435,11 @@ void viscosity_impluses(fluid_particle **fluid_particle_pointers, neighbor *neig
 void predict_positions(fluid_particle **fluid_particle_pointers, AABB_t *boundary_global,
                        param *params)
 {
-    int rc = shim_ready(fluid_particle_pointers, params, boundary_global);
-    if (!rc)
+    if (!g_ctx)
+        g_seq = -1; /* the run's first step: phase by phase */
+    int deferred;
+    int rc = shim_enter(CALL_PREDICT, fluid_particle_pointers, params, boundary_global, &deferred);
+    if (!rc && !deferred)
         rc = sph_phase_predict(g_ctx);
     shim_fail("predict_positions", rc);
 }
@@ -293,13 +463,21 @@ void hash_fluid(fluid_particle **fluid_particle_pointers, neighbor_grid_t *grid,
                 bool compute_density)
 {
     (void)grid;
-    int rc = shim_ready(fluid_particle_pointers, params, NULL);
-    if (!rc)
+    int deferred;
+    int call = compute_density ? CALL_HASH_DENSITY : CALL_HASH;
+    int rc = shim_enter(call, fluid_particle_pointers, params, NULL, &deferred);
+    if (!rc && deferred && call == CALL_HASH) { /* a whole step recorded: run it fused */
+        rc = sph_step_tunables2(g_ctx, &g_first, &g_second);
+        g_seq = 0;
+    } else if (!rc && !deferred) {
         rc = sph_phase_hash(g_ctx, compute_density ? 1 : 0);
+    }
     if (!rc && !compute_density) { /* end of a step (src/fluid.c:341) */
+        if (g_seq < 0)
+            g_seq = 0;
         ++g_steps_since_sync;
         if (g_sync_interval > 0 && g_steps_since_sync >= g_sync_interval)
-            rc = sph_shim_sync();
+            rc = shim_sync_now();
     }
     shim_fail("hash_fluid", rc);
 }
@@ -319,8 +497,9 @@ void double_density_relaxation(fluid_particle **fluid_particle_pointers, neighbo
                                param *params)
 {
     (void)neighbors;
-    int rc = shim_ready(fluid_particle_pointers, params, NULL);
-    if (!rc)
+    int deferred;
+    int rc = shim_enter(CALL_RELAX, fluid_particle_pointers, params, NULL, &deferred);
+    if (!rc && !deferred)
         rc = sph_phase_relax(g_ctx);
     shim_fail("double_density_relaxation", rc);
 }
@@ -330,8 +509,10 @@ void updateVelocities(fluid_particle **fluid_particle_pointers, edge_t *edges,
                       AABB_t *boundary_global, param *params)
 {
     (void)edges;
-    int rc = shim_ready(fluid_particle_pointers, params, boundary_global);
-    if (!rc)
+    int deferred;
+    int rc = shim_enter(CALL_UPDATE_VELOCITIES, fluid_particle_pointers, params, boundary_global,
+                        &deferred);
+    if (!rc && !deferred)
         rc = sph_phase_update_velocities(g_ctx);
     shim_fail("updateVelocities", rc);
 }

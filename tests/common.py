@@ -59,3 +59,27 @@ def max_rel(got, want, floor=1e-6):
     g = np.asarray(got, np.float64)
     w = np.asarray(want, np.float64)
     return float(np.max(np.abs(g - w) / np.maximum(np.abs(w), floor))) if len(g) else 0.0
+
+
+# Phase indices of the reference loop body (src/fluid.c:273-341).
+FIRST_HALF = (po.PH_GRAVITY, po.PH_VISCOSITY, po.PH_PREDICT)
+SECOND_HALF = (po.PH_HASH_DENSITY, po.PH_RELAX, po.PH_UPDATE_VELOCITIES, po.PH_HASH)
+
+
+def run_reference_loop(phase, set_tunables, nsteps, frames, steps_per_frame=4, first_step=0):
+    """Drive phase(ph) through nsteps steps of the reference loop
+    (src/fluid.c:270-372): on the last substep of every frame the next of
+    `frames` (tunables) arrives between predict_positions and hash_fluid, as
+    the render rank's MPI_Scatterv delivers it (src/fluid.c:293-294).
+    first_step counts the steps run before (it sets the substep and the
+    frame).  Returns the frames consumed so far."""
+    frame = (first_step + 1) // steps_per_frame if first_step else 0
+    for step in range(first_step, first_step + nsteps):
+        for ph in FIRST_HALF:
+            phase(ph)
+        if step % steps_per_frame == steps_per_frame - 1 and frame < len(frames):
+            set_tunables(frames[frame])
+            frame += 1
+        for ph in SECOND_HALF:
+            phase(ph)
+    return frame

@@ -11,7 +11,15 @@ Fixtures:
   c1_dambreak.npz  N=4096 dam break (64x64 lattice at 0.5h, box 60x40,
                    rho0=10, k=0.004, k_near=0.01, k_spring=10, sigma=0,
                    beta=0.3, g=9.8, dt=1/60): state after steps 1, 2, 10, 100
-                   and the half-list lengths after step 2.
+                   and 1000, and the half-list lengths after step 2.
+  presets.npz      the reference's own demo scene (src/fluid.c:88-159: 1508
+                   particles filling a 15 x 8.4375 box, h = 2 spacing, dt =
+                   1/120, sphere mover of width 2) under each of its presets
+                   x / y / a / b (src/controls.c:344-401): state after steps 2
+                   and 20; and the reference loop with per-frame tunables
+                   (mover on the attract-mode path, presets taking turns,
+                   arriving mid-step as src/fluid.c:293-294 receives them):
+                   state after steps 48 and 96.
   phases64.npz     N=64 scene with viscosity on (sigma=0.1): state after every
                    phase of 8 steps, and the half lists after each hash.
   movers.npz       N=400 with a sphere mover, then with a rectangle mover:
@@ -69,13 +77,45 @@ def c1():
     ref.load(p0)
     out = {"input": snap(p0), "tunables": tun_array(t), "box": np.array([60, 40], np.float32)}
     done = 0
-    for s in (1, 2, 10, 100):
+    for s in (1, 2, 10, 100, 1000):
         ref.step(s - done)
         done = s
         out[f"step{s}"] = snap(ref.store())
         if s == 2:
             out["halflist_len_step2"] = ref.lists()[0]
     np.savez_compressed(os.path.join(HERE, "c1_dambreak.npz"), **out)
+
+
+def presets():
+    from sph_amd import scenes as sc
+    from tests.common import SECOND_HALF, FIRST_HALF, run_reference_loop  # noqa: F401
+    out = {}
+    for name in ("x", "y", "a", "b"):
+        t, box, p0, info = sc.reference_scene(name)
+        p0 = np.ascontiguousarray(p0).view(po.P32)
+        if "input" not in out:
+            out["input"] = snap(p0)
+            out["box"] = np.array([box.max_x, box.max_y], np.float32)
+        ref = po.CompiledRef(box, t, len(p0))
+        ref.load(p0)
+        ref.step(2)
+        out[f"{name}_step2"] = snap(ref.store())
+        ref.step(18)
+        out[f"{name}_step20"] = snap(ref.store())
+        out[f"{name}_tunables"] = tun_array(t)
+    # the reference loop with the render rank's per-frame tunables
+    t, box, p0, info = sc.reference_scene("x")
+    p0 = np.ascontiguousarray(p0).view(po.P32)
+    frames = sc.reference_frames(t, box, 24)
+    ref = po.CompiledRef(box, t, len(p0))
+    ref.load(p0)
+    run_reference_loop(ref.phase, ref.set_tunables, 48, frames)
+    out["frames_step48"] = snap(ref.store())
+    run_reference_loop(ref.phase, ref.set_tunables, 48, frames, first_step=48)
+    out["frames_step96"] = snap(ref.store())
+    out["frames_tunables"] = np.stack([tun_array(f) for f in frames])
+    out["frames_initial_tunables"] = tun_array(t)
+    np.savez_compressed(os.path.join(HERE, "presets.npz"), **out)
 
 
 def phases64():
@@ -153,10 +193,11 @@ def caps():
 if __name__ == "__main__":
     if not po.have_compiled_ref():
         sys.exit("oracle/_ref/libtinysph_ref.so is not built (make -C oracle ref)")
-    c1()
-    phases64()
-    movers()
-    caps()
+    which = set(sys.argv[1:]) or {"c1", "phases64", "movers", "caps", "presets"}
+    for name, fn in (("c1", c1), ("phases64", phases64), ("movers", movers), ("caps", caps),
+                     ("presets", presets)):
+        if name in which:
+            fn()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")

@@ -68,6 +68,17 @@ def test_library_exports_every_declared_function():
         getattr(L, n)
 
 
+def test_every_exported_c_symbol_is_declared():
+    """The reverse direction: every C-linkage entry point the library exports
+    (sph_* and the reference-named ones) is declared in the header, so the
+    header is the whole boundary."""
+    exported = exported_symbols(sa.LIB_PATH)
+    declared = set(declared_functions())
+    c_symbols = {s for s in exported if not s.startswith("_") and "@" not in s}
+    undeclared = sorted(c_symbols - declared)
+    assert not undeclared, undeclared
+
+
 def test_every_tuning_variant_exports_the_abi():
     """The autotuner's library variants (other launch shapes of the gather
     kernels) are drop-in replacements: same exported entry points."""

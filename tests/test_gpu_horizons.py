@@ -1,0 +1,120 @@
+"""Parity over the configurations' whole 1000-step horizons.
+
+* N = 4096 dam break (the CPU-runnable configuration): bit-exact against
+  the oracle's gather form at steps 250, 500 and 1000, and at step 1000
+  within the ensemble bounds of the reference's own step order - both the
+  oracle's restatement of it and the compiled reference's step-1000 state
+  kept in tests/golden/c1_dambreak.npz.
+* N = 1,048,576 dam break (the benchmark's workload, seed 12345): after
+  250, 500 and 1000 graph-replayed steps every field of the state, in id
+  order, has the SHA-256 that the oracle's gather form gives
+  (tests/golden/c2_digests.json, written by make_c2_digests.py), and the
+  cell occupancy counters agree - including whether the reference's
+  100-particle buckets would have overflowed (src/fluid.c:174).
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import sph_amd as sa
+from oracle import pyoracle as po
+from tests.common import GOLDEN, assert_bitexact, golden, parts_from_snap, tun_from_bytes
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("x", "y", "v_x", "v_y", "x_prev", "y_prev", "density", "density_near")
+
+
+def ensemble(st):
+    rho = np.asarray(st["density"], np.float64)
+    ke = 0.5 * float(np.sum(np.asarray(st["v_x"], np.float64) ** 2 +
+                            np.asarray(st["v_y"], np.float64) ** 2))
+    hist = np.histogram(rho, bins=12, range=(0, 12))[0] / len(rho)
+    return dict(mean_rho=float(rho.mean()), max_rho=float(rho.max()), ke=ke, hist=hist,
+                mean_y=float(np.mean(np.asarray(st["y"], np.float64))))
+
+
+def assert_ensemble_close(a, b, n, what):
+    """Mean density within 2%, max density 10%, density histogram L1 0.05,
+    mean height 2%, kinetic energy within 20% (near rest, plus 1e-3 per
+    particle): the bounds the oracle's two orders are held to."""
+    assert abs(a["mean_rho"] - b["mean_rho"]) < 0.02 * b["mean_rho"], (what, a, b)
+    assert abs(a["max_rho"] - b["max_rho"]) < 0.10 * b["max_rho"], (what, a, b)
+    assert np.abs(a["hist"] - b["hist"]).sum() < 0.05, (what, a, b)
+    assert abs(a["mean_y"] - b["mean_y"]) < 0.02 * b["mean_y"], (what, a, b)
+    assert abs(a["ke"] - b["ke"]) <= 0.2 * max(a["ke"], b["ke"]) + 1e-3 * n, (what, a, b)
+
+
+def test_c1_1000_steps_bitexact_and_ensemble():
+    d = golden("c1_dambreak.npz")
+    parts, tun, box = parts_from_snap(d["input"]), tun_from_bytes(d["tunables"]), po.make_box(60, 40)
+    s = sa.Solver(sa.Tunables.from_buffer_copy(bytes(tun)), sa.AABB.from_buffer_copy(bytes(box)),
+                  len(parts))
+    s.upload(parts)
+    g = po.GatherSim(box, tun, len(parts))
+    g.set_state(parts)
+    done = 0
+    for upto in (250, 500, 1000):
+        print("C1 steps to", upto, flush=True)
+        s.step(upto - done)
+        g.step(upto - done)
+        done = upto
+        assert_bitexact(s.download(), g.get_state(), what=f"C1 step {upto}")
+    got = s.download()
+    c, oc = s.counters(), g.counters()
+    assert c["bucket_overflow"] == oc["bucket_overflow"] == 0 and c["nonfinite"] == 0
+    assert c["max_cell_count"] == oc["max_cell_count"]
+    # the reference's own order: its restatement, and the compiled reference
+    q = parts.copy()
+    po.RefSim(box, tun, len(q)).step(q, 1000)
+    assert_ensemble_close(ensemble(got), ensemble(q), len(q), "vs the reference order")
+    ref1000 = {f: d["step1000"][k] for k, f in enumerate(FIELDS)}
+    assert_ensemble_close(ensemble(got), ensemble(ref1000), len(q), "vs the compiled reference")
+
+
+def field_digests(st):
+    return {f: hashlib.sha256(np.ascontiguousarray(st[f], dtype="<f4").tobytes()).hexdigest()
+            for f in FIELDS}
+
+
+def test_c2_1000_steps_digests():
+    path = os.path.join(GOLDEN, "c2_digests.json")
+    with open(path) as f:
+        want = json.load(f)
+    cps = sorted(int(k) for k in want["checkpoints"])
+    assert cps, "no checkpoints in c2_digests.json"
+    tun = sa.make_tunables(rest_density=10.0, smoothing_radius=1.0, g=9.8, k=0.004, k_near=0.01,
+                           k_spring=10.0, sigma=0.0, beta=0.3, time_step=1.0 / 60.0,
+                           mover_type=0, mover_width=0.0, mover_height=0.0, active=1)
+    box = sa.make_box(1200, 800)
+    parts = sa.scene_lattice(1024, 1024, 0.5, 0.01, seed=want["seed"])
+    s = sa.Solver(tun, box, len(parts))
+    s.upload(parts)
+    done = 0
+    for cp in cps:
+        print("C2 steps to", cp, flush=True)
+        s.step(cp - done)
+        done = cp
+        st = s.download()
+        w = want["checkpoints"][str(cp)]
+        got = field_digests(st)
+        bad = [f for f in FIELDS if got[f] != w["fields"][f]["sha256"]]
+        if bad:
+            f = bad[0]
+            a = np.asarray(st[f], np.float64)
+            raise AssertionError(
+                f"step {cp}: fields {bad} differ from the oracle; {f}: sum {a.sum()} "
+                f"(want {w['fields'][f]['sum']}), min {a.min()} max {a.max()} "
+                f"(want {w['fields'][f]['min']} {w['fields'][f]['max']})")
+        c, oc = s.counters(), w["counters"]
+        assert c["nonfinite"] == 0 and c["key_out_of_range"] == oc["key_out_of_range"]
+        assert c["bucket_overflow"] == oc["bucket_overflow"], (cp, c, oc)
+        assert c["max_cell_count"] == oc["max_cell_count"], (cp, c, oc)
+        # the device takes a hash's neighbour counts in the next viscosity
+        # pass, so its counts lag the oracle's by the checkpoint's last hash
+        assert c["max_neighbors"] <= oc["max_neighbors"], (cp, c, oc)
+        assert c["neighbor_overflow"] <= oc["neighbor_overflow"], (cp, c, oc)
+        print(f"step {cp}: digests equal; counters {c}", flush=True)

@@ -8,7 +8,8 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as po
-from tests.common import SNAP_FIELDS, golden, parts_from_snap, tun_from_bytes
+from tests.common import (SNAP_FIELDS, golden, parts_from_snap, run_reference_loop,
+                          tun_from_bytes)
 
 
 def snap_of(p):
@@ -35,6 +36,47 @@ def test_c1_dambreak_steps():
         if s == 2:
             np.testing.assert_array_equal(sim.lists()[0], d["halflist_len_step2"])
     assert sim.counters() == dict(bucket_overflow=0, neighbor_overflow=0, key_out_of_range=0)
+
+
+def test_c1_dambreak_step_1000():
+    """The whole 1000-step horizon of the N=4096 configuration."""
+    d = golden("c1_dambreak.npz")
+    p = parts_from_snap(d["input"])
+    sim = po.RefSim(po.make_box(60, 40), tun_from_bytes(d["tunables"]), len(p))
+    sim.step(p, 1000)
+    assert_snap_equal(snap_of(p), d["step1000"], "step 1000")
+    assert sim.counters() == dict(bucket_overflow=0, neighbor_overflow=0, key_out_of_range=0)
+
+
+@pytest.mark.parametrize("preset", ["x", "y", "a", "b"])
+def test_reference_presets(preset):
+    """The reference's demo scene (src/fluid.c:88-159) under its presets
+    (src/controls.c:344-401), including super goo's k_spring = -30 and the
+    impulse clamp of src/fluid.c:459 at sigma = 100, beta = 10."""
+    d = golden("presets.npz")
+    p = parts_from_snap(d["input"])
+    box = po.make_box(*d["box"])
+    sim = po.RefSim(box, tun_from_bytes(d[f"{preset}_tunables"]), len(p))
+    sim.step(p, 2)
+    assert_snap_equal(snap_of(p), d[f"{preset}_step2"], f"{preset} step 2")
+    sim.step(p, 18)
+    assert_snap_equal(snap_of(p), d[f"{preset}_step20"], f"{preset} step 20")
+
+
+def test_reference_loop_with_per_frame_tunables():
+    """The reference loop with the render rank's tunables arriving between
+    predict_positions and hash_fluid on every frame's last substep
+    (src/fluid.c:293-294): mover on the attract-mode path, presets taking
+    turns."""
+    d = golden("presets.npz")
+    p = parts_from_snap(d["input"])
+    box = po.make_box(*d["box"])
+    frames = [tun_from_bytes(f) for f in d["frames_tunables"]]
+    sim = po.RefSim(box, tun_from_bytes(d["frames_initial_tunables"]), len(p))
+    run_reference_loop(lambda ph: sim.phase(p, ph), sim.set_tunables, 48, frames)
+    assert_snap_equal(snap_of(p), d["frames_step48"], "frames step 48")
+    run_reference_loop(lambda ph: sim.phase(p, ph), sim.set_tunables, 48, frames, first_step=48)
+    assert_snap_equal(snap_of(p), d["frames_step96"], "frames step 96")
 
 
 def test_phases64_every_phase_and_lists():
