@@ -53,6 +53,19 @@ KERNEL_BYTES = {
     "relax": (52, 12),           # read x, y, xp, yp, rho, rhon, key (28); write x, y, vx, vy, key, slot (24)
     #                              (plus the 16 B of masks read back)
 }
+# The per-kernel timing behind `roofline` always covers the same state,
+# whatever --steps / --warmup are: eager steps KERNEL_WINDOW + 1 ..
+# KERNEL_WINDOW + KERNEL_STEPS after a fresh upload (the settling column,
+# where the gathers are costliest early in the run).  The committed profiles
+# are taken over the same window (--kernels-only).
+KERNEL_WINDOW = 220
+KERNEL_STEPS = 10
+LAUNCHES_PER_STEP = 11
+# Second line: the compressed second half of the run.
+LATE_START, LATE_STEPS = 500, 100
+L2_FILE = os.path.join(ROOT, "profiles", "l2_hit_c2_dam_break.txt")
+SHADOW_EVERY = 8
+
 # profiler names of the kernels (template instances of the fused step)
 KERNEL_SYMBOLS = {"visc_predict": "k_visc_predict<true, true, true>", "relax": "k_relax<true>",
                   "density": "k_density", "reorder": "k_reorder<2>", "scatter": "k_scatter",
@@ -184,12 +197,17 @@ def surrogate_leg(step_eq, nscenes=64, n=65536, tol=0.01, seed=11, sample_every=
     """Batched inference plus fallback: nscenes sloshing dam breaks of n
     particles.  The same scenes run step_eq plain steps (batched graph), and
     step_eq / 4 judged surrogate macro-steps (the plain solver re-takes any
-    scene whose mean density moves by more than tol in a macro-step).  Both
-    runs are timed in chunks of sample_every plain-step equivalents; between
-    chunks, outside the timers, each scene's mean density is sampled (a hash
-    and density pass of the current state, which leaves the run unchanged).
-    The density error is per scene, against the plain run: at the final
-    state, and of the time-averaged mean density."""
+    scene whose mean density is off its plain density trend by more than tol,
+    and every scene on a fixed schedule).  Both runs are timed in chunks of
+    sample_every plain-step equivalents; between chunks, outside the timers,
+    each scene's mean density is sampled (a hash and density pass of the
+    current state, which leaves the run unchanged): the drift of the
+    surrogate run from the plain run, per scene, at the final state and of
+    the time-averaged mean density.  A third run adds shadow judging (one
+    scene in SHADOW_EVERY per macro-step also takes the macro-step plainly
+    from the same start) and reports the density error of the macro-steps
+    the trend judge accepted against that plain result: the per-step error
+    the 1% bound is about."""
     from sph_amd import surrogate as su
     if not os.path.exists(su.WEIGHTS):
         return {"skipped": f"no trained weights at {os.path.relpath(su.WEIGHTS, ROOT)}"}
@@ -229,6 +247,26 @@ def surrogate_leg(step_eq, nscenes=64, n=65536, tol=0.01, seed=11, sample_every=
         flags.append(f)
         rho_s.append(density(sur))
     sur.close()
+    # the same run with shadow judging: 1 scene in SHADOW_EVERY per
+    # macro-step also takes the macro-step plainly from the same start,
+    # which measures the error of the steps the trend judge accepts
+    sh = su.make_batch(scenes)
+    sh.step(1)
+    m2 = sa.Surrogate(seed=0)
+    su.load_weights(m2)
+    m2.set_shadow(SHADOW_EVERY)
+    t_sh, sh_flags = su.run_judged(m2, sh, chunks * per_chunk + 1, tol)
+    shadow = m2.shadow_stats()
+    shadow.update({"every": SHADOW_EVERY, "seconds": t_sh,
+                   "value": total * (chunks * per_chunk + 1) * su.MACRO / t_sh,
+                   "shadow_failure_rate": float((sh_flags == 3).mean()),
+                   "fallback_rate": float((sh_flags != 0).mean()),
+                   "definition": "per-scene relative error of the mean density (and the "
+                                 "per-particle relative RMS density error) of macro-steps the "
+                                 "trend judge accepted, against the plain macro-step from the "
+                                 "same start; 1 scene in SHADOW_EVERY per macro-step, rotating"})
+    sh.close()
+    m2.close()
     m.close()
     rho_p, rho_s = np.array(rho_p), np.array(rho_s)
     err_final = np.abs(rho_s[-1] - rho_p[-1]) / rho_p[-1]
@@ -245,7 +283,8 @@ def surrogate_leg(step_eq, nscenes=64, n=65536, tol=0.01, seed=11, sample_every=
             "fallback_rate": float((flags != 0).mean()) if flags.size else 0.0,
             "judged_failure_rate": float((flags == 1).mean()) if flags.size else 0.0,
             "forced_plain_rate": float((flags == 2).mean()) if flags.size else 0.0,
-            "density_error": {"final_max": float(err_final.max()),
+            "shadow": shadow,
+            "drift_vs_plain_run": {"final_max": float(err_final.max()),
                               "final_mean": float(err_final.mean()),
                               "time_avg_max": float(err_avg.max()),
                               "time_avg_mean": float(err_avg.mean()),
@@ -319,14 +358,20 @@ def profile_summary(db, last=0):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_c2_dam_break.txt")
 
 
-def _counter_means(db, counter):
-    """Per kernel: list of per-dispatch values of one counter."""
+def _counter_means(db, counter, last=0):
+    """Per kernel: {dispatch id: value} of one counter (summed over its
+    records), over the last `last` dispatches of the run when given."""
     import sqlite3
     con = sqlite3.connect(db)
+    rows = con.execute("select dispatch_id, kernel_name, value from counters_collection "
+                       "where counter_name = ?", (counter,)).fetchall()
+    ids = sorted({d for d, _, _ in rows})
+    keep = set(ids[-last:]) if last else set(ids)
     out = {}
-    for d, k, v in con.execute("select dispatch_id, kernel_name, value from counters_collection "
-                               "where counter_name = ?", (counter,)):
-        out.setdefault(re.sub(r"\(.*", "", k).replace("void ", ""), {})[d] = v
+    for d, k, v in rows:
+        if d in keep:
+            m = out.setdefault(re.sub(r"\(.*", "", k).replace("void ", ""), {})
+            m[d] = m.get(d, 0.0) + v
     return out
 
 
@@ -370,14 +415,15 @@ def counter_summary(db, last=0, simds=1024, xcds=8):
     return "\n".join(lines)
 
 
-def traffic_summary(fetch_db, write_db, copy_mb):
+def traffic_summary(fetch_db, write_db, copy_mb, last=0):
     """HBM bytes per launch of every kernel from a FETCH_SIZE and a WRITE_SIZE
     counter run (KiB per dispatch), each scaled by the factor that makes the
     same counter read the known size of the bench's calibration copy
     (--calib-copy-mb: a device-to-device copy of copy_mb MB, which reads and
-    writes copy_mb MB)."""
-    fetch = _counter_means(fetch_db, "FETCH_SIZE")
-    write = _counter_means(write_db, "WRITE_SIZE")
+    writes copy_mb MB).  With `last`, only the last N dispatches of each run
+    (the calibration copies come after the kernel window, so count them in)."""
+    fetch = _counter_means(fetch_db, "FETCH_SIZE", last)
+    write = _counter_means(write_db, "WRITE_SIZE", last)
     known_kib = copy_mb * 1e6 / 1024.0
     cal = {}
     for name, tab in (("fetch", fetch), ("write", write)):
@@ -398,6 +444,124 @@ def traffic_summary(fetch_db, write_db, copy_mb):
         mb = (fm * cal["fetch"] + wm * cal["write"]) * 1024.0 / 1e6
         lines.append(f"{k:42s} {max(len(f), len(w)):10d} {fm:12.1f} {wm:12.1f} {mb:18.3f}")
     return "\n".join(lines)
+
+
+def l2_summary(db, last=0):
+    """L2 (TCC) hit rate per kernel from a rocprofv3 --pmc TCC_HIT_sum
+    TCC_MISS_sum run: hits / (hits + misses) of the L2 requests, summed over
+    the channels, per dispatch (the last `last` dispatches when given)."""
+    hit = _counter_means(db, "TCC_HIT_sum", last)
+    miss = _counter_means(db, "TCC_MISS_sum", last)
+    lines = [f"{'kernel':42s} {'dispatches':>10s} {'TCC_HIT':>14s} {'TCC_MISS':>14s} {'L2_hit_rate':>12s}"]
+    for k in sorted(set(hit) | set(miss)):
+        h = list(hit.get(k, {}).values())
+        m = list(miss.get(k, {}).values())
+        hm = sum(h) / len(h) if h else 0.0
+        mm = sum(m) / len(m) if m else 0.0
+        rate = hm / (hm + mm) if hm + mm > 0 else float("nan")
+        lines.append(f"{k:42s} {max(len(h), len(m)):10d} {hm:14.1f} {mm:14.1f} {rate:12.4f}")
+    return "\n".join(lines)
+
+
+def l2_hit_of(kernel_name):
+    """L2 hit rate of a kernel from the committed summary (None when absent)."""
+    if not os.path.exists(L2_FILE):
+        return None
+    with open(L2_FILE) as f:
+        for line in f:
+            parts = re.split(r"\s{2,}", line.strip())
+            if parts and parts[0] == kernel_name and len(parts) >= 5:
+                try:
+                    return float(parts[-1])
+                except ValueError:
+                    return None
+    return None
+
+
+def kernel_window(s, upload):
+    """Per-kernel mean durations (ms) over eager steps KERNEL_WINDOW + 1 ..
+    KERNEL_WINDOW + KERNEL_STEPS after a fresh upload, each kernel timed by
+    HIP events that its own launch records on the solver's stream."""
+    upload()
+    s.reset_counters()
+    s.step(KERNEL_WINDOW)
+    s.sync()
+    return s.time_kernels(KERNEL_STEPS), s.counters()
+
+
+def late_regime(s, upload):
+    """Graph-replayed steps LATE_START + 1 .. LATE_START + LATE_STEPS: the
+    compressed second half of the dam break."""
+    upload()
+    s.reset_counters()
+    s.step(LATE_START)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(LATE_STEPS)
+    s.sync()
+    el = time.perf_counter() - t0
+    n = s.n
+    return {"value": n * LATE_STEPS / el, "unit": "particle-steps/s",
+            "ms_per_step": el * 1e3 / LATE_STEPS,
+            "steps": f"{LATE_START + 1}-{LATE_START + LATE_STEPS} after a fresh upload",
+            "counters": s.counters()}
+
+
+def c1_line():
+    """The N = 4096 configuration (64 x 64 lattice, box 60 x 40, 1000 steps):
+    the GPU through the step graph, and the reference's own C step
+    (oracle/_ref, -O3 -ffast-math) on one host core over the same 1000
+    steps."""
+    tun, box, parts = workload(64, (60.0, 40.0), seed=20260101)
+    n = len(parts)
+    s = sa.Solver(tun, box, n)
+    s.upload(parts)
+    s.step(1)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(999)
+    s.sync()
+    el = time.perf_counter() - t0
+    s.close()
+    out = {"particles": n, "steps": "2-1000 (graph)", "value": n * 999 / el,
+           "unit": "particle-steps/s", "ms_per_step": el * 1e3 / 999}
+    from oracle import pyoracle as po
+    if po.have_compiled_ref(fast=True):
+        sim = po.CompiledRef(box, tun, n, fast=True)
+        sim.load(parts)
+        sim.step(1)
+        t0 = time.perf_counter()
+        sim.step(999)
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n * 999 / el, "unit": "particle-steps/s", "cores": 1,
+                               "kind": "reference", "cpu": cpu_model(),
+                               "sample": "steps 2-1000 of the same scene, one host core"}
+    return out
+
+
+def drop_in_line(steps=42):
+    """The reference loop (src/fluid.c:270-372) in the C driver linked
+    against the reference-named entry points, on the 1024 x 1024 dam break
+    (no jitter: the reference's own lattice) in the 1200 x 800 box: wall time
+    per step of the loop with the state copied back to the host every step
+    (all fields), every 4 steps (the reference packs coordinates every 4th
+    substep), every 4 steps positions only, and never.  Runs the driver as a
+    child process."""
+    import subprocess
+    exe = os.path.join(ROOT, "sph_amd", "sph_main")
+    if not os.path.exists(exe):
+        return {"skipped": "sph_amd/sph_main not built"}
+    out = {}
+    for name, args in (("sync_every_step", ["-s", "1"]), ("sync_every_4", ["-s", "4"]),
+                       ("sync_every_4_positions", ["-s", "4", "-p"]), ("no_sync", ["-s", "0"])):
+        r = subprocess.run([exe, "-t", *args, "-b", "1200,800", "1024", "1024", str(steps)],
+                           capture_output=True, text=True, timeout=300)
+        m = re.search(r"([0-9.]+) ms/step, ([0-9.e+]+) particle-steps/s", r.stdout)
+        out[name] = ({"ms_per_step": float(m.group(1)), "value": float(m.group(2))} if m
+                     else {"error": (r.stderr or r.stdout)[-300:]})
+    out["unit"] = "particle-steps/s"
+    out["steps"] = f"3-{steps} of the loop"
+    return out
 
 
 def traffic_of(kernel_name):
@@ -442,7 +606,16 @@ def main():
                          "(with --last N: its last N dispatches) and exit")
     ap.add_argument("--traffic-summary", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
                     help="print calibrated HBM traffic per kernel from two rocprofv3 --pmc "
-                         "databases and exit (uses --calib-copy-mb)")
+                         "databases and exit (uses --calib-copy-mb; with --last N, the last "
+                         "N dispatches)")
+    ap.add_argument("--l2-summary", metavar="DB",
+                    help="print per-kernel L2 hit rates of a rocprofv3 --pmc TCC_HIT_sum "
+                         "TCC_MISS_sum run and exit (with --last N, the last N dispatches)")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="run only the per-kernel timing window (for the profiler) and print "
+                         "its table")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the late-regime, N=4096 and drop-in lines")
     args = ap.parse_args()
     if args.profile_summary:
         print(profile_summary(args.profile_summary, args.last))
@@ -451,7 +624,19 @@ def main():
         print(counter_summary(args.counter_summary, args.last))
         return
     if args.traffic_summary:
-        print(traffic_summary(*args.traffic_summary, args.calib_copy_mb or 256.0))
+        print(traffic_summary(*args.traffic_summary, args.calib_copy_mb or 256.0, args.last))
+        return
+    if args.l2_summary:
+        print(l2_summary(args.l2_summary, args.last))
+        return
+    if args.kernels_only:
+        tun, box, parts = workload()
+        s = sa.Solver(tun, box, len(parts))
+        kt, cnt = kernel_window(s, lambda: s.upload(parts))
+        if args.calib_copy_mb > 0:
+            calibration_copy(args.calib_copy_mb)
+        print(json.dumps({"window": f"eager steps {KERNEL_WINDOW + 1}-{KERNEL_WINDOW + KERNEL_STEPS}"
+                                    " after a fresh upload", "kernels_ms": kt, "counters": cnt}))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs one scene on one GPU; --gpus must be 1"}))
@@ -476,7 +661,9 @@ def main():
     # per-kernel HIP-event timing on the solver's stream
     # (kinds this bench does not know, e.g. from another library build, are
     # left out)
-    kt = {k: ms for k, ms in s.time_kernels(10).items() if k in KERNEL_BYTES}
+    late = None if args.no_extra else late_regime(s, lambda: s.upload(parts))
+    kt_all, window_counters = kernel_window(s, lambda: s.upload(parts))
+    kt = {k: ms for k, ms in kt_all.items() if k in KERNEL_BYTES}
     sx, sy = s.cells()[2]
     c = sx * sy / n
     kernel_bw = {}
@@ -510,6 +697,10 @@ def main():
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS if ach else None,
                      "traffic": traffic_of(KERNEL_SYMBOLS.get(dom, dom)),
                      "algorithmic_bytes": kernel_bw[dom]["bytes"],
+                     "kernel_ms": kt[dom],
+                     "window": f"eager steps {KERNEL_WINDOW + 1}-{KERNEL_WINDOW + KERNEL_STEPS} "
+                               "after a fresh upload (the profiles' window)",
+                     "l2_hit_rate": l2_hit_of(KERNEL_SYMBOLS.get(dom, dom)),
                      "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT)
                      if traffic_of(KERNEL_SYMBOLS.get(dom, dom)) is not None else None},
         "step_roofline": {"bytes_per_particle_step": b_step,
@@ -517,12 +708,20 @@ def main():
                           "frac": b_step * value / 1e9 / HBM_PEAK_GBS},
         "kernels": kernel_bw,
         "counters": counters,
+        "window_counters": window_counters,
     }
+    if late:
+        late["step_roofline_frac"] = b_step * late["value"] / 1e9 / HBM_PEAK_GBS
+        out["late_regime"] = late
     if not args.no_batched:
         out["batched"] = batched(max(10, args.steps // 2), max(2, args.warmup // 2),
                                  args.repeats)
         if cpu_batched:
             out["batched"]["cpu_baseline"] = cpu_batched
+    if not args.no_extra:
+        s.close()
+        out["c1"] = c1_line()
+        out["drop_in"] = drop_in_line()
     if not args.no_surrogate:
         out["batched_surrogate"] = surrogate_leg(args.surrogate_steps)
     if args.calib_copy_mb > 0:

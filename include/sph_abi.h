@@ -459,9 +459,25 @@ int sph_surrogate_adam(sph_surrogate *m, float lr, float beta1, float beta2, flo
  * unhashed afterwards and the clock advances by four steps. */
 int sph_surrogate_apply(sph_surrogate *m, sph_ctx *ctx);
 /* One judged macro-step of every scene (see sph_mlp.hip); flags (optional,
- * nscenes ints) marks the scenes re-taken by four plain steps. */
+ * nscenes ints) marks the scenes re-taken by four plain steps: 1 judged
+ * off the plain solver's density trend by more than tol, 2 the scheduled
+ * plain macro-step (the first one, then one in 16), 3 off the plain shadow
+ * (below) by more than tol. */
 int sph_surrogate_macro_step(sph_surrogate *m, sph_ctx *ctx, float tol, int *flags,
                              int *nfallback);
+/* Shadow judging: on each macro-step one scene in every `every` (rotating;
+ * 0 = off, the default) first takes the macro-step plainly from the same
+ * start.  Its surrogate result is then also judged against that plain
+ * result, and for the steps the trend judge accepted the error against it
+ * is recorded.  Resets the statistics. */
+int sph_surrogate_set_shadow(sph_surrogate *m, int every);
+/* Statistics since sph_surrogate_set_shadow: out[0] shadowed scene
+ * macro-steps, out[1] of them accepted by the trend judge, out[2] of those
+ * with |e| > tol, out[3] sum |e|, out[4] max |e|, out[5] sum e, out[6] sum
+ * and out[7] max of the per-particle relative RMS density error, where e is
+ * the relative error of the scene's mean density against the plain
+ * result. */
+int sph_surrogate_shadow_stats(sph_surrogate *m, double *out);
 /* D = A B of one 16x16x16 BF16 MFMA with fixed small-integer operands,
  * row-major (checks the lane layout the kernels assume). */
 int sph_surrogate_mfma_selftest(float *out256);

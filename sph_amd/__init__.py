@@ -177,6 +177,8 @@ def lib():
             "sph_surrogate_apply": (ip, [vp, vp]),
             "sph_surrogate_macro_step": (ip, [vp, vp, ctypes.c_float, vp, ctypes.POINTER(ip)]),
             "sph_surrogate_mfma_selftest": (ip, [vp]),
+            "sph_surrogate_set_shadow": (ip, [vp, ip]),
+            "sph_surrogate_shadow_stats": (ip, [vp, vp]),
             "sph_surrogate_backward_trajectory": (ip, [vp, vp, vp, ctypes.POINTER(ctypes.c_float)]),
             "sph_trajectory_create": (ip, [ctypes.POINTER(vp), ip, ip, ip]),
             "sph_selftest_sqrt": (ip, [ctypes.c_uint, ctypes.c_uint,
@@ -480,6 +482,19 @@ class Surrogate:
 
     def apply(self, solver: Solver):
         _check_m(lib().sph_surrogate_apply(self._h, solver._h), "apply")
+
+    def set_shadow(self, every: int):
+        """Shadow judging of one scene in every `every` per macro-step (0: off)."""
+        _check_m(lib().sph_surrogate_set_shadow(self._h, int(every)), "set_shadow")
+
+    def shadow_stats(self) -> dict:
+        o = np.zeros(8, np.float64)
+        _check_m(lib().sph_surrogate_shadow_stats(self._h, o.ctypes.data), "shadow_stats")
+        acc = max(o[1], 1.0)
+        return {"shadowed": int(o[0]), "accepted": int(o[1]), "accepted_over_tol": int(o[2]),
+                "mean_abs_error": o[3] / acc, "max_abs_error": float(o[4]),
+                "mean_signed_error": o[5] / acc, "mean_rms_error": o[6] / acc,
+                "max_rms_error": float(o[7])}
 
     def macro_step(self, solver: Solver, tol=0.01):
         flags = np.zeros(solver.nscenes, np.int32)

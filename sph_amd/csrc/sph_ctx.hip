@@ -354,7 +354,7 @@ int sph_create_batch(sph_ctx **out, const sph_scene_desc *scenes, int nscenes)
     A(ID0, N); A(ID1, N); A(perm_src, N); A(perm_id, N); A(pos_of_id, N);
     A(K0, N); A(K1, N); A(ukey, N); A(uslot, N);
     A(cnt, MP); A(start, MP); A(bsum, sph_scan_blocks((unsigned)M));
-    A(coords, 2 * N); A(MASK, N); A(dP, (size_t)nscenes); A(dPA, (size_t)nscenes);
+    A(coords, 2 * N); A(MASK, N); A(dP, (size_t)nscenes); A(dPA, (size_t)nscenes); A(ACC, N);
     A(BS, N / kSphTPB); A(dC, 1); A(dclock, 1);
 #undef A
     if (!rc) {
@@ -415,7 +415,7 @@ void sph_destroy(sph_ctx *c)
                     c->cnt,  c->start, c->bsum,    c->coords,  c->dP,        c->BS,
                     c->dC,   c->dclock, c->SPOS,    c->SVEL,    c->SPREV,     c->SRHO,
                     c->SID,  c->SK,    c->Sstart,  c->Sclock,  c->dflags, c->MASK,
-                    c->dPA,  c->XY};
+                    c->dPA,  c->XY,    c->ACC,     c->AOS};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -474,7 +474,7 @@ static int upload_impl(sph_ctx *c, const fluid_particle *const *pp, const fluid_
     for (auto &v : f)
         v.assign(N, make_float2(0.0f, 0.0f));
     std::vector<int> ids(N, 0);
-    std::vector<float> ax((size_t)total, 0.0f), ay((size_t)total, 0.0f);
+    std::vector<float2> acc((size_t)total, make_float2(0.0f, 0.0f));
     std::vector<unsigned> uoff((size_t)ns);
     long u = 0;
     for (int s = 0; s < ns; s++) {
@@ -495,8 +495,7 @@ static int upload_impl(sph_ctx *c, const fluid_particle *const *pp, const fluid_
             f[1][slot] = make_float2(p->v_x, p->v_y);
             f[2][slot] = make_float2(p->x_prev, p->y_prev);
             f[3][slot] = make_float2(p->density, p->density_near);
-            ax[u + p->id] = p->a_x;
-            ay[u + p->id] = p->a_y;
+            acc[u + p->id] = make_float2(p->a_x, p->a_y);
         }
         uoff[s] = (unsigned)u;
         u += n;
@@ -506,13 +505,13 @@ static int upload_impl(sph_ctx *c, const fluid_particle *const *pp, const fluid_
     for (int k = 0; k < 4; k++)
         HIPCHK(hipMemcpy(dst[k], f[k].data(), sizeof(float2) * N, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->ID0, ids.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+    if (total > 0)
+        HIPCHK(hipMemcpy(c->ACC, acc.data(), sizeof(float2) * (size_t)total, hipMemcpyHostToDevice));
     HIPCHK(hipMemset(c->dclock, 0, sizeof(unsigned long long)));
     for (int s = 0; s < ns; s++) {
         c->sc[s].n = counts[s];
         c->sc[s].uoff = uoff[s];
     }
-    c->ax_by_id.swap(ax);
-    c->ay_by_id.swap(ay);
     c->n_total = (int)total;
     c->lists_valid = false;
     c->aux_order = 0;
@@ -959,46 +958,19 @@ int sph_download(sph_ctx *c, fluid_particle *out)
         return rc;
     if (c->n_total > 0 && !out)
         return set_err(SPH_EINVAL, "sph_download: null output");
-    HIPCHK(hipStreamSynchronize(c->st));
-    if (c->n_total == 0)
+    if (c->n_total == 0) {
+        HIPCHK(hipStreamSynchronize(c->st));
         return SPH_OK;
-    const size_t N = c->slots;
-    std::vector<float2> f[4];
-    const float2 *src[4] = {c->POS0, c->VEL, c->PREV, c->RHO};
-    for (int k = 0; k < 4; k++) {
-        f[k].resize(N);
-        HIPCHK(hipMemcpy(f[k].data(), src[k], sizeof(float2) * N, hipMemcpyDeviceToHost));
     }
-    std::vector<int> id0(N), idaux(N);
-    HIPCHK(hipMemcpy(id0.data(), c->ID0, sizeof(int) * N, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(idaux.data(), c->aux_order ? c->ID1 : c->ID0, sizeof(int) * N,
-                     hipMemcpyDeviceToHost));
-    for (const Scene &S : c->sc) {
-        const tunable_parameters &t = S.tun;
-        for (int i = 0; i < S.n; i++) {
-            const size_t slot = S.poff + (size_t)i;
-            const size_t g = S.uoff + (size_t)id0[slot];
-            fluid_particle *p = &out[g];
-            p->x = f[0][slot].x;
-            p->y = f[0][slot].y;
-            p->v_x = f[1][slot].x;
-            p->v_y = f[1][slot].y;
-            p->id = id0[slot];
-            p->a_x = c->ax_by_id[g];
-            p->a_y = c->ay_by_id[g];
-        }
-        for (int i = 0; i < S.n; i++) {
-            const size_t slot = S.poff + (size_t)i;
-            fluid_particle *p = &out[S.uoff + (size_t)idaux[slot]];
-            p->x_prev = f[2][slot].x;
-            p->y_prev = f[2][slot].y;
-            p->density = f[3][slot].x;
-            p->density_near = f[3][slot].y;
-            // src/fluid.c:563-564
-            p->pressure = t.k * (f[3][slot].x - t.rest_density);
-            p->pressure_near = t.k_near * f[3][slot].y;
-        }
-    }
+    if (!c->AOS)
+        if (int rc = dalloc(&c->AOS, (size_t)c->slots * 13))
+            return rc;
+    sph_launch_gather_aos(c->st, c->slots, c->dP, c->BS, c->POS0, c->VEL, c->ID0, c->PREV, c->RHO,
+                          c->aux_order ? c->ID1 : c->ID0, c->ACC, c->AOS);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, c->AOS, sizeof(fluid_particle) * (size_t)c->n_total,
+                          hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
     return SPH_OK;
 }
 

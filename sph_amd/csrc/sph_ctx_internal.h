@@ -54,7 +54,8 @@ struct sph_ctx {
              *start = nullptr, *bsum = nullptr;
     short *coords = nullptr;
     uint4 *MASK = nullptr; // per slot: the density pass's first accepted-candidate masks
-    std::vector<float> ax_by_id, ay_by_id; // a_x/a_y: carried, never changed (src/fluid.c)
+    float2 *ACC = nullptr;  // (a_x, a_y) by caller index: carried, never changed (src/fluid.c)
+    float *AOS = nullptr;   // fluid_particle records by caller index (allocated on first download)
     bool lists_valid = false; // a hash ran since upload: K0/start describe POS0
     int aux_order = 0;        // PREV, RHO ordered like ID0 (0) or ID1 (1)
     // snapshot of the canonical state (sph_snapshot_save / sph_snapshot_restore)

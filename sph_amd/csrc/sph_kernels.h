@@ -91,6 +91,12 @@ void sph_launch_gather_state(hipStream_t st, unsigned slots, const sph_dev_param
 void sph_launch_load_state(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
                            const float4 *in, float2 *POS, float2 *VEL, float2 *PREV, float2 *RHO,
                            int *ID);
+// fluid_particle records (13 words) by caller index uoff + id; IDA orders
+// PREV and RHO, ACC holds (a_x, a_y) by caller index.
+void sph_launch_gather_aos(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                           const float2 *POS, const float2 *VEL, const int *ID0,
+                           const float2 *PREV, const float2 *RHO, const int *IDA,
+                           const float2 *ACC, float *out);
 // Positions (x, y) by caller index uoff + id.
 void sph_launch_gather_xy(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
                           const float2 *POS, const int *ID, float2 *out);

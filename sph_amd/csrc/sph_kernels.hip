@@ -1018,6 +1018,43 @@ k_gather_state(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
     out[P.uoff + (unsigned)ID[s]] = make_float4(p.x, p.y, v.x, v.y);
 }
 
+// The reference's particle records (fluid_particle, 13 words) by caller
+// index: position, velocity, a and id from the canonical order (ID0); x_prev,
+// density and the pressures (src/fluid.c:563-564) from the auxiliary fields'
+// order (IDA: ID0, or ID1 after fused steps).
+__global__ void __launch_bounds__(kTPB)
+k_gather_aos(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+             const float2 *__restrict__ POS, const float2 *__restrict__ VEL,
+             const int *__restrict__ ID0, const float2 *__restrict__ PREV,
+             const float2 *__restrict__ RHO, const int *__restrict__ IDA,
+             const float2 *__restrict__ ACC, float *__restrict__ out)
+{
+    const unsigned blk = xcd_block();
+    const sph_dev_params &P = Pp[BS[blk]];
+    const unsigned s = blk * kTPB + threadIdx.x;
+    if (s >= P.poff + (unsigned)P.n)
+        return;
+    const int id = ID0[s];
+    const size_t g = P.uoff + (unsigned)id;
+    const float2 p = POS[s], v = VEL[s], a = ACC[g];
+    float *r = out + 13 * g;
+    r[2] = p.x;
+    r[3] = p.y;
+    r[4] = v.x;
+    r[5] = v.y;
+    r[6] = a.x;
+    r[7] = a.y;
+    r[12] = __int_as_float(id);
+    const float2 q = PREV[s], rho = RHO[s];
+    float *r2 = out + 13 * ((size_t)P.uoff + (unsigned)IDA[s]);
+    r2[0] = q.x;
+    r2[1] = q.y;
+    r2[8] = rho.x;
+    r2[9] = rho.y;
+    r2[10] = P.k * (rho.x - P.rest_density);
+    r2[11] = P.k_near * rho.y;
+}
+
 // Positions by caller index (scene offset + id), for position-only read-back.
 __global__ void __launch_bounds__(kTPB)
 k_gather_xy(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
@@ -1206,6 +1243,14 @@ void sph_launch_gather_state(hipStream_t st, unsigned slots, const sph_dev_param
                              float4 *out)
 {
     launch(k_gather_state, GRID(slots), kTPB, st, P, BS, POS, VEL, ID, out);
+}
+
+void sph_launch_gather_aos(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
+                           const float2 *POS, const float2 *VEL, const int *ID0,
+                           const float2 *PREV, const float2 *RHO, const int *IDA,
+                           const float2 *ACC, float *out)
+{
+    launch(k_gather_aos, GRID(slots), kTPB, st, P, BS, POS, VEL, ID0, PREV, RHO, IDA, ACC, out);
 }
 
 void sph_launch_gather_xy(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,

@@ -278,7 +278,14 @@ struct sph_surrogate {
     int *jflags = nullptr;
     const sph_ctx *ref_ctx = nullptr;
     bool ref_ok = false;
-    std::vector<int> since_plain; // macro-steps since each scene's last plain macro-step
+    unsigned long long macro = 0; // judged macro-steps so far (the refresh / shadow schedule)
+    // shadow judging: 1 of every shadow_every scenes (rotating) also takes
+    // the macro-step plainly from the same start; 0 = off
+    int shadow_every = 0;
+    float *sh_sum = nullptr;              // per scene: the plain shadow's density sum
+    float *rho_plain = nullptr, *rho_sur = nullptr; // density by caller index
+    float *sh_err = nullptr;              // per scene: sum (sur - plain)^2, sum plain^2
+    double sh_stats[8] = {0};             // see sph_surrogate_shadow_stats
 };
 
 struct sph_trajectory {
@@ -1021,6 +1028,53 @@ k_scene_density(const sph_dev_params *__restrict__ Pp, const float2 *__restrict_
     }
 }
 
+// Density of every particle by caller index (the state is canonical: RHO
+// in ID0 order).
+__global__ void __launch_bounds__(256)
+k_rho_by_id(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS, unsigned slots,
+            const float2 *__restrict__ RHO, const int *__restrict__ ID, float *__restrict__ out)
+{
+    const unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= slots)
+        return;
+    const sph_dev_params &P = Pp[BS[s / 256u]];
+    if (s >= P.poff + (unsigned)P.n)
+        return;
+    out[P.uoff + (unsigned)ID[s]] = RHO[s].x;
+}
+
+// Per shadowed scene (flags[s] != 0): sum over its particles of
+// (sur - plain)^2 and of plain^2, for the relative RMS density error.
+__global__ void __launch_bounds__(256)
+k_shadow_err(const sph_dev_params *__restrict__ Pp, const int *__restrict__ flags,
+             const float *__restrict__ plain, const float *__restrict__ sur,
+             float *__restrict__ out)
+{
+    __shared__ float pa[4], pb[4];
+    const sph_dev_params &P = Pp[blockIdx.x];
+    if (!flags[blockIdx.x])
+        return;
+    float a = 0.0f, b = 0.0f;
+    for (unsigned i = threadIdx.x; i < (unsigned)P.n; i += blockDim.x) {
+        const float x = plain[P.uoff + i], y = sur[P.uoff + i];
+        a += (y - x) * (y - x);
+        b += x * x;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+    }
+    if (lane() == 0) {
+        pa[threadIdx.x / 64u] = a;
+        pb[threadIdx.x / 64u] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out[2 * blockIdx.x] = (pa[0] + pa[1]) + (pa[2] + pa[3]);
+        out[2 * blockIdx.x + 1] = (pb[0] + pb[1]) + (pb[2] + pb[3]);
+    }
+}
+
 // Judge of a macro-step.  Per scene, ref[s] is the mean density of the
 // current state and dref[s] = ref[nscenes + s] the change of the scene's last
 // macro-step computed by the plain solver (its density trend).  A predicted
@@ -1141,12 +1195,11 @@ int ensure_judge(sph_surrogate *m, const sph_ctx *c)
     const size_t ns = c->sc.size();
     if (m->ns >= ns)
         return SPH_OK;
-    void *old[] = {m->ref, m->jsum, m->jmax, m->jflags};
+    void *old[] = {m->ref, m->jsum, m->jmax, m->jflags, m->sh_sum, m->sh_err};
     for (void *p : old)
         if (p)
             (void)hipFree(p);
-    m->ref = m->jsum = nullptr;
-    m->jmax = nullptr;
+    m->ref = m->jsum = m->jmax = m->sh_sum = m->sh_err = nullptr;
     m->jflags = nullptr;
     m->ref_ok = false;
     int rc = SPH_OK;
@@ -1154,6 +1207,8 @@ int ensure_judge(sph_surrogate *m, const sph_ctx *c)
     if (!rc) rc = malloc_dev(&m->jsum, ns);
     if (!rc) rc = malloc_dev(&m->jmax, ns);
     if (!rc) rc = malloc_dev(&m->jflags, ns);
+    if (!rc) rc = malloc_dev(&m->sh_sum, ns);
+    if (!rc) rc = malloc_dev(&m->sh_err, 2 * ns);
     m->ns = rc ? 0 : ns;
     return rc;
 }
@@ -1283,7 +1338,8 @@ void sph_surrogate_destroy(sph_surrogate *m)
         return;
     void *ptrs[] = {m->params, m->grads, m->adam_m, m->adam_v, m->W1b, m->W2b, m->W3b, m->S,
                     m->deg,    m->dS,    m->disp,   m->tgt,    m->U,   m->dP3, m->dM,  m->loss,
-                    m->tmp2, m->ref, m->jsum, m->jmax, m->jflags};
+                    m->tmp2, m->ref, m->jsum, m->jmax, m->jflags, m->sh_sum,
+                    m->rho_plain, m->rho_sur, m->sh_err};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -1547,9 +1603,16 @@ int sph_surrogate_apply(sph_surrogate *m, sph_ctx *c)
 // (relative) from the plain solver's trend (its mean density plus the change
 // of its last plain macro-step) is restored and re-taken with four plain
 // steps.  Every scene is also re-taken plainly on its first macro-step and
-// then every kTrendRefresh macro-steps, which keeps its trend current.
-// flags (optional, nscenes ints) receives 1 for failed scenes, 2 for forced
-// re-takes, 0 for accepted ones; *nfallback the number re-taken.
+// then on a fixed schedule, one macro-step in kTrendRefresh (staggered over
+// the scenes), which keeps its trend current whatever the failures.
+// With shadow judging on (sph_surrogate_set_shadow), one scene in every
+// shadow_every (rotating) first takes the macro-step plainly from the same
+// start: its surrogate result is then judged against that plain result as
+// well (error above tol: re-taken, flag 3), and the error of the steps the
+// trend judge accepted is recorded (sph_surrogate_shadow_stats).
+// flags (optional, nscenes ints) receives 1 for failed scenes, 2 for
+// scheduled re-takes, 3 for shadow failures, 0 for accepted ones;
+// *nfallback the number re-taken.
 int sph_surrogate_macro_step(sph_surrogate *m, sph_ctx *c, float tol, int *flags, int *nfallback)
 {
     if (!m || !c)
@@ -1567,6 +1630,7 @@ int sph_surrogate_macro_step(sph_surrogate *m, sph_ctx *c, float tol, int *flags
         if (sph_phase_hash(c, 0))
             return merr(SPH_EHIP, "hash failed: %s", sph_last_error());
     const dim3 jg((ns + 255) / 256);
+    bool first = false;
     if (!m->ref_ok || m->ref_ctx != c) {
         if (int rc = density_now(c))
             return rc;
@@ -1576,13 +1640,44 @@ int sph_surrogate_macro_step(sph_surrogate *m, sph_ctx *c, float tol, int *flags
         MCHK(hipGetLastError());
         m->ref_ok = true;
         m->ref_ctx = c;
-        m->since_plain.assign((size_t)ns, kTrendRefresh); // first macro-step: plain
+        first = true;
     }
-    std::vector<int> hf((size_t)ns);
-    for (int s = 0; s < ns; s++)
-        hf[s] = m->since_plain[s] >= kTrendRefresh ? 1 : 0;
+    const unsigned long long k = m->macro++;
+    std::vector<int> hf((size_t)ns), shadow((size_t)ns, 0);
+    int nshadow = 0;
+    for (int s = 0; s < ns; s++) {
+        hf[s] = first || (k + (unsigned)s) % kTrendRefresh == 0 ? 1 : 0;
+        if (m->shadow_every > 0 && !hf[s] && (k + (unsigned)s) % (unsigned)m->shadow_every == 0 &&
+            c->sc[s].n > 0) {
+            shadow[s] = 1;
+            nshadow++;
+        }
+    }
     if (sph_snapshot_save(c))
         return merr(SPH_EHIP, "snapshot failed: %s", sph_last_error());
+    const unsigned S = c->slots;
+    if (nshadow) { // the shadowed scenes' plain macro-step, then back to the start
+        if (!m->rho_plain) {
+            if (int rc = malloc_dev(&m->rho_plain, S))
+                return rc;
+            if (int rc = malloc_dev(&m->rho_sur, S))
+                return rc;
+        }
+        std::vector<int> frozen((size_t)ns), none((size_t)ns, 0);
+        for (int s = 0; s < ns; s++)
+            frozen[s] = shadow[s] ? 0 : 1;
+        if (sph_set_scene_frozen(c, frozen.data()) || sph_step(c, 4) ||
+            sph_set_scene_frozen(c, none.data()))
+            return merr(SPH_EHIP, "shadow step failed: %s", sph_last_error());
+        if (int rc = density_now(c))
+            return rc;
+        k_scene_density<<<dim3((unsigned)ns), 256, 0, c->st>>>(c->dP, c->RHO, m->sh_sum, m->jmax);
+        k_rho_by_id<<<dim3(S / 256), 256, 0, c->st>>>(c->dP, c->BS, S, c->RHO, c->ID0,
+                                                      m->rho_plain);
+        MCHK(hipGetLastError());
+        if (sph_snapshot_restore(c, shadow.data()))
+            return merr(SPH_EHIP, "shadow restore failed: %s", sph_last_error());
+    }
     if (int rc = forward(m, c))
         return rc;
     if (int rc = sph_surrogate_apply(m, c))
@@ -1595,12 +1690,45 @@ int sph_surrogate_macro_step(sph_surrogate *m, sph_ctx *c, float tol, int *flags
     k_judge<<<jg, 256, 0, c->st>>>(c->dP, ns, m->jsum, m->ref, tol, 0, m->jflags);
     MCHK(hipGetLastError());
     MCHK(hipMemcpyAsync(hf.data(), m->jflags, sizeof(int) * ns, hipMemcpyDeviceToHost, c->st));
-    MCHK(hipStreamSynchronize(c->st));
-    int nf = 0;
-    for (int s = 0; s < ns; s++) {
-        nf += hf[s] != 0;
-        m->since_plain[s] = hf[s] ? 0 : m->since_plain[s] + 1;
+    if (nshadow) {
+        std::vector<float> sur((size_t)ns), pl((size_t)ns), err(2 * (size_t)ns);
+        k_rho_by_id<<<dim3(S / 256), 256, 0, c->st>>>(c->dP, c->BS, S, c->RHO, c->ID0, m->rho_sur);
+        MCHK(hipMemcpyAsync(m->jflags, shadow.data(), sizeof(int) * ns, hipMemcpyHostToDevice,
+                            c->st));
+        k_shadow_err<<<dim3((unsigned)ns), 256, 0, c->st>>>(c->dP, m->jflags, m->rho_plain,
+                                                            m->rho_sur, m->sh_err);
+        MCHK(hipGetLastError());
+        MCHK(hipMemcpyAsync(sur.data(), m->jsum, sizeof(float) * ns, hipMemcpyDeviceToHost, c->st));
+        MCHK(hipMemcpyAsync(pl.data(), m->sh_sum, sizeof(float) * ns, hipMemcpyDeviceToHost, c->st));
+        MCHK(hipMemcpyAsync(err.data(), m->sh_err, sizeof(float) * 2 * ns, hipMemcpyDeviceToHost,
+                            c->st));
+        MCHK(hipStreamSynchronize(c->st));
+        for (int s = 0; s < ns; s++) {
+            if (!shadow[s])
+                continue;
+            const double e = ((double)sur[s] - (double)pl[s]) / (double)pl[s];
+            const double rms = sqrt((double)err[2 * s] / (double)err[2 * s + 1]);
+            m->sh_stats[0] += 1.0;
+            if (hf[s] == 0) { // accepted by the trend judge: the error it let through
+                m->sh_stats[1] += 1.0;
+                m->sh_stats[2] += fabs(e) > tol ? 1.0 : 0.0;
+                m->sh_stats[3] += fabs(e);
+                m->sh_stats[4] = fmax(m->sh_stats[4], fabs(e));
+                m->sh_stats[5] += e;
+                m->sh_stats[6] += rms;
+                m->sh_stats[7] = fmax(m->sh_stats[7], rms);
+                if (fabs(e) > tol)
+                    hf[s] = 3;
+            }
+        }
+        // the judge's mean for a shadow failure is the surrogate's; the
+        // re-take below replaces it with the plain one (mode 2)
+    } else {
+        MCHK(hipStreamSynchronize(c->st));
     }
+    int nf = 0;
+    for (int s = 0; s < ns; s++)
+        nf += hf[s] != 0;
     if (nf) {
         std::vector<int> retake((size_t)ns), frozen((size_t)ns), none((size_t)ns, 0);
         for (int s = 0; s < ns; s++) {
@@ -1624,6 +1752,25 @@ int sph_surrogate_macro_step(sph_surrogate *m, sph_ctx *c, float tol, int *flags
         memcpy(flags, hf.data(), sizeof(int) * ns);
     if (nfallback)
         *nfallback = nf;
+    return SPH_OK;
+}
+
+int sph_surrogate_set_shadow(sph_surrogate *m, int every)
+{
+    if (!m || every < 0)
+        return merr(SPH_EINVAL, "sph_surrogate_set_shadow: null model or negative period");
+    m->shadow_every = every;
+    for (double &x : m->sh_stats)
+        x = 0.0;
+    return SPH_OK;
+}
+
+int sph_surrogate_shadow_stats(sph_surrogate *m, double *out)
+{
+    if (!m || !out)
+        return merr(SPH_EINVAL, "null argument");
+    for (int i = 0; i < 8; i++)
+        out[i] = m->sh_stats[i];
     return SPH_OK;
 }
 

@@ -58,6 +58,10 @@ static fluid_particle **g_ptrs;
 static int g_n;
 static int g_capacity; /* max_fluid_particles_local given to initParticles */
 static AABB_t g_box;   /* the box the context was created with */
+/* Do the caller's pointers address one array in order (ptrs[i] == ptrs[0] + i,
+ * as the reference's initParticles lays them out)?  Then the state is copied
+ * back into it in one piece. */
+static int g_contiguous;
 static int g_dirty = DIRTY_NEW_RUN;
 static int g_sync_interval = 1;
 static int g_sync_fields = SPH_SYNC_ALL;
@@ -117,6 +121,9 @@ static int shim_upload(fluid_particle **ptrs, param *params, AABB_t *box)
         g_n = n;
         g_dirty = DIRTY_NONE;
         g_steps_since_sync = 0;
+        g_contiguous = 1;
+        for (int i = 1; i < n && g_contiguous; i++)
+            g_contiguous = ptrs[i] == ptrs[0] + i;
     }
     return SPH_OK;
 }
@@ -187,32 +194,48 @@ static int shim_enter(int call, fluid_particle **ptrs, param *params, AABB_t *bo
     return SPH_OK;
 }
 
+/* Host buffer reused by every copy back (grown as needed). */
+static void *g_stage;
+static size_t g_stage_bytes;
+
+static void *stage(size_t bytes)
+{
+    if (bytes > g_stage_bytes) {
+        void *p = realloc(g_stage, bytes);
+        if (!p)
+            return NULL;
+        g_stage = p;
+        g_stage_bytes = bytes;
+    }
+    return g_stage;
+}
+
 static int shim_sync_now(void)
 {
-    if (!g_ctx || !g_ptrs)
+    if (!g_ctx || !g_ptrs || g_n == 0)
         return SPH_OK;
     int rc = SPH_OK;
-    size_t n = (size_t)(g_n ? g_n : 1);
+    size_t n = (size_t)g_n;
     if (g_sync_fields == SPH_SYNC_POSITIONS) {
-        float *xy = (float *)malloc(sizeof(float) * 2 * n);
+        float *xy = (float *)stage(sizeof(float) * 2 * n);
         if (!xy)
             return SPH_ENOMEM;
         rc = sph_download_positions(g_ctx, xy);
         if (!rc)
-            for (int i = 0; i < g_n; i++) { /* pointer index == id on one domain */
+            for (size_t i = 0; i < n; i++) { /* pointer index == id on one domain */
                 g_ptrs[i]->x = xy[2 * i];
                 g_ptrs[i]->y = xy[2 * i + 1];
             }
-        free(xy);
+    } else if (g_contiguous) {
+        rc = sph_download(g_ctx, g_ptrs[0]);
     } else {
-        fluid_particle *tmp = (fluid_particle *)malloc(sizeof(fluid_particle) * n);
+        fluid_particle *tmp = (fluid_particle *)stage(sizeof(fluid_particle) * n);
         if (!tmp)
             return SPH_ENOMEM;
         rc = sph_download(g_ctx, tmp);
         if (!rc)
-            for (int i = 0; i < g_n; i++)
+            for (size_t i = 0; i < n; i++)
                 *g_ptrs[i] = tmp[i];
-        free(tmp);
     }
     g_steps_since_sync = 0;
     return rc;
@@ -264,6 +287,9 @@ void sph_shim_release(void)
     g_sync_interval = 1;
     g_sync_fields = SPH_SYNC_ALL;
     g_reported = 0;
+    free(g_stage);
+    g_stage = NULL;
+    g_stage_bytes = 0;
 }
 
 sph_ctx *sph_shim_context(void) { return g_ctx; }
