@@ -483,6 +483,13 @@ int sph_surrogate_shadow_stats(sph_surrogate *m, double *out);
 int sph_surrogate_mfma_selftest(float *out256);
 /* Per-scene mean and maximum density of the last density pass. */
 int sph_scene_density(sph_ctx *ctx, float *mean, float *maxv);
+/* The same without any allocation (no counterpart in the reference): for
+ * callers that cannot allocate device memory - code under HIP graph capture,
+ * serving loops or harnesses that forbid hipMalloc.  The per-scene partial
+ * results use a fixed 32 KiB device array per GPU (at most 4096 scenes; more
+ * is rejected with SPH_EINVAL).  Calls that overlap on one GPU race; the
+ * context's device must be current. */
+int sph_scene_density_noalloc(sph_ctx *ctx, float *mean, float *maxv);
 
 /* Trajectory store for training, resident in device memory: frames of the
  * state (x, y, v_x, v_y per particle) of nscenes scenes of n particles.

@@ -162,6 +162,7 @@ def lib():
             "sph_gather_state_dev": (ip, [vp, vp]),
             "sph_load_state_dev": (ip, [vp, vp]),
             "sph_scene_density": (ip, [vp, vp, vp]),
+            "sph_scene_density_noalloc": (ip, [vp, vp, vp]),
             "sph_surrogate_last_error": (cp, []),
             "sph_surrogate_num_params": (ip, []),
             "sph_surrogate_create": (ip, [ctypes.POINTER(vp), ctypes.c_uint64]),

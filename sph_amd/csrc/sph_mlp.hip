@@ -28,6 +28,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -1820,3 +1821,76 @@ int sph_scene_density(sph_ctx *c, float *mean, float *maxv)
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------------------
+// sph_scene_density without allocation.  The per-scene partial results (the
+// density sum and maximum of every scene) go to a fixed device array instead
+// of a buffer allocated per call.  The array's name is a host-side stand-in
+// for its device copy: host code only ever uses the address that
+// hipGetSymbolAddress returns for the current device, looked up once per
+// device on the first call there.  Every call on a device shares that
+// device's array and overwrites the entries it reads back (one sum and one
+// maximum per scene, each written by that scene's workgroup), so nothing
+// needs clearing; calls that overlap on one device race.
+
+constexpr int kNoAllocMaxScenes = 4096; // 32 KiB of partial results per device
+static __device__ __align__(256) float scratch[2 * kNoAllocMaxScenes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[kMaxDevices]; // per device: the device address of `scratch`
+
+int scratch_for_current_device(float **out)
+{
+    int dev = 0;
+    MCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return merr(SPH_ENODEV, "sph_scene_density_noalloc: device %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return merr(SPH_EHIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                        hipGetErrorString(e));
+        g_scratch_ptr[dev] = (float *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SPH_OK;
+}
+
+} // namespace
+
+extern "C" int sph_scene_density_noalloc(sph_ctx *c, float *mean, float *maxv)
+{
+    if (!c || !mean || !maxv)
+        return merr(SPH_EINVAL, "null argument");
+    const size_t ns = c->sc.size();
+    if (ns > (size_t)kNoAllocMaxScenes)
+        return merr(SPH_EINVAL,
+                    "sph_scene_density_noalloc: %zu scenes exceed the fixed scratch of %d scenes "
+                    "(use sph_scene_density)",
+                    ns, kNoAllocMaxScenes);
+    int dev = 0;
+    MCHK(hipGetDevice(&dev));
+    if (dev != c->device)
+        return merr(SPH_EINVAL, "sph_scene_density_noalloc: the context lives on device %d, "
+                                "device %d is current", c->device, dev);
+    float *buf = nullptr;
+    if (int rc = scratch_for_current_device(&buf))
+        return rc;
+    if (int rc = sph_internal_canonicalize(c))
+        return rc;
+    k_scene_density<<<dim3((unsigned)ns), 256, 0, c->st>>>(c->dP, c->RHO, buf, buf + ns);
+    MCHK(hipGetLastError());
+    float h[2 * kNoAllocMaxScenes];
+    MCHK(hipMemcpyAsync(h, buf, sizeof(float) * 2 * ns, hipMemcpyDeviceToHost, c->st));
+    MCHK(hipStreamSynchronize(c->st));
+    for (size_t s = 0; s < ns; s++) {
+        mean[s] = c->sc[s].n ? h[s] / (float)c->sc[s].n : 0.0f;
+        maxv[s] = h[ns + s];
+    }
+    return SPH_OK;
+}
