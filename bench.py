@@ -316,8 +316,9 @@ def profile_summary(db, last=0):
     """Text summary of a rocprofv3 results database (its default output):
     per kernel, the dispatch count and mean / total duration from the kernel
     trace, and the mean per dispatch of every collected counter (FETCH_SIZE
-    and WRITE_SIZE are in KiB).  `last` keeps only the last N dispatches of
-    the trace (the steady state)."""
+    and WRITE_SIZE are in KiB).  `last` keeps only the library's last N
+    dispatches of the trace (the steady state; the runtime's own blit
+    kernels are left out)."""
     import sqlite3
     con = sqlite3.connect(db)
     names = {r[0] for r in con.execute("select name from sqlite_master")}
@@ -326,7 +327,8 @@ def profile_summary(db, last=0):
     if "kernels" in names:
         rows = con.execute("select name, dispatch_id, start, end, duration from kernels "
                            "order by start").fetchall()
-        rows = rows[-last:] if last else rows
+        if last:  # the library's last N dispatches (runtime blit kernels left out)
+            rows = [r for r in rows if not short(r[0]).startswith("__amd")][-last:]
         agg = {}
         for name, _, _, _, dur in rows:
             agg.setdefault(short(name), []).append(dur)
@@ -358,19 +360,29 @@ def profile_summary(db, last=0):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_c2_dam_break.txt")
 
 
+def _short(k):
+    return re.sub(r"\(.*", "", k).replace("void ", "")
+
+
+def _window_ids(rows, last):
+    """Dispatch ids of the library's last `last` dispatches (the runtime's
+    blit kernels left out), or of every dispatch when last is 0."""
+    lib = sorted({d for d, k in rows if not _short(k).startswith("__amd")})
+    return set(lib[-last:]) if last else {d for d, _ in rows}
+
+
 def _counter_means(db, counter, last=0):
     """Per kernel: {dispatch id: value} of one counter (summed over its
-    records), over the last `last` dispatches of the run when given."""
+    records), over the library's last `last` dispatches when given."""
     import sqlite3
     con = sqlite3.connect(db)
     rows = con.execute("select dispatch_id, kernel_name, value from counters_collection "
                        "where counter_name = ?", (counter,)).fetchall()
-    ids = sorted({d for d, _, _ in rows})
-    keep = set(ids[-last:]) if last else set(ids)
+    keep = _window_ids([(d, k) for d, k, _ in rows], last)
     out = {}
     for d, k, v in rows:
         if d in keep:
-            m = out.setdefault(re.sub(r"\(.*", "", k).replace("void ", ""), {})
+            m = out.setdefault(_short(k), {})
             m[d] = m.get(d, 0.0) + v
     return out
 
@@ -388,8 +400,7 @@ def counter_summary(db, last=0, simds=1024, xcds=8):
     con = sqlite3.connect(db)
     rows = con.execute("select dispatch_id, kernel_name, counter_name, value "
                        "from counters_collection").fetchall()
-    ids = sorted({r[0] for r in rows})
-    keep = set(ids[-last:] if last else ids)
+    keep = _window_ids([(r[0], r[1]) for r in rows], last)
     agg, disp, recs = {}, {}, {}
     for d, k, cn, v in rows:
         if d not in keep:
@@ -420,13 +431,15 @@ def traffic_summary(fetch_db, write_db, copy_mb, last=0):
     counter run (KiB per dispatch), each scaled by the factor that makes the
     same counter read the known size of the bench's calibration copy
     (--calib-copy-mb: a device-to-device copy of copy_mb MB, which reads and
-    writes copy_mb MB).  With `last`, only the last N dispatches of each run
-    (the calibration copies come after the kernel window, so count them in)."""
+    writes copy_mb MB).  With `last`, only the library's last N dispatches of
+    each run (the calibration copies are taken from the whole run)."""
     fetch = _counter_means(fetch_db, "FETCH_SIZE", last)
     write = _counter_means(write_db, "WRITE_SIZE", last)
     known_kib = copy_mb * 1e6 / 1024.0
     cal = {}
-    for name, tab in (("fetch", fetch), ("write", write)):
+    # the calibration copies run after the window: take them from the whole run
+    for name, tab in (("fetch", _counter_means(fetch_db, "FETCH_SIZE")),
+                      ("write", _counter_means(write_db, "WRITE_SIZE"))):
         vals = [v for k, m in tab.items() if "copyBuffer" in k for v in m.values()
                 if v > 0.5 * known_kib]
         if not vals:
