@@ -349,8 +349,9 @@ int sph_create_batch(sph_ctx **out, const sph_scene_desc *scenes, int nscenes)
 #define A(p, count)                                                                              \
     if (!rc)                                                                                     \
     rc = dalloc(&c->p, count)
-    A(POS0, N); A(VEL, N); A(PREV, N); A(RHO, N);
-    A(POS1, N); A(UPOS, N); A(UVEL, N); A(T2a, N); A(T2b, N);
+    // the arrays the gathers walk carry kSphWalkPad spare records
+    A(POS0, N + kSphWalkPad); A(VEL, N + kSphWalkPad); A(PREV, N); A(RHO, N + kSphWalkPad);
+    A(POS1, N + kSphWalkPad); A(UPOS, N); A(UVEL, N); A(T2a, N); A(T2b, N);
     A(ID0, N); A(ID1, N); A(perm_src, N); A(perm_id, N); A(pos_of_id, N);
     A(K0, N); A(K1, N); A(ukey, N); A(uslot, N);
     A(cnt, MP); A(start, MP); A(bsum, sph_scan_blocks((unsigned)M));

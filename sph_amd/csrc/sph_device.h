@@ -90,17 +90,46 @@ __device__ __forceinline__ unsigned sph_cell_key(float x, float y, const sph_dev
 // arithmetic with the right result (the residuals are NaN or zero there).
 // Results equal sqrtf's (IEEE round-to-nearest) for every input: checked
 // exhaustively over all floats (sph_selftest_sqrt).
-__device__ __forceinline__ float sph_sqrt(float x)
+__device__ __forceinline__ float sph_sqrt_rounded(float xs)
 {
-    const bool tiny = x < 0x1p-96f;
-    const float xs = tiny ? x * 0x1p64f : x;
     const float s = __builtin_amdgcn_sqrtf(xs);
     const float sd = __int_as_float(__float_as_int(s) - 1);
     const float su = __int_as_float(__float_as_int(s) + 1);
     float r = s;
     r = __builtin_fmaf(-sd, s, xs) <= 0.0f ? sd : r;
     r = __builtin_fmaf(-su, s, xs) > 0.0f ? su : r;
-    return tiny ? r * 0x1p-32f : r;
+    return r;
+}
+
+// The scaling is taken only when some active lane of the wave needs it: a
+// pair closer than 2^-48 (coincident particles) is rare, and the common
+// path then costs the root and its correction alone.
+__device__ __forceinline__ float sph_sqrt(float x)
+{
+    if (__builtin_expect(__ballot(x < 0x1p-96f) != 0ull, 0)) {
+        const bool tiny = x < 0x1p-96f;
+        const float r = sph_sqrt_rounded(tiny ? x * 0x1p64f : x);
+        return tiny ? r * 0x1p-32f : r;
+    }
+    return sph_sqrt_rounded(x);
+}
+
+// Records a walk may read past the end of a candidate row (the test loop
+// works in whole groups of 8): every array a walk reads from global memory,
+// and every LDS tile, has this many spare records at its end.
+constexpr unsigned kSphWalkPad = 8;
+
+// One candidate's bit of an accepted-candidate mask: mask = 2 mask + !(r2 >
+// h2), the reference's pair test (src/hash.c:99, 185, 221; a NaN distance is
+// accepted, as there) as one compare into VCC and one add with carry.
+__device__ __forceinline__ unsigned sph_mask_push(unsigned mask, float r2, float h2)
+{
+    asm("v_cmp_nlt_f32_e32 vcc, %1, %2\n\t"
+        "v_addc_co_u32_e32 %0, vcc, %0, %0, vcc"
+        : "+v"(mask)
+        : "s"(h2), "v"(r2)
+        : "vcc");
+    return mask;
 }
 
 // Correctly rounded 1/x, branch-free, for x = +-0, +-inf, NaN and
@@ -340,7 +369,8 @@ struct sph_masks_replay {
 };
 
 // The same walk in two passes, for SIMD efficiency.  Pass 1 tests the
-// candidates 32 at a time (test(rec) -> within h), branch-free, into a mask
+// candidates 32 at a time (dist2(rec): squared distance, within h when not
+// above h2), branch-free, into a mask
 // of accepted ones, from which the particle's own slot `self` is cleared;
 // non-empty masks wait in a per-lane queue of up to three chunks (mask,
 // source index of bit 0, owner threshold).  Pass 2 drains the queue in the
@@ -360,9 +390,9 @@ struct sph_masks_replay {
 // that test for the candidate when OWN is set (false otherwise); idx is the
 // candidate's index in the source.  mk supplies or keeps the masks of the
 // lane's first chunks (sph_masks_*).  Returns (accepted, in the half list).
-template <bool OWN = false, class Src, class Test, class Body, class Masks>
+template <bool OWN = false, class Src, class Dist2, class Body, class Masks>
 __device__ __forceinline__ uint2 sph_walk_masked(unsigned self, const sph_rows &R, const Src &src,
-                                                 Test &&test, Body &&body, Masks &mk)
+                                                 float h2, Dist2 &&dist2, Body &&body, Masks &mk)
 {
     unsigned ci = 0; // chunks of this lane so far (all rows, walk order)
     unsigned m0 = 0, m1 = 0, m2 = 0; // pending masks, m0 drained first
@@ -414,9 +444,21 @@ __device__ __forceinline__ uint2 sph_walk_masked(unsigned self, const sph_rows &
                 if (mk.have(ci)) {
                     mask = mk.get(ci);
                 } else {
-#pragma unroll 8
-                    for (unsigned t = 0; t < m; t++)
-                        mask |= (unsigned)test(src.at(c + t)) << t;
+                    // groups of 8 candidates, pushed without predicates: the
+                    // last group reads up to 7 records past the row (sources
+                    // are padded by kSphWalkPad); the first candidate ends in
+                    // the top pushed bit, so a bit reversal and a shift give
+                    // candidate t bit t, and the padding's bits are cut off
+                    const unsigned groups = (m + 7u) >> 3;
+                    unsigned acc = 0;
+                    for (unsigned g = 0; g < groups; g++) {
+                        const unsigned base = c + 8u * g;
+#pragma unroll
+                        for (unsigned k = 0; k < 8u; k++)
+                            acc = sph_mask_push(acc, dist2(src.at(base + k)), h2);
+                    }
+                    mask = __builtin_bitreverse32(acc) >> (32u - 8u * groups);
+                    mask &= m < 32u ? (1u << m) - 1u : ~0u;
                     if (me - c < m) // the particle itself is no neighbour
                         mask &= ~(1u << (me - c));
                 }

@@ -302,9 +302,9 @@ __device__ __forceinline__ uint2 walk_group(const TilePlan &T, Rec *tile, Load l
     if (T.ok) {
         stage_tile(T, tile, load);
         if (live)
-            fh = sph_walk_masked<OWN>(self, R, lds_src(tile, T, key, P), test, body, mk);
+            fh = sph_walk_masked<OWN>(self, R, lds_src(tile, T, key, P), P.h2, test, body, mk);
     } else if (live) {
-        fh = sph_walk_masked<OWN>(self, R, global_src<Rec>(load), test, body, mk);
+        fh = sph_walk_masked<OWN>(self, R, global_src<Rec>(load), P.h2, test, body, mk);
     }
     return fh;
 }
@@ -333,7 +333,7 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
                float2 *__restrict__ held_vel)
 {
     __shared__ TilePlan T;
-    __shared__ float4 tile[VISC ? kTileCap : 1];
+    __shared__ float4 tile[VISC ? kTileCap + kSphWalkPad : 1];
     const unsigned blk = xcd_block();
     const sph_dev_params P = Pp[BS[blk]];
     const unsigned s0 = blk * kTPB;
@@ -381,7 +381,7 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
         };
         auto test = [&](float4 b) {
             const float dx = b.x - a.x, ddy = b.y - a.y;
-            return !(dx * dx + ddy * ddy > P.h2);
+            return dx * dx + ddy * ddy;
         };
         auto body = [&](unsigned, bool, float4 b) {
             float ix, iy;
@@ -689,7 +689,7 @@ k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
           uint4 *__restrict__ masks, sph_dev_counters *__restrict__ C)
 {
     __shared__ TilePlan T;
-    __shared__ float2 tile[kTileCap];
+    __shared__ float2 tile[kTileCap + kSphWalkPad];
     const unsigned blk = xcd_block();
     const sph_dev_params P = Pp[BS[blk]];
     const unsigned s0 = blk * kTPB;
@@ -707,7 +707,7 @@ k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     auto load = [&](unsigned g) { return POS[g]; };
     auto test = [&](float2 b) {
         const float dx = a.x - b.x, ddy = a.y - b.y;
-        return !(dx * dx + ddy * ddy > P.h2);
+        return dx * dx + ddy * ddy;
     };
     auto body = [&](unsigned, bool, float2 b) {
         const float dx = a.x - b.x, ddy = a.y - b.y;
@@ -744,7 +744,7 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
         unsigned long long *__restrict__ clock)
 {
     __shared__ TilePlan T;
-    __shared__ float4 tile[kTileCap];
+    __shared__ float4 tile[kTileCap + kSphWalkPad];
     const unsigned blk = xcd_block();
     const sph_dev_params P = Pp[BS[blk]];
     const unsigned s0 = blk * kTPB;
@@ -788,7 +788,7 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     float nx = a.x, ny = a.y;
     auto test = [&](float4 b) {
         const float dx = a.x - b.x, ddy = a.y - b.y;
-        return !(dx * dx + ddy * ddy > P.h2);
+        return dx * dx + ddy * ddy;
     };
     auto body = [&](unsigned, bool own, float4 b) {
         const float dx = a.x - b.x, ddy = a.y - b.y;
