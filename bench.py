@@ -187,10 +187,19 @@ def batched(steps, warmup, repeats=3):
     s = sa.Solver.batch([(t, b, len(p)) for t, b, p, _ in scenes])
     el, runs = timed_repeats(s, lambda: s.upload_batch(parts, counts), steps, warmup, repeats)
     n = int(counts.sum())
+    counters = s.counters()
+    # the gaps between the step's 11 launches: graph-replayed step time against
+    # the sum of the kernels' own durations (HIP events on each launch) over
+    # the steps right after the timed ones
+    kt = s.time_kernels(5)
+    kernel_sum = sum(v * KERNEL_LAUNCHES.get(k, 1) for k, v in kt.items() if k in KERNEL_LAUNCHES)
+    s.close()
     return {"value": n * steps / el, "unit": "particle-steps/s", "ms_per_step": el * 1e3 / steps,
             "steps": steps, "warmup": warmup, "scenes": len(scenes), "particles": n,
             "repeats_ms_per_step": [r * 1e3 / steps for r in runs],
-            "counters": s.counters()}
+            "kernels_ms": kt, "kernel_sum_ms": kernel_sum,
+            "launch_gap_ms": el * 1e3 / steps - kernel_sum,
+            "counters": counters}
 
 
 def surrogate_leg(step_eq, nscenes=64, n=65536, tol=0.01, seed=11, sample_every=100):
