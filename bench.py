@@ -67,8 +67,9 @@ L2_FILE = os.path.join(ROOT, "profiles", "l2_hit_c2_dam_break.txt")
 SHADOW_EVERY = 8
 
 # profiler names of the kernels (template instances of the fused step)
-KERNEL_SYMBOLS = {"visc_predict": "k_visc_predict<true, true, true>", "relax": "k_relax<true>",
-                  "density": "k_density", "reorder": "k_reorder<2>", "scatter": "k_scatter",
+KERNEL_SYMBOLS = {"visc_predict": "k_visc_predict<true, true, true, false>",
+                  "relax": "k_relax<true, false>", "density": "k_density<false>",
+                  "reorder": "k_reorder<2>", "scatter": "k_scatter",
                   "scan_apply": "k_scan_apply", "scan_reduce": "k_scan_reduce"}
 # launches of each kind per step
 KERNEL_LAUNCHES = {"visc_predict": 1, "scan_reduce": 2, "scan_apply": 2, "scatter": 2,
@@ -529,6 +530,34 @@ def late_regime(s, upload):
             "counters": s.counters()}
 
 
+def capped_line(s, upload, steps, warmup, repeats):
+    """The same workload under the reference's 100-slot buckets
+    (sph_set_bucket_cap): the timed steps as the headline times them, the
+    per-kernel window, and graph-replayed steps 601-700, where the first
+    cells pass 100 particles and the cap starts to drop particles."""
+    s.set_bucket_cap(sa.REF_MAX_BUCKET)
+    try:
+        el, runs = timed_repeats(s, upload, steps, warmup, repeats)
+        kt, _ = kernel_window(s, upload)
+        upload()
+        s.reset_counters()
+        s.step(600)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(100)
+        s.sync()
+        el_late = time.perf_counter() - t0
+        counters = s.counters()
+    finally:
+        s.set_bucket_cap(0)
+    n = s.n
+    return {"bucket_cap": sa.REF_MAX_BUCKET, "value": n * steps / el, "unit": "particle-steps/s",
+            "ms_per_step": el * 1e3 / steps, "repeats_ms_per_step": [r * 1e3 / steps for r in runs],
+            "kernels_ms": kt,
+            "steps_601_700": {"ms_per_step": el_late * 1e3 / 100, "value": n * 100 / el_late,
+                              "counters": counters}}
+
+
 def c1_line():
     """The N = 4096 configuration (64 x 64 lattice, box 60 x 40, 1000 steps):
     the GPU through the step graph, and the reference's own C step
@@ -567,15 +596,17 @@ def drop_in_line(steps=42):
     (no jitter: the reference's own lattice) in the 1200 x 800 box: wall time
     per step of the loop with the state copied back to the host every step
     (all fields), every 4 steps (the reference packs coordinates every 4th
-    substep), every 4 steps positions only, and never.  Runs the driver as a
-    child process."""
+    substep), every 4 steps positions only, and never - all with the grid
+    record's 100-slot buckets, as the reference sets them - and never with
+    uncapped buckets (-u).  Runs the driver as a child process."""
     import subprocess
     exe = os.path.join(ROOT, "sph_amd", "sph_main")
     if not os.path.exists(exe):
         return {"skipped": "sph_amd/sph_main not built"}
     out = {}
     for name, args in (("sync_every_step", ["-s", "1"]), ("sync_every_4", ["-s", "4"]),
-                       ("sync_every_4_positions", ["-s", "4", "-p"]), ("no_sync", ["-s", "0"])):
+                       ("sync_every_4_positions", ["-s", "4", "-p"]), ("no_sync", ["-s", "0"]),
+                       ("no_sync_uncapped", ["-s", "0", "-u"])):
         r = subprocess.run([exe, "-t", *args, "-b", "1200,800", "1024", "1024", str(steps)],
                            capture_output=True, text=True, timeout=300)
         m = re.search(r"([0-9.]+) ms/step, ([0-9.e+]+) particle-steps/s", r.stdout)
@@ -637,7 +668,9 @@ def main():
                     help="run only the per-kernel timing window (for the profiler) and print "
                          "its table")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the late-regime, N=4096 and drop-in lines")
+                    help="skip the late-regime, bucket-cap, N=4096 and drop-in lines")
+    ap.add_argument("--bucket-cap", type=int, default=0,
+                    help="with --kernels-only: run under this bucket cap (the reference's: 100)")
     args = ap.parse_args()
     if args.profile_summary:
         print(profile_summary(args.profile_summary, args.last))
@@ -654,6 +687,8 @@ def main():
     if args.kernels_only:
         tun, box, parts = workload()
         s = sa.Solver(tun, box, len(parts))
+        if args.bucket_cap:
+            s.set_bucket_cap(args.bucket_cap)
         kt, cnt = kernel_window(s, lambda: s.upload(parts))
         if args.calib_copy_mb > 0:
             calibration_copy(args.calib_copy_mb)
@@ -684,6 +719,8 @@ def main():
     # (kinds this bench does not know, e.g. from another library build, are
     # left out)
     late = None if args.no_extra else late_regime(s, lambda: s.upload(parts))
+    capped = (None if args.no_extra else
+              capped_line(s, lambda: s.upload(parts), args.steps, args.warmup, args.repeats))
     kt_all, window_counters = kernel_window(s, lambda: s.upload(parts))
     kt = {k: ms for k, ms in kt_all.items() if k in KERNEL_BYTES}
     sx, sy = s.cells()[2]
@@ -735,6 +772,8 @@ def main():
     if late:
         late["step_roofline_frac"] = b_step * late["value"] / 1e9 / HBM_PEAK_GBS
         out["late_regime"] = late
+    if capped:
+        out["bucket_cap_100"] = capped
     if not args.no_batched:
         out["batched"] = batched(max(10, args.steps // 2), max(2, args.warmup // 2),
                                  args.repeats)

@@ -175,9 +175,10 @@ SPH_STATIC_ASSERT(sizeof(neighbor_grid_t) == 40, "neighbor_grid_t must be 40 byt
 #define SPH_ENODEV (-5)    /* no usable gfx950 device                       */
 
 /* Capacity limits the reference silently applies (src/fluid.c:173-175).
- * The device path has no cap; it counts how often the reference's caps
- * would have dropped work, because parity with the reference holds only
- * while those counts stay zero. */
+ * By default the device path has no cap; it counts how often the
+ * reference's caps would have dropped work, because parity with the
+ * reference's lists holds only while those counts stay zero.
+ * sph_set_bucket_cap reproduces the bucket cap on request. */
 #define SPH_REF_MAX_BUCKET 100
 #define SPH_REF_MAX_NEIGHBORS 400
 
@@ -237,6 +238,24 @@ int sph_set_scene_tunables(sph_ctx *ctx, int scene, const tunable_parameters *tu
  * (used to re-take single scenes of a batch). */
 int sph_set_scene_forcing(sph_ctx *ctx, int scene, float amplitude, float period);
 int sph_set_scene_frozen(sph_ctx *ctx, const int *flags);
+
+/* The reference's bucket cap (src/hash.c:153-164, max_bucket_size = 100 at
+ * src/fluid.c:174): each cell's bucket holds its first `cap` particles in
+ * insertion order, and a particle left out is in no bucket of that hash -
+ * it has no neighbours, is nobody's neighbour, gets density 0 and no
+ * relaxation displacement, and its next viscosity pass sees no pairs;
+ * gravity, prediction, boundary and velocity update still move it.  The
+ * rank is taken by id within the cell, which is the reference's insertion
+ * order when its pointer array is in id order (a single process:
+ * src/geometry.c:56 sets id = array index and nothing reorders the array;
+ * the multi-rank exchange renumbers at src/communication.c:240, 428).  cap = 0 (the default) turns the cap off; SPH_REF_MAX_BUCKET is
+ * the reference's value.  Applies to every scene of the context and to the
+ * solver's walks (viscosity, density, relaxation, sph_read_neighbors); the
+ * surrogate's learned step does not apply it.  A change applies at once:
+ * the next viscosity pass walks the last hash's sets under the new cap (the
+ * reference's cap never changes).  The 400-neighbour list cap is not
+ * reproduced. */
+int sph_set_bucket_cap(sph_ctx *ctx, int cap);
 /* Sloshing clock offset of one scene: its forcing is evaluated at
  * t = (steps since the last upload or load + steps) * dt, so a scene loaded
  * mid-trajectory keeps its phase. */

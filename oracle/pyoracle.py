@@ -113,6 +113,7 @@ def lib(real="f32"):
             ("gather_create", vp, [ctypes.POINTER(AABB), ctypes.POINTER(Tunables), ip]),
             ("gather_destroy", None, [vp]),
             ("gather_set_tunables", None, [vp, ctypes.POINTER(Tunables)]),
+            ("gather_set_cap", None, [vp, up]),
             ("gather_set_state", None, [vp, vp]),
             ("gather_get_state", None, [vp, vp]),
             ("gather_phase", ip, [vp, ip]),
@@ -199,6 +200,11 @@ class GatherSim:
 
     def set_tunables(self, tun):
         self.L.gather_set_tunables(self.h, ctypes.byref(_tun(tun)))
+
+    def set_cap(self, cap):
+        """The reference's bucket cap (0: off): particles ranked cap or more
+        by id in their cell are in no bucket of that hash."""
+        self.L.gather_set_cap(self.h, int(cap))
 
     def set_state(self, parts):
         p = np.ascontiguousarray(parts.astype(self.dtype) if parts.dtype != self.dtype else parts)

@@ -542,6 +542,8 @@ typedef struct {
     uint64_t bucket_overflow, neighbor_overflow, key_oor;
     unsigned max_cell, max_full_neighbors;
     uint64_t impulse_clamps; /* pair impulses clamped to +-5 (each end counts) */
+    unsigned cap;            /* 0: uncapped; else the reference's bucket size */
+    unsigned char *out;      /* by id: left out of the last hash's buckets (rank >= cap) */
 } ogat;
 
 ogat *OFN(gather_create)(const AABB_t *box, const tunable_parameters *tp, int n)
@@ -565,12 +567,14 @@ ogat *OFN(gather_create)(const AABB_t *box, const tunable_parameters *tp, int n)
     s->key = (unsigned *)calloc(nn, sizeof(unsigned));
     s->sorted = (int *)calloc(nn, sizeof(int));
     s->start = (unsigned *)calloc((size_t)s->ncells + 1, sizeof(unsigned));
-    if (!ok || !s->key || !s->sorted || !s->start) {
+    s->out = (unsigned char *)calloc(nn, 1);
+    if (!ok || !s->key || !s->sorted || !s->start || !s->out) {
         for (unsigned i = 0; i < sizeof(f) / sizeof(f[0]); i++)
             free(*f[i]);
         free(s->key);
         free(s->sorted);
         free(s->start);
+        free(s->out);
         free(s);
         return NULL;
     }
@@ -587,10 +591,27 @@ void OFN(gather_destroy)(ogat *s)
     free(s->key);
     free(s->sorted);
     free(s->start);
+    free(s->out);
     free(s);
 }
 
 void OFN(gather_set_tunables)(ogat *s, const tunable_parameters *tp) { tun_from(&s->t, tp); }
+
+/* The reference's bucket cap (src/hash.c:156-163): with cap > 0, a particle
+ * whose rank by id in its cell is cap or more is in no bucket of that hash -
+ * no neighbours, nobody's neighbour, no density - until the next hash.  A new
+ * cap applies at once, to the last hash's sets too (the reference's cap never
+ * changes; this is the device library's rule for a change between steps). */
+void OFN(gather_set_cap)(ogat *s, unsigned cap)
+{
+    s->cap = cap;
+    if (!s->lists_valid)
+        return;
+    for (unsigned k = 0; k < (unsigned)s->n; k++) {
+        const int i = s->sorted[k];
+        s->out[i] = cap && k - s->start[s->key[i]] >= cap;
+    }
+}
 
 /* State in/out as AoS indexed by id (P[id]).  set_state invalidates the
  * neighbour lists, like a fresh reference run. */
@@ -649,8 +670,11 @@ static void gather_sort(ogat *s)
     }
     unsigned *fill = (unsigned *)malloc(sizeof(unsigned) * (nc ? nc : 1));
     memcpy(fill, s->start, sizeof(unsigned) * nc);
-    for (int i = 0; i < s->n; i++)
-        s->sorted[fill[s->key[i]]++] = i;
+    for (int i = 0; i < s->n; i++) {
+        const unsigned k = fill[s->key[i]]++;
+        s->sorted[k] = i;
+        s->out[i] = s->cap && k - s->start[s->key[i]] >= s->cap;
+    }
     free(fill);
     s->lists_valid = 1;
 }
@@ -704,10 +728,10 @@ static void gather_viscosity(ogat *s)
         int nr = gather_ranges(s, s->key[a], lo, hi);
         REAL ax = s->x[a], ay = s->y[a], avx = s->vx[a], avy = s->vy[a];
         REAL nvx = avx, nvy = avy;
-        for (int r = 0; r < nr; r++)
+        for (int r = 0; r < nr && !s->out[a]; r++)
             for (unsigned k = lo[r]; k < hi[r]; k++) {
                 int b = s->sorted[k];
-                if (b == a)
+                if (b == a || s->out[b])
                     continue;
                 REAL dx = s->x[b] - ax, dy = s->y[b] - ay;
                 if (dx * dx + dy * dy > h2)
@@ -749,10 +773,10 @@ static void gather_hash(ogat *s, int compute_density)
         REAL ax = s->x[a], ay = s->y[a];
         REAL rho = RC(0.0), rhon = RC(0.0);
         unsigned full = 0, half = 0;
-        for (int r = 0; r < nr; r++)
+        for (int r = 0; r < nr && !s->out[a]; r++)
             for (unsigned k = lo[r]; k < hi[r]; k++) {
                 int b = s->sorted[k];
-                if (b == a)
+                if (b == a || s->out[b])
                     continue;
                 REAL dx = ax - s->x[b], dy = ay - s->y[b];
                 REAL r2 = dx * dx + dy * dy;
@@ -792,10 +816,10 @@ static void gather_relax(ogat *s, int fuse_update)
         REAL ax = s->x[a], ay = s->y[a];
         REAL aP = t->k * (s->rho[a] - t->rest_density), aPn = t->k_near * s->rhon[a];
         REAL nx = ax, ny = ay;
-        for (int r = 0; r < nr; r++)
+        for (int r = 0; r < nr && !s->out[a]; r++)
             for (unsigned k = lo[r]; k < hi[r]; k++) {
                 int b = s->sorted[k];
-                if (b == a)
+                if (b == a || s->out[b])
                     continue;
                 REAL bx = s->x[b], by = s->y[b];
                 REAL dx = ax - bx, dy = ay - by;
@@ -897,10 +921,10 @@ void OFN(gather_neighbor_sets)(const ogat *s, int *counts, int *lists, int strid
     for (int a = 0; a < s->n; a++) {
         unsigned lo[3], hi[3];
         int nr = gather_ranges(s, s->key[a], lo, hi), c = 0;
-        for (int r = 0; r < nr; r++)
+        for (int r = 0; r < nr && !s->out[a]; r++)
             for (unsigned k = lo[r]; k < hi[r]; k++) {
                 int b = s->sorted[k];
-                if (b == a)
+                if (b == a || s->out[b])
                     continue;
                 REAL dx = s->x[a] - s->x[b], dy = s->y[a] - s->y[b];
                 if (dx * dx + dy * dy > h2)

@@ -54,6 +54,8 @@ _FIELDS = ("x_prev", "y_prev", "x", "y", "v_x", "v_y", "a_x", "a_y",
 PARTICLE = np.dtype([(f, np.float32) for f in _FIELDS] + [("id", np.int32)])
 assert PARTICLE.itemsize == 52
 
+REF_MAX_BUCKET = 100  # the reference's bucket cap (src/fluid.c:174)
+
 SPH_FIELD = dict(x=0, y=1, v_x=2, v_y=3, x_prev=4, y_prev=5, density=6, density_near=7,
                  id=8, key=9)
 
@@ -123,6 +125,7 @@ def lib():
             "sph_set_scene_tunables": (ip, [vp, ip, ctypes.POINTER(Tunables)]),
             "sph_set_scene_forcing": (ip, [vp, ip, ctypes.c_float, ctypes.c_float]),
             "sph_set_scene_frozen": (ip, [vp, vp]),
+            "sph_set_bucket_cap": (ip, [vp, ip]),
             "sph_upload": (ip, [vp, vp, ip]),
             "sph_set_tunables": (ip, [vp, ctypes.POINTER(Tunables)]),
             "sph_step": (ip, [vp, ip]),
@@ -300,6 +303,11 @@ class Solver:
     def set_frozen(self, flags):
         f = np.ascontiguousarray(flags, dtype=np.int32)
         _check(lib().sph_set_scene_frozen(self._h, f.ctypes.data), "sph_set_scene_frozen")
+
+    def set_bucket_cap(self, cap: int = REF_MAX_BUCKET):
+        """The reference's per-cell bucket cap (src/hash.c:153-164); 0 turns
+        it off (the default)."""
+        _check(lib().sph_set_bucket_cap(self._h, int(cap)), "sph_set_bucket_cap")
 
     def set_scene_tunables(self, scene: int, t: Tunables):
         _check(lib().sph_set_scene_tunables(self._h, scene, ctypes.byref(t)), "sph_set_scene_tunables")

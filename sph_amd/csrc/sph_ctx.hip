@@ -94,6 +94,7 @@ void derive_params(sph_ctx *c)
         p.slosh_omega = S.slosh_omega;
         p.slosh_t0 = S.slosh_t0;
         p.frozen = S.frozen;
+        p.bucket_cap = c->bucket_cap;
         int e = 0;
         p.h_pow2 = std::frexp(p.h, &e) == 0.5f && std::fpclassify(p.h_recip) == FP_NORMAL;
     }
@@ -176,11 +177,12 @@ void enqueue_sort_tail(sph_ctx *c, const int *id_src, unsigned *key_dst)
 void enqueue_fused_step(sph_ctx *c, bool visc)
 {
     const unsigned S = c->slots;
+    const bool cap = c->bucket_cap != 0;
     {
         Tick t(c, KV);
-        sph_launch_visc_predict(c->st, S, true, visc, true, c->dPA, c->BS, c->POS0, c->VEL, c->K0,
-                                c->start, c->UPOS, c->ukey, c->uslot, c->cnt, c->dC, c->dclock,
-                                c->UVEL);
+        sph_launch_visc_predict(c->st, S, true, visc, true, cap, c->dPA, c->BS, c->POS0, c->VEL,
+                                c->K0, c->start, c->UPOS, c->ukey, c->uslot, c->cnt, c->dC,
+                                c->dclock, c->UVEL);
     }
     enqueue_sort_tail(c, c->ID0, c->K1);
     {
@@ -191,13 +193,13 @@ void enqueue_fused_step(sph_ctx *c, bool visc)
     }
     {
         Tick t(c, KDE);
-        sph_launch_density(c->st, S, c->dP, c->BS, c->POS1, c->K1, c->start, c->RHO, c->MASK,
-                           c->dC);
+        sph_launch_density(c->st, S, cap, c->dP, c->BS, c->POS1, c->K1, c->start, c->RHO,
+                           c->MASK, c->dC);
     }
     {
         Tick t(c, KRE);
-        sph_launch_relax(c->st, S, true, c->dP, c->BS, c->POS1, c->PREV, c->RHO, c->K1, c->start,
-                         c->UPOS, c->UVEL, c->ukey, c->uslot, c->cnt, c->MASK, c->dC,
+        sph_launch_relax(c->st, S, true, cap, c->dP, c->BS, c->POS1, c->PREV, c->RHO, c->K1,
+                         c->start, c->UPOS, c->UVEL, c->ukey, c->uslot, c->cnt, c->MASK, c->dC,
                          c->dclock);
     }
     enqueue_sort_tail(c, c->ID1, c->K0);
@@ -625,6 +627,19 @@ int sph_set_scene_frozen(sph_ctx *c, const int *frozen)
     return push_params(c);
 }
 
+int sph_set_bucket_cap(sph_ctx *c, int cap)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (cap < 0)
+        return set_err(SPH_EINVAL, "sph_set_bucket_cap: cap %d < 0", cap);
+    HIPCHK(hipStreamSynchronize(c->st));
+    if ((c->bucket_cap != 0) != (cap != 0))
+        free_graphs(c); // the captured steps launch the other kernel variants
+    c->bucket_cap = (unsigned)cap;
+    return push_params(c);
+}
+
 int sph_step_eager(sph_ctx *c, int nsteps)
 {
     if (int rc = check_ctx(c))
@@ -765,9 +780,9 @@ int sph_phase_viscosity(sph_ctx *c)
         return rc;
     if (!c->lists_valid || c->n_total == 0)
         return SPH_OK; // empty lists (src/fluid.c:202)
-    sph_launch_visc_predict(c->st, c->slots, false, true, false, c->dP, c->BS, c->POS0, c->VEL,
-                            c->K0, c->start, c->UVEL, nullptr, nullptr, c->cnt, c->dC, c->dclock,
-                            nullptr);
+    sph_launch_visc_predict(c->st, c->slots, false, true, false, c->bucket_cap != 0, c->dP, c->BS,
+                            c->POS0, c->VEL, c->K0, c->start, c->UVEL, nullptr, nullptr, c->cnt,
+                            c->dC, c->dclock, nullptr);
     HIPCHK(hipMemcpyAsync(c->VEL, c->UVEL, sizeof(float2) * (size_t)c->slots,
                           hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipGetLastError());
@@ -808,8 +823,8 @@ int sph_phase_hash(sph_ctx *c, int compute_density)
     HIPCHK(hipMemcpyAsync(c->K0, c->K1, sizeof(unsigned) * (size_t)S, hipMemcpyDeviceToDevice,
                           c->st));
     if (compute_density)
-        sph_launch_density(c->st, S, c->dP, c->BS, c->POS0, c->K0, c->start, c->RHO, c->MASK,
-                           c->dC);
+        sph_launch_density(c->st, S, c->bucket_cap != 0, c->dP, c->BS, c->POS0, c->K0, c->start,
+                           c->RHO, c->MASK, c->dC);
     HIPCHK(hipGetLastError());
     c->lists_valid = true;
     c->hashes++;
@@ -826,9 +841,9 @@ int sph_phase_relax(sph_ctx *c)
         return SPH_OK;
     if (!c->lists_valid)
         return set_err(SPH_ESTATE, "sph_phase_relax: no hash since upload");
-    sph_launch_relax(c->st, c->slots, false, c->dP, c->BS, c->POS0, c->PREV, c->RHO, c->K0,
-                     c->start, c->UPOS, nullptr, nullptr, nullptr, nullptr, nullptr, c->dC,
-                     c->dclock);
+    sph_launch_relax(c->st, c->slots, false, c->bucket_cap != 0, c->dP, c->BS, c->POS0, c->PREV,
+                     c->RHO, c->K0, c->start, c->UPOS, nullptr, nullptr, nullptr, nullptr,
+                     nullptr, c->dC, c->dclock);
     HIPCHK(hipMemcpyAsync(c->POS0, c->UPOS, sizeof(float2) * (size_t)c->slots,
                           hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipGetLastError());

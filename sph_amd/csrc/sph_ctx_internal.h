@@ -38,6 +38,7 @@ struct sph_ctx {
     // the first half's and must be refreshed before the next fused step)
     sph_dev_params *dPA = nullptr;
     bool dPA_stale = false;
+    unsigned bucket_cap = 0; // the reference's bucket cap (0: none)
     float2 *XY = nullptr; // positions by id for sph_download_positions (allocated on first use)
     int *BS = nullptr; // workgroup -> scene
     sph_dev_counters *dC = nullptr;

@@ -37,6 +37,9 @@ struct sph_dev_params {
     int frozen;        // 1: the fused step leaves this scene's state unchanged
     int h_pow2;        // 1: h is a power of two with a normal reciprocal, so that
                        //    x / h == x * h_recip for every x (both round x 2^-k once)
+    unsigned bucket_cap; // 0, or the reference's per-cell bucket cap: a particle
+                         // ranked bucket_cap or later by id in its cell is in
+                         // no neighbour set of the hash (src/hash.c:153-164)
 };
 
 // Horizontal box acceleration of a sloshing scene at the step counted by
@@ -390,10 +393,21 @@ struct sph_masks_replay {
 // that test for the candidate when OWN is set (false otherwise); idx is the
 // candidate's index in the source.  mk supplies or keeps the masks of the
 // lane's first chunks (sph_masks_*).  Returns (accepted, in the half list).
-template <bool OWN = false, class Src, class Dist2, class Body, class Masks>
+//
+// CAP applies the reference's bucket cap `cap`: a candidate ranked cap or
+// later in its cell (its sorted position minus the cell's start) is masked
+// out, and a lane whose own rank is cap or more walks nothing.  A separate
+// instantiation, so that the uncapped walk keeps its registers.
+template <bool OWN = false, bool CAP = false, class Src, class Dist2, class Body, class Masks>
 __device__ __forceinline__ uint2 sph_walk_masked(unsigned self, const sph_rows &R, const Src &src,
-                                                 float h2, Dist2 &&dist2, Body &&body, Masks &mk)
+                                                 float h2, unsigned cap, Dist2 &&dist2,
+                                                 Body &&body, Masks &mk)
 {
+    // bits t of a chunk starting at c with c + t < x
+    auto below = [](unsigned c, unsigned x) {
+        return x <= c ? 0u : x - c >= 32u ? ~0u : (1u << (x - c)) - 1u;
+    };
+    const bool dropped = CAP && self - R.b1[1] >= cap;
     unsigned ci = 0; // chunks of this lane so far (all rows, walk order)
     unsigned m0 = 0, m1 = 0, m2 = 0; // pending masks, m0 drained first
     unsigned x0 = 0, x1 = 0, x2 = 0; // source index of each mask's bit 0
@@ -432,7 +446,12 @@ __device__ __forceinline__ uint2 sph_walk_masked(unsigned self, const sph_rows &
         const unsigned thr = dy == 1 ? R.b1[dy + 1] : dy == 0 ? self + 1u : R.b2[dy + 1];
         const unsigned tt = thr + off, me = self + off;
         const unsigned b3 = R.b3[dy + 1] + off;
-        unsigned c = R.b0[dy + 1] + off;
+        unsigned c = dropped ? b3 : R.b0[dy + 1] + off;
+        // does the cap cut one of the row's three cells?  (rarely: then the
+        // chunk masks below are cut to each cell's first cap positions)
+        const bool cut = CAP && (R.b1[dy + 1] - R.b0[dy + 1] > cap ||
+                                 R.b2[dy + 1] - R.b1[dy + 1] > cap ||
+                                 R.b3[dy + 1] - R.b2[dy + 1] > cap);
         // The chunk loop runs while any lane of the wave has candidates left
         // in this row (lanes that are done contribute empty masks), so a
         // drain is always taken by the whole wave at once: lanes' pair work
@@ -461,6 +480,16 @@ __device__ __forceinline__ uint2 sph_walk_masked(unsigned self, const sph_rows &
                     mask &= m < 32u ? (1u << m) - 1u : ~0u;
                     if (me - c < m) // the particle itself is no neighbour
                         mask &= ~(1u << (me - c));
+                    if (cut) {
+                        // the capped part [a, e) of each cell, source index space
+                        const unsigned a0 = R.b0[dy + 1] + off, a1 = R.b1[dy + 1] + off,
+                                       a2 = R.b2[dy + 1] + off;
+                        const unsigned e0 = a1 - a0 > cap ? a0 + cap : a1;
+                        const unsigned e1 = a2 - a1 > cap ? a1 + cap : a2;
+                        const unsigned e2 = b3 - a2 > cap ? a2 + cap : b3;
+                        mask &= (below(c, e0) & ~below(c, a0)) | (below(c, e1) & ~below(c, a1)) |
+                                (below(c, e2) & ~below(c, a2));
+                    }
                 }
                 mk.put(ci, mask);
                 ci++;

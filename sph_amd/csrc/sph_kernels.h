@@ -14,8 +14,9 @@ struct sph_reorder_fields {
 // Workgroup size of every particle kernel; scenes are padded to multiples.
 constexpr unsigned kSphTPB = 256;
 
+// cap: the reference's bucket cap is on (sph_dev_params::bucket_cap != 0)
 void sph_launch_visc_predict(hipStream_t st, unsigned slots, bool grav, bool visc, bool pred,
-                             const sph_dev_params *P, const int *BS, const float2 *POS,
+                             bool cap, const sph_dev_params *P, const int *BS, const float2 *POS,
                              const float2 *VEL, const unsigned *K, const unsigned *start,
                              float2 *opos, unsigned *okey, unsigned *oslot, unsigned *cnt,
                              sph_dev_counters *C, const unsigned long long *clock,
@@ -48,14 +49,15 @@ void sph_launch_reorder4(hipStream_t st, unsigned slots, const sph_dev_params *P
                          const unsigned *start, const int *perm_src, const int *perm_id,
                          const unsigned *skey, sph_reorder_fields<4> f, int *id_dst,
                          sph_dev_counters *C);
-void sph_launch_density(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
-                        const float2 *POS, const unsigned *K, const unsigned *start, float2 *RHO,
-                        uint4 *masks, sph_dev_counters *C);
-void sph_launch_relax(hipStream_t st, unsigned slots, bool fused, const sph_dev_params *P,
-                      const int *BS, const float2 *POS, const float2 *PREV, const float2 *RHO,
-                      const unsigned *K, const unsigned *start, float2 *opos, float2 *ovel,
-                      unsigned *okey, unsigned *oslot, unsigned *cnt, const uint4 *masks,
-                      sph_dev_counters *C, unsigned long long *clock);
+void sph_launch_density(hipStream_t st, unsigned slots, bool cap, const sph_dev_params *P,
+                        const int *BS, const float2 *POS, const unsigned *K,
+                        const unsigned *start, float2 *RHO, uint4 *masks, sph_dev_counters *C);
+void sph_launch_relax(hipStream_t st, unsigned slots, bool fused, bool cap,
+                      const sph_dev_params *P, const int *BS, const float2 *POS,
+                      const float2 *PREV, const float2 *RHO, const unsigned *K,
+                      const unsigned *start, float2 *opos, float2 *ovel, unsigned *okey,
+                      unsigned *oslot, unsigned *cnt, const uint4 *masks, sph_dev_counters *C,
+                      unsigned long long *clock);
 void sph_launch_pack_coords(hipStream_t st, unsigned slots, const sph_dev_params *P,
                             const int *BS, const float2 *POS, const int *ID, short *out);
 void sph_launch_neighbor_lists(hipStream_t st, unsigned slots, const sph_dev_params *P,

@@ -45,6 +45,13 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kTPB = 256;
+// The gathers ask the compiler for at least 6 resident workgroups per CU
+// (6 waves per SIMD, at most 80 VGPRs): it then schedules them in 62-80
+// VGPRs without scratch, where left alone it took 72-84 (5-6 waves).  LDS
+// tiles allow 7 workgroups.  Measured on the 1M dam break at steps 221-230:
+// density -2%, relaxation -4%, the capped variants -3 to -7%, viscosity
+// unchanged.
+constexpr int kGatherMinBlocks = 6;
 #ifndef SPH_TILE_CAP
 #define SPH_TILE_CAP 1280
 #endif
@@ -292,7 +299,7 @@ __device__ __forceinline__ GlobalSrc<Rec, Load> global_src(Load l)
 // else from global memory (same order, same results).  Every thread of the
 // workgroup calls this (the staging and its barrier need all of them);
 // `live` lanes walk.
-template <bool OWN, class Rec, class Load, class Test, class Body, class Masks>
+template <bool OWN, bool CAP, class Rec, class Load, class Test, class Body, class Masks>
 __device__ __forceinline__ uint2 walk_group(const TilePlan &T, Rec *tile, Load load, bool live,
                                             unsigned self, const sph_rows &R, unsigned key,
                                             const sph_dev_params &P, Test &test, Body &body,
@@ -302,9 +309,11 @@ __device__ __forceinline__ uint2 walk_group(const TilePlan &T, Rec *tile, Load l
     if (T.ok) {
         stage_tile(T, tile, load);
         if (live)
-            fh = sph_walk_masked<OWN>(self, R, lds_src(tile, T, key, P), P.h2, test, body, mk);
+            fh = sph_walk_masked<OWN, CAP>(self, R, lds_src(tile, T, key, P), P.h2, P.bucket_cap,
+                                           test, body, mk);
     } else if (live) {
-        fh = sph_walk_masked<OWN>(self, R, global_src<Rec>(load), P.h2, test, body, mk);
+        fh = sph_walk_masked<OWN, CAP>(self, R, global_src<Rec>(load), P.h2, P.bucket_cap, test,
+                                       body, mk);
     }
     return fh;
 }
@@ -322,8 +331,8 @@ __device__ __forceinline__ uint2 walk_group(const TilePlan &T, Rec *tile, Load l
 // PRED: x_prev = x; x += v*dt; boundary; then the new cell key and a slot
 //       in that cell (histogram of the next sort).  Without PRED the
 //       post-viscosity velocity is written to out_pos.
-template <bool GRAV, bool VISC, bool PRED>
-__global__ void __launch_bounds__(kTPB)
+template <bool GRAV, bool VISC, bool PRED, bool CAP>
+__global__ void __launch_bounds__(kTPB, kGatherMinBlocks)
 k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
                const float2 *__restrict__ POS, const float2 *__restrict__ VEL,
                const unsigned *__restrict__ K, const unsigned *__restrict__ start,
@@ -391,7 +400,7 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
         };
         plan_tile_par(s0, s1, K, start, P, T);
         sph_masks_none mk;
-        const uint2 fh = walk_group<false>(T, tile, load, live, s, R, key, P, test, body, mk);
+        const uint2 fh = walk_group<false, CAP>(T, tile, load, live, s, R, key, P, test, body, mk);
         full = fh.x;
         half = fh.y;
     }
@@ -682,7 +691,8 @@ k_reorder(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
 // ---------------------------------------------------------------------------
 // Density (src/hash.c:190-194, 225-229 with src/fluid.c:527-539) in gather
 // form: each particle sums (1-r/h)^2 and (1-r/h)^3 over its neighbours.
-__global__ void __launch_bounds__(kTPB)
+template <bool CAP>
+__global__ void __launch_bounds__(kTPB, kGatherMinBlocks)
 k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
           const float2 *__restrict__ POS, const unsigned *__restrict__ K,
           const unsigned *__restrict__ start, float2 *__restrict__ RHO,
@@ -719,7 +729,7 @@ k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
         rhon = in ? rhon + OmR2 * (1.0f - ratio) : rhon;
     };
     sph_masks_record mk;
-    const uint2 fh = walk_group<false>(T, tile, load, live, s, R, key, P, test, body, mk);
+    const uint2 fh = walk_group<false, CAP>(T, tile, load, live, s, R, key, P, test, body, mk);
     if (live) {
         RHO[s] = make_float2(rho, rhon);
         masks[s] = mk.m; // for the relaxation of the same step
@@ -732,8 +742,8 @@ k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
 // FUSED adds the velocity update (src/fluid.c:642-653): boundary, then
 // v = (x - x_prev)/dt clamped, and the histogram of the next sort; without
 // FUSED only the relaxed positions are written.
-template <bool FUSED>
-__global__ void __launch_bounds__(kTPB)
+template <bool FUSED, bool CAP>
+__global__ void __launch_bounds__(kTPB, kGatherMinBlocks)
 k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
         const float2 *__restrict__ POS, const float2 *__restrict__ PREV,
         const float2 *__restrict__ RHO, const unsigned *__restrict__ K,
@@ -811,10 +821,10 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     };
     if (FUSED) { // the density pass of this step walked the same neighbour set
         sph_masks_replay mk{live ? masks[s] : make_uint4(0u, 0u, 0u, 0u)};
-        walk_group<true>(T, tile, load, live, s, R, key, P, test, body, mk);
+        walk_group<true, CAP>(T, tile, load, live, s, R, key, P, test, body, mk);
     } else {
         sph_masks_none mk;
-        walk_group<true>(T, tile, load, live, s, R, key, P, test, body, mk);
+        walk_group<true, CAP>(T, tile, load, live, s, R, key, P, test, body, mk);
     }
     if (!live)
         return;
@@ -869,16 +879,19 @@ k_neighbor_lists(const sph_dev_params *__restrict__ Pp, const int *__restrict__ 
     const int me = (int)P.uoff + ID[s];
     const float2 a = POS[s];
     int full = 0, half = 0;
-    sph_walk(s, K[s], P, start, global_src<float2>([&](unsigned g) { return POS[g]; }),
-             [&](unsigned k, int dy, unsigned b1, unsigned b2, float2 b) {
-                 const float dx = a.x - b.x, ddy = a.y - b.y;
-                 if (k == s || dx * dx + ddy * ddy > P.h2)
-                     return;
-                 if (full < stride)
-                     ids_by_id[(size_t)me * stride + full] = ID[k];
-                 full++;
-                 half += sph_owns(k, s, dy, b1, b2);
-             });
+    // the bucket cap: rank in the cell = sorted position - the cell's start
+    auto dropped = [&](unsigned k) { return P.bucket_cap && k - start[K[k]] >= P.bucket_cap; };
+    if (!dropped(s))
+        sph_walk(s, K[s], P, start, global_src<float2>([&](unsigned g) { return POS[g]; }),
+                 [&](unsigned k, int dy, unsigned b1, unsigned b2, float2 b) {
+                     const float dx = a.x - b.x, ddy = a.y - b.y;
+                     if (k == s || dx * dx + ddy * ddy > P.h2 || dropped(k))
+                         return;
+                     if (full < stride)
+                         ids_by_id[(size_t)me * stride + full] = ID[k];
+                     full++;
+                     half += sph_owns(k, s, dy, b1, b2);
+                 });
     full_by_id[me] = full;
     half_by_id[me] = half;
 }
@@ -1088,27 +1101,36 @@ k_load_state(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
 }
 
 void sph_launch_visc_predict(hipStream_t st, unsigned slots, bool grav, bool visc, bool pred,
-                             const sph_dev_params *P, const int *BS, const float2 *POS,
+                             bool cap, const sph_dev_params *P, const int *BS, const float2 *POS,
                              const float2 *VEL, const unsigned *K, const unsigned *start,
                              float2 *opos, unsigned *okey, unsigned *oslot, unsigned *cnt,
                              sph_dev_counters *C, const unsigned long long *clock,
                              float2 *held_vel)
 {
-#define VP(G, V, R)                                                                              \
-    launch(k_visc_predict<G, V, R>, GRID(slots), kTPB, st, P, BS, POS, VEL, K, start, opos, okey, \
-                                                          oslot, cnt, C, clock, held_vel)
+#define VP(G, V, R, CAP)                                                                         \
+    launch(k_visc_predict<G, V, R, CAP>, GRID(slots), kTPB, st, P, BS, POS, VEL, K, start, opos, \
+           okey, oslot, cnt, C, clock, held_vel)
+#define VPC(G, V, R)                                                                             \
+    do {                                                                                         \
+        if (cap)                                                                                 \
+            VP(G, V, R, true);                                                                   \
+        else                                                                                     \
+            VP(G, V, R, false);                                                                  \
+    } while (0)
+    // without viscosity the walk does not run: no capped variant
     if (grav && visc && pred)
-        VP(true, true, true);
+        VPC(true, true, true);
     else if (grav && !visc && pred)
-        VP(true, false, true);
+        VP(true, false, true, false);
     else if (!grav && visc && !pred)
-        VP(false, true, false);
+        VPC(false, true, false);
     else if (!grav && visc && pred)
-        VP(false, true, true);
+        VPC(false, true, true);
     else if (!grav && !visc && pred)
-        VP(false, false, true);
+        VP(false, false, true, false);
     else
-        VP(true, true, false);
+        VPC(true, true, false);
+#undef VPC
 #undef VP
 }
 
@@ -1183,25 +1205,35 @@ void sph_launch_reorder4(hipStream_t st, unsigned slots, const sph_dev_params *P
                                                C);
 }
 
-void sph_launch_density(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
-                        const float2 *POS, const unsigned *K, const unsigned *start, float2 *RHO,
-                        uint4 *masks, sph_dev_counters *C)
+void sph_launch_density(hipStream_t st, unsigned slots, bool cap, const sph_dev_params *P,
+                        const int *BS, const float2 *POS, const unsigned *K,
+                        const unsigned *start, float2 *RHO, uint4 *masks, sph_dev_counters *C)
 {
-    launch(k_density, GRID(slots), kTPB, st, P, BS, POS, K, start, RHO, masks, C);
+    if (cap)
+        launch(k_density<true>, GRID(slots), kTPB, st, P, BS, POS, K, start, RHO, masks, C);
+    else
+        launch(k_density<false>, GRID(slots), kTPB, st, P, BS, POS, K, start, RHO, masks, C);
 }
 
-void sph_launch_relax(hipStream_t st, unsigned slots, bool fused, const sph_dev_params *P,
-                      const int *BS, const float2 *POS, const float2 *PREV, const float2 *RHO,
-                      const unsigned *K, const unsigned *start, float2 *opos, float2 *ovel,
-                      unsigned *okey, unsigned *oslot, unsigned *cnt, const uint4 *masks,
-                      sph_dev_counters *C, unsigned long long *clock)
+void sph_launch_relax(hipStream_t st, unsigned slots, bool fused, bool cap,
+                      const sph_dev_params *P, const int *BS, const float2 *POS,
+                      const float2 *PREV, const float2 *RHO, const unsigned *K,
+                      const unsigned *start, float2 *opos, float2 *ovel, unsigned *okey,
+                      unsigned *oslot, unsigned *cnt, const uint4 *masks, sph_dev_counters *C,
+                      unsigned long long *clock)
 {
-    if (fused)
-        launch(k_relax<true>, GRID(slots), kTPB, st, P, BS, POS, PREV, RHO, K, start, opos, ovel,
-                                                    okey, oslot, cnt, masks, C, clock);
+#define RL(F, CAP)                                                                               \
+    launch(k_relax<F, CAP>, GRID(slots), kTPB, st, P, BS, POS, PREV, RHO, K, start, opos, ovel,   \
+           okey, oslot, cnt, masks, C, clock)
+    if (fused && cap)
+        RL(true, true);
+    else if (fused)
+        RL(true, false);
+    else if (cap)
+        RL(false, true);
     else
-        launch(k_relax<false>, GRID(slots), kTPB, st, P, BS, POS, PREV, RHO, K, start, opos, ovel,
-                                                     okey, oslot, cnt, masks, C, clock);
+        RL(false, false);
+#undef RL
 }
 
 void sph_launch_pack_coords(hipStream_t st, unsigned slots, const sph_dev_params *P,

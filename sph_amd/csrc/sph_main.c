@@ -5,8 +5,8 @@
  * It demonstrates the drop-in: the loop body below is the reference's loop
  * body (src/fluid.c:273-365) minus its MPI lines.
  *
- *   sph_main [-s N] [-p] [-t] [-b W,H] nx ny steps [out.bin]
- *   sph_main [-s N] [-p] [-t] -f frames.bin steps [out.bin]
+ *   sph_main [-s N] [-p] [-t] [-u] [-b W,H] nx ny steps [out.bin]
+ *   sph_main [-s N] [-p] [-t] [-u] -f frames.bin steps [out.bin]
  *
  * Lattice mode: an nx x ny lattice at spacing 0.5 (h = 1) filling the
  * lower-left of a box sized as the benchmark scenes (60 x 40 for 64 x 64)
@@ -20,6 +20,9 @@
  *   -s N  copy the state back to the host every N steps (default 1; 0 never)
  *   -p    copy back positions only (the fields the loop reads)
  *   -t    print the loop's wall time per step after its first two steps
+ *   -u    uncapped buckets (grid max_bucket_size 0); by default the grid
+ *         record carries the reference's 100 (src/fluid.c:174), which the
+ *         entry points apply as its bucket cap
  *
  * The final particles are written as raw fluid_particle records to out.bin.
  *
@@ -85,21 +88,22 @@ static double now_s(void)
 static void usage(const char *argv0)
 {
     fprintf(stderr,
-            "usage: %s [-s N] [-p] [-t] [-b W,H] nx ny steps [out.bin]\n"
-            "       %s [-s N] [-p] [-t] -f frames.bin steps [out.bin]\n",
+            "usage: %s [-s N] [-p] [-t] [-u] [-b W,H] nx ny steps [out.bin]\n"
+            "       %s [-s N] [-p] [-t] [-u] -f frames.bin steps [out.bin]\n",
             argv0, argv0);
 }
 
 int main(int argc, char **argv)
 {
-    int sync_interval = 1, positions_only = 0, timing = 0, opt;
+    int sync_interval = 1, positions_only = 0, timing = 0, uncapped = 0, opt;
     const char *frames_path = NULL;
     float bw = 0.0f, bh = 0.0f;
-    while ((opt = getopt(argc, argv, "s:ptb:f:")) != -1) {
+    while ((opt = getopt(argc, argv, "s:ptub:f:")) != -1) {
         switch (opt) {
         case 's': sync_interval = atoi(optarg); break;
         case 'p': positions_only = 1; break;
         case 't': timing = 1; break;
+        case 'u': uncapped = 1; break;
         case 'b':
             if (sscanf(optarg, "%f,%f", &bw, &bh) != 2 || bw <= 0.0f || bh <= 0.0f) {
                 usage(argv[0]);
@@ -184,6 +188,8 @@ int main(int argc, char **argv)
     neighbor_grid_t grid; /* the reference's grid record; lists stay on the device */
     memset(&grid, 0, sizeof grid);
     grid.spacing = params.tunable_params.smoothing_radius;
+    grid.max_bucket_size = uncapped ? 0u : SPH_REF_MAX_BUCKET; /* src/fluid.c:174-175 */
+    grid.max_neighbors = SPH_REF_MAX_NEIGHBORS;
     grid.size_x = (unsigned)(boundary.max_x / grid.spacing + 0.999999f);
     grid.size_y = (unsigned)(boundary.max_y / grid.spacing + 0.999999f);
     if (!particles || !ptrs || !coords) {

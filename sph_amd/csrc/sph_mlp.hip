@@ -1230,8 +1230,8 @@ int density_now(sph_ctx *c)
         return rc;
     if (!c->lists_valid)
         return merr(SPH_ESTATE, "surrogate: state not hashed");
-    sph_launch_density(c->st, c->slots, c->dP, c->BS, c->POS0, c->K0, c->start, c->RHO, c->MASK,
-                       c->dC);
+    sph_launch_density(c->st, c->slots, c->bucket_cap != 0, c->dP, c->BS, c->POS0, c->K0,
+                       c->start, c->RHO, c->MASK, c->dC);
     MCHK(hipGetLastError());
     return SPH_OK;
 }

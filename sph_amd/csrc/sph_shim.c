@@ -30,7 +30,9 @@
  * code that reads p->x afterwards (src/fluid.c:356-363) sees the current
  * state: all fields, or only x and y (sph_shim_set_sync_fields), which is
  * all the reference's loop reads.  neighbor / bucket storage is left
- * untouched: the neighbour sets live on the device.
+ * untouched: the neighbour sets live on the device.  The grid's
+ * max_bucket_size is applied as the reference's bucket cap
+ * (sph_set_bucket_cap); 0 runs uncapped.
  *
  * After sph_shim_mark_host_dirty() the next call re-uploads the caller's
  * particles (growing the context when their number exceeds its capacity)
@@ -58,6 +60,7 @@ static fluid_particle **g_ptrs;
 static int g_n;
 static int g_capacity; /* max_fluid_particles_local given to initParticles */
 static AABB_t g_box;   /* the box the context was created with */
+static int g_cap_set = -1; /* bucket cap last given to the context (-1: none yet) */
 /* Do the caller's pointers address one array in order (ptrs[i] == ptrs[0] + i,
  * as the reference's initParticles lays them out)?  Then the state is copied
  * back into it in one piece. */
@@ -100,6 +103,7 @@ static int shim_upload(fluid_particle **ptrs, param *params, AABB_t *box)
         rc = sph_create(&g_ctx, params, &b, cap);
         if (rc)
             return rc;
+        g_cap_set = -1;
         g_capacity = cap;
         g_box = b;
         if (g_dirty == DIRTY_NONE)
@@ -484,14 +488,30 @@ void identify_oob_particles(fluid_particle **fluid_particle_pointers,
     (void)params;
 }
 
+/* The grid's max_bucket_size is the reference's bucket cap (src/hash.c:137,
+ * 153-164; src/fluid.c:174 sets 100): the context applies it from this hash
+ * on.  0 leaves the buckets uncapped. */
+static int shim_apply_cap(const neighbor_grid_t *grid)
+{
+    const unsigned v = grid ? grid->max_bucket_size : 0u;
+    const int cap = v > 0x7fffffffu ? 0x7fffffff : (int)v;
+    if (!g_ctx || cap == g_cap_set)
+        return SPH_OK;
+    int rc = sph_set_bucket_cap(g_ctx, cap);
+    if (!rc)
+        g_cap_set = cap;
+    return rc;
+}
+
 /* src/hash.c:127-242 */
 void hash_fluid(fluid_particle **fluid_particle_pointers, neighbor_grid_t *grid, param *params,
                 bool compute_density)
 {
-    (void)grid;
     int deferred;
     int call = compute_density ? CALL_HASH_DENSITY : CALL_HASH;
     int rc = shim_enter(call, fluid_particle_pointers, params, NULL, &deferred);
+    if (!rc)
+        rc = shim_apply_cap(grid);
     if (!rc && deferred && call == CALL_HASH) { /* a whole step recorded: run it fused */
         rc = sph_step_tunables2(g_ctx, &g_first, &g_second);
         g_seq = 0;

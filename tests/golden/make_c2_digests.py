@@ -12,6 +12,10 @@ what the repository keeps.
 
 Run in the build container (CPU only; about an hour):
     python tests/golden/make_c2_digests.py
+
+With --cap 100 the oracle applies the reference's 100-slot buckets
+(src/hash.c:153-164; the run first overflows a cell between steps 600 and
+610) and writes tests/golden/c2_digests_cap100.json at steps 650 and 1000.
 """
 import hashlib
 import json
@@ -60,18 +64,22 @@ def c2_particles():
 
 
 def main():
+    cap = int(sys.argv[sys.argv.index("--cap") + 1]) if "--cap" in sys.argv else 0
+    out = os.path.join(HERE, f"c2_digests_cap{cap}.json") if cap else OUT
+    checkpoints = (650, 1000) if cap else CHECKPOINTS
     tun = c2_tunables()
     box = po.make_box(1200, 800)
     parts = c2_particles()
     sim = po.GatherSim(box, tun, len(parts))
+    sim.set_cap(cap)
     sim.set_state(parts)
     res = {"workload": "1024 x 1024 lattice at 0.5h, box 1200 x 800, jitter 0.01h, seed 12345; "
                        "g 9.8, dt 1/60, rho0 10, k 0.004, k_near 0.01, k_spring 10, sigma 0, "
                        "beta 0.3, no mover; gather form of the oracle (float32)",
-           "seed": SEED, "checkpoints": {}}
+           "seed": SEED, "bucket_cap": cap, "checkpoints": {}}
     done = 0
     t0 = time.time()
-    for cp in CHECKPOINTS:
+    for cp in checkpoints:
         while done < cp:
             k = min(10, cp - done)
             sim.step(k)
@@ -79,9 +87,9 @@ def main():
             print(f"step {done}  {time.time() - t0:.0f} s  {sim.counters()}", flush=True)
         st = sim.get_state()
         res["checkpoints"][str(cp)] = {"fields": field_digests(st), "counters": sim.counters()}
-        with open(OUT + ".tmp", "w") as f:
+        with open(out + ".tmp", "w") as f:
             json.dump(res, f, indent=1)
-        os.replace(OUT + ".tmp", OUT)
+        os.replace(out + ".tmp", out)
         print("wrote checkpoint", cp, flush=True)
 
 

@@ -1,0 +1,154 @@
+"""The reference's bucket cap on the device (sph_set_bucket_cap) against the
+oracle's gather form with the same cap, bit for bit.
+
+The cap leaves a cell's particles ranked cap or later (by id) out of every
+neighbour set of that hash (src/hash.c:153-164).  The cases put the cut
+everywhere the device's walk can see it: inside a row's three cells at cap 3
+on the dam break, inside the second 32-candidate chunk of a cell on a dense
+lattice staged in LDS (caps 33 and 40), and in the fourth chunk on the
+overfull lattice that walks global memory (cap 100); the fixture of five
+101-particle clusters drops one particle each.  The fused step runs both
+captured and eager, and switching the cap on and off between captured steps
+must take the other kernels.
+"""
+import numpy as np
+import pytest
+
+import sph_amd as sa
+from oracle import pyoracle as po
+from tests.common import assert_bitexact, dam_break_tunables, golden, parts_from_snap, tun_from_bytes
+
+pytestmark = pytest.mark.gpu
+
+
+def make_pair(parts, tun, box, cap):
+    s = sa.Solver(sa.Tunables.from_buffer_copy(bytes(tun)), sa.AABB.from_buffer_copy(bytes(box)),
+                  len(parts))
+    s.upload(parts)
+    s.set_bucket_cap(cap)
+    g = po.GatherSim(box, tun, len(parts))
+    g.set_cap(cap)
+    g.set_state(parts.view(po.P32) if parts.dtype != po.P32 else parts)
+    return s, g
+
+
+def caps_case():
+    d = golden("caps.npz")
+    return parts_from_snap(d["input"]), tun_from_bytes(d["tunables"]), po.make_box(4, 4)
+
+
+def c1_case():
+    d = golden("c1_dambreak.npz")
+    return parts_from_snap(d["input"]), tun_from_bytes(d["tunables"]), po.make_box(60, 40)
+
+
+def test_caps_fixture_neighbor_sets_at_first_hash():
+    parts, tun, box = caps_case()
+    n = len(parts)
+    s, g = make_pair(parts, tun, box, sa.REF_MAX_BUCKET)
+    ref = po.RefSim(box, tun, n, max_bucket=100, max_neighbors=4096)
+    rp = parts.copy()
+    for ph, name in enumerate(po.PHASES[:4]):
+        ref.phase(rp, ph)
+        g.phase(ph)
+        s.phase(name)
+    full, half, ids = s.neighbors(stride=1024)
+    gc, gl = g.neighbor_sets(stride=1024)
+    np.testing.assert_array_equal(full, gc)
+    counts, _ = ref.lists(stride=4096)
+    np.testing.assert_array_equal(half, counts)
+    for i in range(n):
+        assert sorted(ids[i, :full[i]].tolist()) == sorted(gl[i, :gc[i]].tolist()), i
+    assert np.sum(full == 0) == 5
+    assert_bitexact(s.download(), g.get_state(), what="caps fixture, first hash")
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_caps_fixture_steps(graph):
+    parts, tun, box = caps_case()
+    s, g = make_pair(parts, tun, box, sa.REF_MAX_BUCKET)
+    for step in range(1, 11):
+        s.step(1, graph=graph)
+        g.step(1)
+        assert_bitexact(s.download(), g.get_state(), what=f"caps fixture, step {step}")
+    assert s.counters()["bucket_overflow"] == g.counters()["bucket_overflow"] > 0
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_dam_break_cap_3(graph):
+    parts, tun, box = c1_case()
+    s, g = make_pair(parts, tun, box, 3)
+    for upto in (1, 2, 5, 10, 20):
+        while s.counters()["steps"] < upto:
+            s.step(1, graph=graph)
+            g.step(1)
+        assert_bitexact(s.download(), g.get_state(), what=f"cap 3, step {upto}")
+
+
+@pytest.mark.parametrize("cap", [33, 40])
+def test_dense_lattice_cut_in_second_chunk(cap):
+    parts = sa.scene_lattice(128, 128, 0.15, 0.01, seed=7)
+    s, g = make_pair(parts, dam_break_tunables(), po.make_box(60, 40), cap)
+    for step in range(1, 4):
+        s.step(1, graph=False)
+        g.step(1)
+        assert_bitexact(s.download(), g.get_state(), what=f"dense, cap {cap}, step {step}")
+    s.step(2)
+    g.step(2)
+    assert_bitexact(s.download(), g.get_state(), what=f"dense, cap {cap}, graph steps 4-5")
+
+
+def test_overfull_lattice_cap_100():
+    parts = sa.scene_lattice(48, 48, 0.06, 0.01, seed=5)
+    s, g = make_pair(parts, dam_break_tunables(), po.make_box(60, 40), sa.REF_MAX_BUCKET)
+    for step in range(1, 4):
+        s.step(1, graph=False)
+        g.step(1)
+        assert_bitexact(s.download(), g.get_state(), what=f"overfull, cap 100, step {step}")
+    s.step(2)
+    g.step(2)
+    assert_bitexact(s.download(), g.get_state(), what="overfull, cap 100, graph steps 4-5")
+    assert s.counters()["max_cell_count"] > 200
+
+
+def test_switching_the_cap_between_captured_steps():
+    parts, tun, box = c1_case()
+    s, g = make_pair(parts, tun, box, 0)
+    for k, cap in enumerate((0, 3, 0, 5, 5)):
+        s.set_bucket_cap(cap)
+        g.set_cap(cap)
+        s.step(4)
+        g.step(4)
+        assert_bitexact(s.download(), g.get_state(), what=f"segment {k}, cap {cap}")
+
+
+@pytest.fixture
+def shim():
+    sa.lib().sph_shim_release()
+    yield
+    sa.lib().sph_shim_release()
+
+
+def test_reference_entry_points_apply_the_grid_cap(shim):
+    """The reference-named entry points take the cap from the grid record
+    (max_bucket_size, src/hash.c:137): initParticles builds a lattice at
+    0.09h (about 120 particles per cell), and 12 steps of the reference's
+    loop through them equal the capped oracle bit for bit."""
+    import ctypes
+    from tests.common import run_reference_loop
+    from tests.test_gpu_reference_scenes import Shim
+    t = dam_break_tunables()
+    box = sa.make_box(8, 8)
+    sh = Shim(sa.Tunables.from_buffer_copy(bytes(t)), box, 2000)
+    ctypes.c_uint.from_buffer(sh.grid, 36).value = sa.REF_MAX_BUCKET  # max_bucket_size
+    sh.init(sa.make_box(3.6, 3.65), 40, 0.09)  # 40 x 40 particles from the origin
+    assert sh.n == 1600
+    host = sh.parts[:sh.n].copy()
+    sa.lib().sph_shim_set_sync_interval(0)
+    g = po.GatherSim(box, t, sh.n)
+    g.set_cap(sa.REF_MAX_BUCKET)
+    g.set_state(host.view(po.P32))
+    run_reference_loop(sh.phase, lambda _: None, 12, [])
+    run_reference_loop(g.phase, lambda _: None, 12, [])
+    assert g.counters()["bucket_overflow"] > 0
+    assert_bitexact(sh.state(), g.get_state(), what="entry points with the grid's cap, 12 steps")

@@ -10,7 +10,11 @@
   order, has the SHA-256 that the oracle's gather form gives
   (tests/golden/c2_digests.json, written by make_c2_digests.py), and the
   cell occupancy counters agree - including whether the reference's
-  100-particle buckets would have overflowed (src/fluid.c:174).
+  100-particle buckets would have overflowed (src/fluid.c:174).  The same
+  run under the reference's 100-slot buckets (sph_set_bucket_cap) matches
+  the capped oracle's digests at steps 650 and 1000
+  (tests/golden/c2_digests_cap100.json): past step 605 the cap drops
+  particles, which then pile up in cells of over a thousand.
 """
 import hashlib
 import json
@@ -80,12 +84,14 @@ def field_digests(st):
             for f in FIELDS}
 
 
-def test_c2_1000_steps_digests():
-    path = os.path.join(GOLDEN, "c2_digests.json")
+@pytest.mark.parametrize("digests", ["c2_digests.json", "c2_digests_cap100.json"])
+def test_c2_1000_steps_digests(digests):
+    path = os.path.join(GOLDEN, digests)
     with open(path) as f:
         want = json.load(f)
     cps = sorted(int(k) for k in want["checkpoints"])
-    assert cps, "no checkpoints in c2_digests.json"
+    assert cps, f"no checkpoints in {digests}"
+    cap = want.get("bucket_cap", 0)
     tun = sa.make_tunables(rest_density=10.0, smoothing_radius=1.0, g=9.8, k=0.004, k_near=0.01,
                            k_spring=10.0, sigma=0.0, beta=0.3, time_step=1.0 / 60.0,
                            mover_type=0, mover_width=0.0, mover_height=0.0, active=1)
@@ -93,9 +99,10 @@ def test_c2_1000_steps_digests():
     parts = sa.scene_lattice(1024, 1024, 0.5, 0.01, seed=want["seed"])
     s = sa.Solver(tun, box, len(parts))
     s.upload(parts)
+    s.set_bucket_cap(cap)
     done = 0
     for cp in cps:
-        print("C2 steps to", cp, flush=True)
+        print(f"C2 (bucket cap {cap}) steps to", cp, flush=True)
         s.step(cp - done)
         done = cp
         st = s.download()

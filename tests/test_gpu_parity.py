@@ -266,8 +266,9 @@ def test_full_size_c2_bitexact_and_sorted():
 
 def test_reference_loop_through_shims_matches_step(tmp_path):
     """The C driver runs the reference's loop body (src/fluid.c:273-348)
-    through the reference-named entry points; its result equals the fused,
-    graph-replayed sph_step bit for bit."""
+    through the reference-named entry points, its grid record carrying the
+    reference's 100-slot buckets; its result equals the fused,
+    graph-replayed sph_step with that bucket cap bit for bit."""
     import os
     import subprocess
     exe = os.path.join(sa.PKG_DIR, "sph_main")
@@ -281,5 +282,6 @@ def test_reference_loop_through_shims_matches_step(tmp_path):
     s = sa.Solver(sa.Tunables.from_buffer_copy(bytes(tun)), sa.AABB.from_buffer_copy(bytes(box)),
                   len(parts))
     s.upload(parts)
+    s.set_bucket_cap(sa.REF_MAX_BUCKET)
     s.step(40)
     assert_bitexact(got, s.download(), what="shim loop vs sph_step")

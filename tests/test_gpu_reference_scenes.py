@@ -133,6 +133,7 @@ def test_reference_loop_driver_with_per_frame_tunables(tmp_path, sync):
     got = np.fromfile(out, sa.PARTICLE)
     assert len(got) == info["n"]
     g = po.GatherSim(box, t, len(parts))
+    g.set_cap(sa.REF_MAX_BUCKET)  # the driver's grid record carries the reference's 100
     g.set_state(np.ascontiguousarray(parts).view(po.P32))
     run_reference_loop(g.phase, g.set_tunables, 96, frames)
     assert_bitexact(got, g.get_state(), what="driver, 96 steps")

@@ -379,3 +379,177 @@ def test_ensemble_statistics_gather_vs_reference_1000_steps():
     assert np.abs(a["hist"] - b["hist"]).sum() < 0.05
     assert abs(a["mean_y"] - b["mean_y"]) < 0.02 * b["mean_y"]
     assert abs(a["ke"] - b["ke"]) <= 0.2 * max(a["ke"], b["ke"]) + 1e-3 * len(p)
+
+
+# ---- the reference's bucket cap --------------------------------------------
+
+def test_bucket_cap_neighbor_sets_match_reference_lists():
+    """With the reference's bucket cap (100 per cell, src/hash.c:156-163),
+    the gather form's neighbour sets at the first hash are the reference's
+    half lists (its restatement with that cap and no neighbour cap) seen from
+    both ends, on the fixture whose five clusters of 101 overflow their
+    buckets; the particles left out have no neighbours and zero density in
+    both, and densities agree to float rounding (the orders differ)."""
+    from tests.common import golden, parts_from_snap, tun_from_bytes
+    d = golden("caps.npz")
+    parts, tun = parts_from_snap(d["input"]), tun_from_bytes(d["tunables"])
+    box = po.make_box(4, 4)
+    n = len(parts)
+    ref = po.RefSim(box, tun, n, max_bucket=100, max_neighbors=4096)
+    rp = parts.copy()
+    g = po.GatherSim(box, tun, n)
+    g.set_cap(100)
+    g.set_state(parts)
+    for ph in range(4):
+        ref.phase(rp, ph)
+        g.phase(ph)
+    counts, lists = ref.lists(stride=4096)
+    want = [set() for _ in range(n)]
+    for i in range(n):
+        for q in lists[i, :counts[i]]:
+            want[i].add(int(q))
+            want[int(q)].add(i)
+    gc, gl = g.neighbor_sets(stride=4096)
+    for i in range(n):
+        assert set(gl[i, :gc[i]].tolist()) == want[i], i
+    out = [i for i in range(n) if not want[i]]
+    gs = g.get_state()
+    assert len(out) == 5  # one id per 101-particle cluster: the highest
+    assert np.all(gs["density"][out] == 0) and np.all(rp["density"][out] == 0)
+    np.testing.assert_allclose(gs["density"], rp["density"], rtol=1e-5, atol=1e-6)
+    assert g.counters()["bucket_overflow"] == ref.counters()["bucket_overflow"] > 0
+
+
+def test_bucket_cap_two_particles():
+    """cap 1: of two particles within h in one cell, the higher id is in no
+    bucket, so neither has a neighbour; in different cells both are in."""
+    tun = dam_break_tunables(g=0.0)
+    box = po.make_box(10, 10)
+    for xs, linked in (([2.2, 2.6], False), ([2.8, 3.3], True)):
+        p = np.zeros(2, po.P32)
+        p["x"], p["y"] = xs, [5.5, 5.5]
+        p["id"] = [0, 1]
+        g = po.GatherSim(box, tun, 2)
+        g.set_cap(1)
+        g.set_state(p)
+        g.phase(po.PH_GRAVITY)
+        g.phase(po.PH_VISCOSITY)
+        g.phase(po.PH_PREDICT)
+        g.phase(po.PH_HASH_DENSITY)
+        c, _ = g.neighbor_sets(stride=4)
+        assert list(c) == ([1, 1] if linked else [0, 0]), xs
+
+
+def _crowded_lattice(side=40, spacing=0.09, seed=4):
+    """side x side particles at `spacing` h with a little jitter, id = index:
+    about 123 per cell."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(side * side, po.P32)
+    j, i = np.divmod(np.arange(side * side), side)
+    p["x"] = (0.3 + i * spacing + rng.uniform(0, 0.01, side * side)).astype(np.float32)
+    p["y"] = (0.3 + j * spacing + rng.uniform(0, 0.01, side * side)).astype(np.float32)
+    p["id"] = np.arange(side * side)
+    return p
+
+
+@pytest.mark.skipif(not po.have_compiled_ref(), reason="oracle/_ref not built here")
+def test_bucket_cap_neighbor_sets_match_compiled_reference():
+    """The reference's own hash (oracle/_ref, 100-slot buckets) on a crowded
+    lattice, about 123 particles per cell and under 400 half-neighbours each:
+    its half lists at the first hash, seen from both ends, are the capped
+    gather form's neighbour sets."""
+    tun = dam_break_tunables(g=0.0)
+    box = po.make_box(6, 6)
+    p = _crowded_lattice()
+    n = len(p)
+    ref = po.CompiledRef(box, tun, n, max_bucket=100)
+    ref.load(p)
+    g = po.GatherSim(box, tun, n)
+    g.set_cap(100)
+    g.set_state(p)
+    for ph in range(4):
+        ref.phase(ph)
+        g.phase(ph)
+    counts, lists = ref.lists(stride=400)
+    assert counts.max() < 400  # no list was cut
+    want = [set() for _ in range(n)]
+    for i in range(n):
+        for q in lists[i, :counts[i]]:
+            want[i].add(int(q))
+            want[int(q)].add(i)
+    gc, gl = g.neighbor_sets(stride=1024)
+    for i in range(n):
+        assert set(gl[i, :gc[i]].tolist()) == want[i], i
+    # the particles with no neighbour are exactly each cell's excess over 100
+    keys_by_id, _, _ = g.cells()
+    excess = np.maximum(np.bincount(keys_by_id).astype(np.int64) - 100, 0).sum()
+    assert sum(1 for i in range(n) if not want[i]) == excess > 100
+
+
+def test_bucket_cap_dropped_particles_move_without_pairs():
+    """One step of the crowded lattice with 100-slot buckets: the particles
+    left out of the buckets at the step's hash have no pair terms, so both the
+    reference's step (restated, scatter form) and the capped gather form leave
+    them at their predicted positions, bit for bit, with the velocity the
+    update derives from them; every other particle moved further (the
+    two forms differ there by the reference's in-place relaxation order)."""
+    tun = dam_break_tunables()
+    box = po.make_box(6, 6)
+    p = _crowded_lattice(side=32)
+    n = len(p)
+    ref = po.RefSim(box, tun, n, max_bucket=100, max_neighbors=4096)
+    rp = p.copy()
+    g = po.GatherSim(box, tun, n)
+    g.set_cap(100)
+    g.set_state(p)
+    for ph in range(4):
+        ref.phase(rp, ph)
+        g.phase(ph)
+        if ph == po.PH_PREDICT:
+            predicted = rp.copy()
+    counts, lists = ref.lists(stride=4096)
+    linked = np.zeros(n, bool)
+    for i in range(n):
+        if counts[i]:
+            linked[i] = True
+            linked[lists[i, :counts[i]]] = True
+    out = ~linked
+    assert 0 < out.sum() < n
+    for ph in (po.PH_RELAX, po.PH_UPDATE_VELOCITIES):
+        ref.phase(rp, ph)
+        g.phase(ph)
+    gs = g.get_state()
+    order = np.argsort(gs["id"])
+    for f in ("x", "y", "v_x", "v_y", "x_prev", "y_prev"):
+        np.testing.assert_array_equal(gs[f][order][out], rp[f][out], err_msg=f)
+    np.testing.assert_array_equal(rp["x"][out], predicted["x"][out])
+    np.testing.assert_array_equal(rp["y"][out], predicted["y"][out])
+    moved = (rp["x"] != predicted["x"]) | (rp["y"] != predicted["y"])
+    assert moved[linked].mean() > 0.99
+
+
+def test_bucket_cap_set_after_a_hash_applies_to_its_sets():
+    """A cap set after a hash cuts that hash's sets as if it had been set
+    before it (the rule the device library follows for a change between
+    steps); taking it off again restores the full sets."""
+    from tests.common import golden, parts_from_snap, tun_from_bytes
+    d = golden("caps.npz")
+    parts, tun = parts_from_snap(d["input"]), tun_from_bytes(d["tunables"])
+    box = po.make_box(4, 4)
+    sims = []
+    for before in (True, False):
+        g = po.GatherSim(box, tun, len(parts))
+        g.set_state(parts)
+        if before:
+            g.set_cap(100)
+        for ph in range(4):
+            g.phase(ph)
+        sims.append(g)
+    full_before = sims[1].neighbor_sets(stride=1024)
+    sims[1].set_cap(100)
+    a, b = sims[0].neighbor_sets(stride=1024), sims[1].neighbor_sets(stride=1024)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    assert np.sum(a[0] == 0) == 5
+    sims[1].set_cap(0)
+    np.testing.assert_array_equal(sims[1].neighbor_sets(stride=1024)[0], full_before[0])
