@@ -171,10 +171,11 @@ __device__ __forceinline__ unsigned wave_slot(unsigned *cnt, unsigned key)
 // each row the sorted range of cells c0-1 .. c1+1.  Made by the whole
 // workgroup: every thread reads the two end keys, threads 0..11 read one
 // segment-bounding cell start each, and thread 0 assembles the plan from
-// those with no memory access of its own.  Ends with a barrier.
+// those with no memory access of its own.  Ends with a barrier.  The group
+// is staged when its rows hold at most `cap` records.
 __device__ void plan_tile_par(unsigned s0, unsigned s1, const unsigned *__restrict__ K,
                               const unsigned *__restrict__ start, const sph_dev_params &P,
-                              TilePlan &T)
+                              TilePlan &T, unsigned cap)
 {
     const unsigned sx = P.sx, sy = P.sy;
     const unsigned kf = K[s0] - P.cell_base, kl = K[s1 - 1] - P.cell_base;
@@ -218,7 +219,7 @@ __device__ void plan_tile_par(unsigned s0, unsigned s1, const unsigned *__restri
                     T.tb[q] = total;
                     total += b - a;
                 }
-            T.ok = total <= (unsigned)kTileCap;
+            T.ok = total <= cap;
             T.total = total;
         }
     }
@@ -230,10 +231,10 @@ __device__ void plan_tile_par(unsigned s0, unsigned s1, const unsigned *__restri
 // t, t + 256, ...: it finds each record's segment (the last one beginning at
 // or before it), issues all of its loads, and only then stores them - one
 // memory round trip per workgroup, not one per segment.
-template <class Rec, class Load>
+template <int CAPACITY, class Rec, class Load>
 __device__ __forceinline__ void stage_tile(const TilePlan &T, Rec *tile, Load load)
 {
-    constexpr int kPer = (kTileCap + kTPB - 1) / kTPB;
+    constexpr int kPer = (CAPACITY + kTPB - 1) / kTPB;
     const unsigned total = T.total;
     Rec v[kPer];
 #pragma unroll
@@ -295,11 +296,12 @@ __device__ __forceinline__ GlobalSrc<Rec, Load> global_src(Load l)
     return GlobalSrc<Rec, Load>{l};
 }
 
-// Walk a workgroup's candidates from its LDS tile if the plan staged one,
-// else from global memory (same order, same results).  Every thread of the
-// workgroup calls this (the staging and its barrier need all of them);
-// `live` lanes walk.
-template <bool OWN, bool CAP, class Rec, class Load, class Test, class Body, class Masks>
+// Walk a workgroup's candidates from its LDS tile (CAPACITY records) if the
+// plan staged one, else from global memory (same order, same results).  Every
+// thread of the workgroup calls this (the staging and its barrier need all of
+// them); `live` lanes walk.
+template <bool OWN, bool CAP, int CAPACITY, class Rec, class Load, class Test, class Body,
+          class Masks>
 __device__ __forceinline__ uint2 walk_group(const TilePlan &T, Rec *tile, Load load, bool live,
                                             unsigned self, const sph_rows &R, unsigned key,
                                             const sph_dev_params &P, Test &test, Body &body,
@@ -307,7 +309,7 @@ __device__ __forceinline__ uint2 walk_group(const TilePlan &T, Rec *tile, Load l
 {
     uint2 fh = make_uint2(0u, 0u);
     if (T.ok) {
-        stage_tile(T, tile, load);
+        stage_tile<CAPACITY>(T, tile, load);
         if (live)
             fh = sph_walk_masked<OWN, CAP>(self, R, lds_src(tile, T, key, P), P.h2, P.bucket_cap,
                                            test, body, mk);
@@ -398,9 +400,10 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
             nvx = push ? nvx - ix * 0.5f : nvx;
             nvy = push ? nvy - iy * 0.5f : nvy;
         };
-        plan_tile_par(s0, s1, K, start, P, T);
+        plan_tile_par(s0, s1, K, start, P, T, kTileCap);
         sph_masks_none mk;
-        const uint2 fh = walk_group<false, CAP>(T, tile, load, live, s, R, key, P, test, body, mk);
+        const uint2 fh =
+            walk_group<false, CAP, kTileCap>(T, tile, load, live, s, R, key, P, test, body, mk);
         full = fh.x;
         half = fh.y;
     }
@@ -699,7 +702,10 @@ k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
           uint4 *__restrict__ masks, sph_dev_counters *__restrict__ C)
 {
     __shared__ TilePlan T;
-    __shared__ float2 tile[kTileCap + kSphWalkPad];
+    // positions only: twice the candidates of the other gathers' float4
+    // tiles in the same LDS bytes, so denser rows stay in LDS at no cost in
+    // occupancy (registers allow 7 workgroups per CU either way)
+    __shared__ float2 tile[2 * kTileCap + kSphWalkPad];
     const unsigned blk = xcd_block();
     const sph_dev_params P = Pp[BS[blk]];
     const unsigned s0 = blk * kTPB;
@@ -712,7 +718,7 @@ k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     const bool live = s < s1;
     const unsigned key = live ? K[s] : P.cell_base; // pad lanes: any valid cell
     const sph_rows R = sph_row_bounds(key, P, start);
-    plan_tile_par(s0, s1, K, start, P, T);
+    plan_tile_par(s0, s1, K, start, P, T, 2 * kTileCap);
     float rho = 0.0f, rhon = 0.0f;
     auto load = [&](unsigned g) { return POS[g]; };
     auto test = [&](float2 b) {
@@ -729,7 +735,8 @@ k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
         rhon = in ? rhon + OmR2 * (1.0f - ratio) : rhon;
     };
     sph_masks_record mk;
-    const uint2 fh = walk_group<false, CAP>(T, tile, load, live, s, R, key, P, test, body, mk);
+    const uint2 fh =
+        walk_group<false, CAP, 2 * kTileCap>(T, tile, load, live, s, R, key, P, test, body, mk);
     if (live) {
         RHO[s] = make_float2(rho, rhon);
         masks[s] = mk.m; // for the relaxation of the same step
@@ -786,7 +793,7 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     const bool live = s < s1;
     const unsigned key = live ? K[s] : P.cell_base; // pad lanes: any valid cell
     const sph_rows R = sph_row_bounds(key, P, start);
-    plan_tile_par(s0, s1, K, start, P, T);
+    plan_tile_par(s0, s1, K, start, P, T, kTileCap);
     // a neighbour's record carries its pressure and near-pressure
     // (src/fluid.c:563-564), evaluated once per record rather than per pair
     auto load = [&](unsigned g) {
@@ -821,10 +828,10 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     };
     if (FUSED) { // the density pass of this step walked the same neighbour set
         sph_masks_replay mk{live ? masks[s] : make_uint4(0u, 0u, 0u, 0u)};
-        walk_group<true, CAP>(T, tile, load, live, s, R, key, P, test, body, mk);
+        walk_group<true, CAP, kTileCap>(T, tile, load, live, s, R, key, P, test, body, mk);
     } else {
         sph_masks_none mk;
-        walk_group<true, CAP>(T, tile, load, live, s, R, key, P, test, body, mk);
+        walk_group<true, CAP, kTileCap>(T, tile, load, live, s, R, key, P, test, body, mk);
     }
     if (!live)
         return;

@@ -1,18 +1,22 @@
 """Dense tiles: the gather kernels' two candidate paths on one scene.
 
 A workgroup stages the candidate rows of its 256 sorted particles in LDS
-when they fit the library's tile (1,024-2,048 records by build); a workgroup
-whose tile would not fit, or that spans more than two cell rows, walks the
-same candidates in global memory in the same order.  A dense lattice
+when they fit the library's tile (1,024-2,048 records by build; the density
+pass, whose records are positions only, twice as many in the same bytes); a
+workgroup whose tile would not fit, or that spans more than two cell rows,
+walks the same candidates in global memory in the same order.  A dense lattice
 (spacing 0.15h, about 44 particles per cell) makes tiles of 1,050-1,220
 records, and one workgroup spans several rows; it is compared bit for bit
 with the oracle.  The same scene then runs on the test build with
 256-record tiles (nearly every workgroup on the global-memory walk, XCD runs
 of one tile) in a child process, and must equal the default library bit for
 bit.  An overfull lattice (spacing 0.06h, about 280 particles per cell) takes
-every workgroup off LDS and the reorder's ranking past its staged ids, and
+the viscosity and relaxation workgroups off LDS (the density pass still
+stages most of them) and the reorder's ranking past its staged ids, and
 exceeds both of the reference's caps; it too is compared bit for bit, with the
-overflow counters.
+overflow counters.  A lattice at 0.1h (about 100 particles per cell) puts most
+tiles between the two capacities: density from LDS, the other gathers from
+global memory, bit for bit.
 """
 import os
 import subprocess
@@ -69,7 +73,7 @@ def test_overfull_cells_bitexact_against_oracle():
     100-slot buckets and 400-entry neighbour lists, both counted).  Cells this
     large reach past the ids the reorder stages around each workgroup (its
     positions +-64), so the ranking's global-memory path runs, and every tile
-    is too large for LDS."""
+    is too large for the viscosity and relaxation passes' LDS tiles."""
     parts = sa.scene_lattice(48, 48, 0.06, 0.01, seed=5)
     tun, box = dam_break_tunables(), po.make_box(60, 40)
     s = sa.Solver(sa.Tunables.from_buffer_copy(bytes(tun)), sa.AABB.from_buffer_copy(bytes(box)),
@@ -98,6 +102,26 @@ def test_overfull_cells_bitexact_against_oracle():
     for k in ("neighbor_overflow", "max_neighbors"):
         assert c[k] == before_last_hash[k], k
     assert before_last_hash["neighbor_overflow"] > 0
+
+
+def test_tiles_between_the_density_and_record_capacities():
+    """Spacing 0.1h: 16,384 particles, about 100 per cell, tiles of 1,200-1,800
+    candidates - staged by the density pass, read from global memory by the
+    viscosity and relaxation passes - bit-exact against the oracle."""
+    parts = sa.scene_lattice(128, 128, 0.1, 0.01, seed=7)
+    tun, box = dam_break_tunables(), po.make_box(60, 40)
+    s = sa.Solver(sa.Tunables.from_buffer_copy(bytes(tun)), sa.AABB.from_buffer_copy(bytes(box)),
+                  len(parts))
+    s.upload(parts)
+    g = po.GatherSim(box, tun, len(parts))
+    g.set_state(parts.view(po.P32))
+    for step in range(1, 4):
+        s.step(1, graph=False)
+        g.step(1)
+        assert_bitexact(s.download(), g.get_state(), what=f"0.1h lattice step {step}")
+    s.step(2)
+    g.step(2)
+    assert_bitexact(s.download(), g.get_state(), what="0.1h lattice, graph steps 4-5")
 
 
 def test_small_window_build_equals_default(tmp_path):
