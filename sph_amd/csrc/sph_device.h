@@ -398,17 +398,21 @@ struct sph_masks_replay {
 // later in its cell (its sorted position minus the cell's start) is masked
 // out, and a lane whose own rank is cap or more walks nothing.  A separate
 // instantiation, so that the uncapped walk keeps its registers.
-template <bool OWN = false, bool CAP = false, class Src, class Dist2, class Body, class Masks>
-__device__ __forceinline__ uint2 sph_walk_masked(unsigned self, const sph_rows &R, const Src &src,
-                                                 float h2, unsigned cap, Dist2 &&dist2,
-                                                 Body &&body, Masks &mk)
+//
+// DY0..DY1 limit the walk to those candidate rows, and ci counts the lane's
+// chunks across calls: walking the rows one call at a time (a tile staged row
+// by row) visits the same chunks in the same order as one call over all three.
+template <bool OWN = false, bool CAP = false, int DY0 = -1, int DY1 = 1, class Src, class Dist2,
+          class Body, class Masks>
+__device__ __forceinline__ uint2 sph_walk_rows(unsigned self, const sph_rows &R, const Src &src,
+                                               float h2, unsigned cap, Dist2 &&dist2, Body &&body,
+                                               Masks &mk, unsigned &ci)
 {
     // bits t of a chunk starting at c with c + t < x
     auto below = [](unsigned c, unsigned x) {
         return x <= c ? 0u : x - c >= 32u ? ~0u : (1u << (x - c)) - 1u;
     };
     const bool dropped = CAP && self - R.b1[1] >= cap;
-    unsigned ci = 0; // chunks of this lane so far (all rows, walk order)
     unsigned m0 = 0, m1 = 0, m2 = 0; // pending masks, m0 drained first
     unsigned x0 = 0, x1 = 0, x2 = 0; // source index of each mask's bit 0
     unsigned t0 = 0, t1 = 0, t2 = 0; // owner threshold, source index space
@@ -438,7 +442,7 @@ __device__ __forceinline__ uint2 sph_walk_masked(unsigned self, const sph_rows &
         nq = 0;
     };
 #pragma unroll
-    for (int dy = -1; dy <= 1; dy++) {
+    for (int dy = DY0; dy <= DY1; dy++) {
         // the row, its owner threshold and self in source index space (each
         // a true index in the row's range where it matters: the arithmetic
         // is modular, so a negative offset cancels)
@@ -521,4 +525,14 @@ __device__ __forceinline__ uint2 sph_walk_masked(unsigned self, const sph_rows &
     }
     drain();
     return make_uint2(full, half);
+}
+
+// All three candidate rows in one call.
+template <bool OWN = false, bool CAP = false, class Src, class Dist2, class Body, class Masks>
+__device__ __forceinline__ uint2 sph_walk_masked(unsigned self, const sph_rows &R, const Src &src,
+                                                 float h2, unsigned cap, Dist2 &&dist2,
+                                                 Body &&body, Masks &mk)
+{
+    unsigned ci = 0; // chunks of this lane so far (all rows, walk order)
+    return sph_walk_rows<OWN, CAP>(self, R, src, h2, cap, dist2, body, mk, ci);
 }

@@ -41,6 +41,8 @@
 
 #include <hip/hip_ext.h>
 
+#include <type_traits>
+
 namespace {
 
 constexpr int kWave = 64;
@@ -50,7 +52,8 @@ constexpr int kTPB = 256;
 // VGPRs without scratch, where left alone it took 72-84 (5-6 waves).  LDS
 // tiles allow 7 workgroups.  Measured on the 1M dam break at steps 221-230:
 // density -2%, relaxation -4%, the capped variants -3 to -7%, viscosity
-// unchanged.
+// unchanged.  The uncapped relaxation asks for 7, which keeps it at 72 VGPRs
+// (7 waves) with its row-by-row staging path; 6 let it grow to 74 (6 waves).
 constexpr int kGatherMinBlocks = 6;
 #ifndef SPH_TILE_CAP
 #define SPH_TILE_CAP 1280
@@ -127,6 +130,7 @@ __device__ __forceinline__ void add_pad(const sph_dev_params &P, unsigned s0, un
 // Tiles: the candidate ranges of one workgroup of sorted particles.
 struct TilePlan {
     int ok;           // 1: staged in LDS; 0: read candidates from global memory
+    int rows_ok;      // 1 (when not ok): each candidate row (dy) fits on its own
     unsigned jA;      // first cell row of the group
     unsigned g0[6];   // segment sorted begin, segment = (row - jA) * 3 + dy + 1
     unsigned len[6];  // segment length
@@ -197,6 +201,7 @@ __device__ void plan_tile_par(unsigned s0, unsigned s1, const unsigned *__restri
     if (t == 0) {
         T.jA = jA;
         T.ok = 0;
+        T.rows_ok = 0;
         for (int q = 0; q < 6; q++) {
             T.g0[q] = 0;
             T.len[q] = 0;
@@ -221,6 +226,8 @@ __device__ void plan_tile_par(unsigned s0, unsigned s1, const unsigned *__restri
                 }
             T.ok = total <= cap;
             T.total = total;
+            T.rows_ok = T.len[0] + T.len[3] <= cap && T.len[1] + T.len[4] <= cap &&
+                        T.len[2] + T.len[5] <= cap;
         }
     }
     __syncthreads();
@@ -257,6 +264,32 @@ __device__ __forceinline__ void stage_tile(const TilePlan &T, Rec *tile, Load lo
     __syncthreads();
 }
 
+// Stage only candidate row dy (segments dy + 1 of the first cell row and
+// dy + 4 of the second, back to back) into the tile, then barrier: for groups
+// whose rows fit one at a time but not all together.
+template <int CAPACITY, class Rec, class Load>
+__device__ __forceinline__ void stage_row(const TilePlan &T, int dy, Rec *tile, Load load)
+{
+    constexpr int kPer = (CAPACITY + kTPB - 1) / kTPB;
+    const int qa = dy + 1, qb = dy + 4;
+    const unsigned la = T.len[qa], total = la + T.len[qb];
+    const unsigned ga = T.g0[qa], gb = T.g0[qb];
+    Rec v[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+        const unsigned i = threadIdx.x + (unsigned)u * kTPB;
+        if (i < total)
+            v[u] = load(i < la ? ga + i : gb + (i - la));
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+        const unsigned i = threadIdx.x + (unsigned)u * kTPB;
+        if (i < total)
+            tile[i] = v[u];
+    }
+    __syncthreads();
+}
+
 // Candidate source reading the LDS tile: the tile index of sorted position
 // k of row dy is k + row(dy).
 template <class Rec>
@@ -277,6 +310,22 @@ __device__ __forceinline__ LdsSrc<Rec> lds_src(const Rec *tile, const TilePlan &
     for (int q = 0; q < 3; q++) {
         r.offA[q] = T.off[q];
         r.offB[q] = T.off[3 + q];
+    }
+    r.rowB = (key - P.cell_base) / P.sx != T.jA;
+    return r;
+}
+
+// The source of a tile staged by stage_row(dy): only row dy's offsets matter.
+template <class Rec>
+__device__ __forceinline__ LdsSrc<Rec> lds_row_src(const Rec *tile, const TilePlan &T, int dy,
+                                                  unsigned key, const sph_dev_params &P)
+{
+    LdsSrc<Rec> r;
+    r.tile = tile;
+    const int qa = dy + 1, qb = dy + 4;
+    for (int q = 0; q < 3; q++) {
+        r.offA[q] = -(int)T.g0[qa];
+        r.offB[q] = (int)T.len[qa] - (int)T.g0[qb];
     }
     r.rowB = (key - P.cell_base) / P.sx != T.jA;
     return r;
@@ -313,6 +362,24 @@ __device__ __forceinline__ uint2 walk_group(const TilePlan &T, Rec *tile, Load l
         if (live)
             fh = sph_walk_masked<OWN, CAP>(self, R, lds_src(tile, T, key, P), P.h2, P.bucket_cap,
                                            test, body, mk);
+    } else if (T.rows_ok) {
+        // one candidate row at a time: same chunks, same order
+        unsigned ci = 0;
+        auto pass = [&](auto row) {
+            constexpr int dy = decltype(row)::value;
+            stage_row<CAPACITY>(T, dy, tile, load);
+            if (live) {
+                const uint2 r =
+                    sph_walk_rows<OWN, CAP, dy, dy>(self, R, lds_row_src(tile, T, dy, key, P),
+                                                    P.h2, P.bucket_cap, test, body, mk, ci);
+                fh.x += r.x;
+                fh.y += r.y;
+            }
+            __syncthreads(); // the next row overwrites the tile
+        };
+        pass(std::integral_constant<int, -1>{});
+        pass(std::integral_constant<int, 0>{});
+        pass(std::integral_constant<int, 1>{});
     } else if (live) {
         fh = sph_walk_masked<OWN, CAP>(self, R, global_src<Rec>(load), P.h2, P.bucket_cap, test,
                                        body, mk);
@@ -750,7 +817,7 @@ k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
 // v = (x - x_prev)/dt clamped, and the histogram of the next sort; without
 // FUSED only the relaxed positions are written.
 template <bool FUSED, bool CAP>
-__global__ void __launch_bounds__(kTPB, kGatherMinBlocks)
+__global__ void __launch_bounds__(kTPB, CAP ? kGatherMinBlocks : kGatherMinBlocks + 1)
 k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
         const float2 *__restrict__ POS, const float2 *__restrict__ PREV,
         const float2 *__restrict__ RHO, const unsigned *__restrict__ K,
