@@ -501,13 +501,13 @@ def l2_hit_of(kernel_name):
     return None
 
 
-def kernel_window(s, upload):
-    """Per-kernel mean durations (ms) over eager steps KERNEL_WINDOW + 1 ..
-    KERNEL_WINDOW + KERNEL_STEPS after a fresh upload, each kernel timed by
+def kernel_window(s, upload, start=KERNEL_WINDOW):
+    """Per-kernel mean durations (ms) over eager steps start + 1 ..
+    start + KERNEL_STEPS after a fresh upload, each kernel timed by
     HIP events that its own launch records on the solver's stream."""
     upload()
     s.reset_counters()
-    s.step(KERNEL_WINDOW)
+    s.step(start)
     s.sync()
     return s.time_kernels(KERNEL_STEPS), s.counters()
 
@@ -669,6 +669,9 @@ def main():
                          "its table")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the late-regime, bucket-cap, N=4096 and drop-in lines")
+    ap.add_argument("--kernel-window", type=int, default=KERNEL_WINDOW,
+                    help="with --kernels-only: the steps before the timed window (default: "
+                         "the headline's window; 550 profiles the late regime)")
     ap.add_argument("--bucket-cap", type=int, default=0,
                     help="with --kernels-only: run under this bucket cap (the reference's: 100)")
     args = ap.parse_args()
@@ -689,10 +692,11 @@ def main():
         s = sa.Solver(tun, box, len(parts))
         if args.bucket_cap:
             s.set_bucket_cap(args.bucket_cap)
-        kt, cnt = kernel_window(s, lambda: s.upload(parts))
+        kt, cnt = kernel_window(s, lambda: s.upload(parts), args.kernel_window)
         if args.calib_copy_mb > 0:
             calibration_copy(args.calib_copy_mb)
-        print(json.dumps({"window": f"eager steps {KERNEL_WINDOW + 1}-{KERNEL_WINDOW + KERNEL_STEPS}"
+        print(json.dumps({"window": f"eager steps {args.kernel_window + 1}-"
+                                    f"{args.kernel_window + KERNEL_STEPS}"
                                     " after a fresh upload", "kernels_ms": kt, "counters": cnt}))
         return
     if args.gpus != 1:
