@@ -233,14 +233,59 @@ __device__ void plan_tile_par(unsigned s0, unsigned s1, const unsigned *__restri
     __syncthreads();
 }
 
+// LDS storage of a tile of CAPACITY records (plus the walk's padding): as
+// the records are, or - float4 records, SPH_SPLIT_TILES - as two planes of
+// float2, (x, y) and (z, w), in the same bytes, so that the candidate tests
+// read 8-byte positions at an 8-byte stride.
+#ifndef SPH_SPLIT_TILES
+#define SPH_SPLIT_TILES 1
+#endif
+template <class R>
+struct TileMem {
+    using Rec = R;
+    R *p;
+    __device__ void put(unsigned i, R v) const { p[i] = v; }
+    __device__ R get(unsigned i) const { return p[i]; }
+};
+struct TileMemSplit {
+    using Rec = float4;
+    float2 *xy, *zw;
+    __device__ void put(unsigned i, float4 v) const
+    {
+        xy[i] = make_float2(v.x, v.y);
+        zw[i] = make_float2(v.z, v.w);
+    }
+    __device__ float4 get(unsigned i) const
+    {
+        const float2 a = xy[i], b = zw[i];
+        return make_float4(a.x, a.y, b.x, b.y);
+    }
+};
+template <int CAPACITY, class R>
+__device__ __forceinline__ TileMem<R> tile_mem(R *tile)
+{
+    return TileMem<R>{tile};
+}
+template <int CAPACITY>
+__device__ __forceinline__ auto tile_mem(float4 *tile)
+{
+    if constexpr (SPH_SPLIT_TILES) {
+        float2 *xy = reinterpret_cast<float2 *>(tile);
+        return TileMemSplit{xy, xy + (CAPACITY + kSphWalkPad)};
+    } else {
+        return TileMem<float4>{tile};
+    }
+}
+
 // Copy the planned segments into LDS (all threads), then barrier.  The
 // segments lie back to back in the tile, so thread t stages tile records
 // t, t + 256, ...: it finds each record's segment (the last one beginning at
 // or before it), issues all of its loads, and only then stores them - one
 // memory round trip per workgroup, not one per segment.
-template <int CAPACITY, class Rec, class Load>
-__device__ __forceinline__ void stage_tile(const TilePlan &T, Rec *tile, Load load)
+template <int CAPACITY, class Mem, class Load>
+__device__ __forceinline__ void stage_tile(const TilePlan &T, const Mem &tile, Load load)
 {
+    using Rec = typename Mem::Rec;
     constexpr int kPer = (CAPACITY + kTPB - 1) / kTPB;
     const unsigned total = T.total;
     Rec v[kPer];
@@ -259,7 +304,7 @@ __device__ __forceinline__ void stage_tile(const TilePlan &T, Rec *tile, Load lo
     for (int u = 0; u < kPer; u++) {
         const unsigned i = threadIdx.x + (unsigned)u * kTPB;
         if (i < total)
-            tile[i] = v[u];
+            tile.put(i, v[u]);
     }
     __syncthreads();
 }
@@ -267,9 +312,10 @@ __device__ __forceinline__ void stage_tile(const TilePlan &T, Rec *tile, Load lo
 // Stage only candidate row dy (segments dy + 1 of the first cell row and
 // dy + 4 of the second, back to back) into the tile, then barrier: for groups
 // whose rows fit one at a time but not all together.
-template <int CAPACITY, class Rec, class Load>
-__device__ __forceinline__ void stage_row(const TilePlan &T, int dy, Rec *tile, Load load)
+template <int CAPACITY, class Mem, class Load>
+__device__ __forceinline__ void stage_row(const TilePlan &T, int dy, const Mem &tile, Load load)
 {
+    using Rec = typename Mem::Rec;
     constexpr int kPer = (CAPACITY + kTPB - 1) / kTPB;
     const int qa = dy + 1, qb = dy + 4;
     const unsigned la = T.len[qa], total = la + T.len[qb];
@@ -285,27 +331,27 @@ __device__ __forceinline__ void stage_row(const TilePlan &T, int dy, Rec *tile, 
     for (int u = 0; u < kPer; u++) {
         const unsigned i = threadIdx.x + (unsigned)u * kTPB;
         if (i < total)
-            tile[i] = v[u];
+            tile.put(i, v[u]);
     }
     __syncthreads();
 }
 
 // Candidate source reading the LDS tile: the tile index of sorted position
 // k of row dy is k + row(dy).
-template <class Rec>
+template <class Mem>
 struct LdsSrc {
-    const Rec *tile;
+    Mem tile;
     int offA[3], offB[3];
     bool rowB;
     __device__ int row(int dy) const { return rowB ? offB[dy + 1] : offA[dy + 1]; }
-    __device__ Rec at(unsigned idx) const { return tile[idx]; }
+    __device__ typename Mem::Rec at(unsigned idx) const { return tile.get(idx); }
 };
 
-template <class Rec>
-__device__ __forceinline__ LdsSrc<Rec> lds_src(const Rec *tile, const TilePlan &T, unsigned key,
+template <class Mem>
+__device__ __forceinline__ LdsSrc<Mem> lds_src(const Mem &tile, const TilePlan &T, unsigned key,
                                               const sph_dev_params &P)
 {
-    LdsSrc<Rec> r;
+    LdsSrc<Mem> r;
     r.tile = tile;
     for (int q = 0; q < 3; q++) {
         r.offA[q] = T.off[q];
@@ -316,11 +362,11 @@ __device__ __forceinline__ LdsSrc<Rec> lds_src(const Rec *tile, const TilePlan &
 }
 
 // The source of a tile staged by stage_row(dy): only row dy's offsets matter.
-template <class Rec>
-__device__ __forceinline__ LdsSrc<Rec> lds_row_src(const Rec *tile, const TilePlan &T, int dy,
+template <class Mem>
+__device__ __forceinline__ LdsSrc<Mem> lds_row_src(const Mem &tile, const TilePlan &T, int dy,
                                                   unsigned key, const sph_dev_params &P)
 {
-    LdsSrc<Rec> r;
+    LdsSrc<Mem> r;
     r.tile = tile;
     const int qa = dy + 1, qb = dy + 4;
     for (int q = 0; q < 3; q++) {
@@ -357,20 +403,21 @@ __device__ __forceinline__ uint2 walk_group(const TilePlan &T, Rec *tile, Load l
                                             Masks &mk)
 {
     uint2 fh = make_uint2(0u, 0u);
+    const auto mem = tile_mem<CAPACITY>(tile);
     if (T.ok) {
-        stage_tile<CAPACITY>(T, tile, load);
+        stage_tile<CAPACITY>(T, mem, load);
         if (live)
-            fh = sph_walk_masked<OWN, CAP>(self, R, lds_src(tile, T, key, P), P.h2, P.bucket_cap,
+            fh = sph_walk_masked<OWN, CAP>(self, R, lds_src(mem, T, key, P), P.h2, P.bucket_cap,
                                            test, body, mk);
     } else if (T.rows_ok) {
         // one candidate row at a time: same chunks, same order
         unsigned ci = 0;
         auto pass = [&](auto row) {
             constexpr int dy = decltype(row)::value;
-            stage_row<CAPACITY>(T, dy, tile, load);
+            stage_row<CAPACITY>(T, dy, mem, load);
             if (live) {
                 const uint2 r =
-                    sph_walk_rows<OWN, CAP, dy, dy>(self, R, lds_row_src(tile, T, dy, key, P),
+                    sph_walk_rows<OWN, CAP, dy, dy>(self, R, lds_row_src(mem, T, dy, key, P),
                                                     P.h2, P.bucket_cap, test, body, mk, ci);
                 fh.x += r.x;
                 fh.y += r.y;
