@@ -152,3 +152,50 @@ def test_reference_entry_points_apply_the_grid_cap(shim):
     run_reference_loop(g.phase, lambda _: None, 12, [])
     assert g.counters()["bucket_overflow"] > 0
     assert_bitexact(sh.state(), g.get_state(), what="entry points with the grid's cap, 12 steps")
+
+
+def test_cap_in_a_batch_of_scenes():
+    """Every scene of a batched context takes the cap, each as it would alone;
+    scene offsets in the sorted arrays do not shift any scene's ranks."""
+    from sph_amd import scenes as sc
+    rng = np.random.default_rng(8)
+    scenes = []
+    for i in range(4):
+        nx, ny = int(rng.integers(12, 30)), int(rng.integers(12, 30))
+        box = sa.make_box(float(nx) * 0.6 + 4.0, float(ny) * 0.5 + 5.0)
+        parts = sa.scene_lattice(nx, ny, 0.25, 0.01, seed=300 + i, x0=0.25, y0=0.25)
+        scenes.append((sc.preset_tunables(i % 4), box, parts, i % 4))
+    s = sa.Solver.batch([(t, b, len(p)) for t, b, p, _ in scenes])
+    parts, counts = sc.concat(scenes)
+    s.upload_batch(parts, counts)
+    s.set_bucket_cap(5)
+    s.step(8)
+    out = s.download()
+    offs = np.concatenate([[0], np.cumsum(counts)])
+    for i, (t, b, p, _) in enumerate(scenes):
+        g = po.GatherSim(b, t, len(p))
+        g.set_cap(5)
+        g.set_state(p.view(po.P32))
+        g.step(8)
+        assert_bitexact(out[offs[i]:offs[i + 1]], g.get_state(), what=f"scene {i}, cap 5")
+
+
+def test_cap_with_split_tunables():
+    """sph_step_tunables2 (new tunables between prediction and the hash) under
+    the cap: the same as the oracle's phases with the same switch."""
+    from tests.common import FIRST_HALF, SECOND_HALF
+    parts, tun, box = c1_case()
+    s, g = make_pair(parts, tun, box, 3)
+    t2 = type(tun).from_buffer_copy(bytes(tun))
+    t2.k = tun.k * 2.0
+    t2.g = tun.g * 0.5
+    for step in range(6):
+        s.step_tunables2(sa.Tunables.from_buffer_copy(bytes(tun)),
+                         sa.Tunables.from_buffer_copy(bytes(t2)))
+        for ph in FIRST_HALF:
+            g.phase(ph)
+        g.set_tunables(t2)
+        for ph in SECOND_HALF:
+            g.phase(ph)
+        g.set_tunables(tun)
+    assert_bitexact(s.download(), g.get_state(), what="split tunables, cap 3, 6 steps")
