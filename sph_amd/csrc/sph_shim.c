@@ -60,7 +60,8 @@ static fluid_particle **g_ptrs;
 static int g_n;
 static int g_capacity; /* max_fluid_particles_local given to initParticles */
 static AABB_t g_box;   /* the box the context was created with */
-static int g_cap_set = -1; /* bucket cap last given to the context (-1: none yet) */
+static int g_cap_set = -1;  /* bucket cap last given to the context (-1: none yet) */
+static int g_grid_cap = 0;  /* the cap of the last grid record hash_fluid received */
 /* Do the caller's pointers address one array in order (ptrs[i] == ptrs[0] + i,
  * as the reference's initParticles lays them out)?  Then the state is copied
  * back into it in one piece. */
@@ -103,7 +104,14 @@ static int shim_upload(fluid_particle **ptrs, param *params, AABB_t *box)
         rc = sph_create(&g_ctx, params, &b, cap);
         if (rc)
             return rc;
+        /* a context made again (growth) keeps the run's cap from its first pass */
         g_cap_set = -1;
+        if (g_grid_cap) {
+            rc = sph_set_bucket_cap(g_ctx, g_grid_cap);
+            if (rc)
+                return rc;
+            g_cap_set = g_grid_cap;
+        }
         g_capacity = cap;
         g_box = b;
         if (g_dirty == DIRTY_NONE)
@@ -291,6 +299,8 @@ void sph_shim_release(void)
     g_sync_interval = 1;
     g_sync_fields = SPH_SYNC_ALL;
     g_reported = 0;
+    g_grid_cap = 0;
+    g_cap_set = -1;
     free(g_stage);
     g_stage = NULL;
     g_stage_bytes = 0;
@@ -495,6 +505,7 @@ static int shim_apply_cap(const neighbor_grid_t *grid)
 {
     const unsigned v = grid ? grid->max_bucket_size : 0u;
     const int cap = v > 0x7fffffffu ? 0x7fffffff : (int)v;
+    g_grid_cap = cap;
     if (!g_ctx || cap == g_cap_set)
         return SPH_OK;
     int rc = sph_set_bucket_cap(g_ctx, cap);

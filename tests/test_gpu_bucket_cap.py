@@ -199,3 +199,47 @@ def test_cap_with_split_tunables():
             g.phase(ph)
         g.set_tunables(tun)
     assert_bitexact(s.download(), g.get_state(), what="split tunables, cap 3, 6 steps")
+
+
+def test_entry_points_keep_the_cap_when_the_context_grows(shim):
+    """Particles added beyond the context's room make the entry points create
+    it again; the grid record's cap holds from the new context's first pass
+    on - also for a viscosity pass run at once (a host read in the middle of
+    the step), before the step's hash_fluid has seen the grid again."""
+    import ctypes
+    from tests.common import FIRST_HALF, SECOND_HALF, run_reference_loop
+    from tests.test_gpu_reference_scenes import Shim
+    t = dam_break_tunables()
+    box = sa.make_box(8, 8)
+    sh = Shim(sa.Tunables.from_buffer_copy(bytes(t)), box, 1600)
+    ctypes.c_uint.from_buffer(sh.grid, 36).value = sa.REF_MAX_BUCKET
+    sh.init(sa.make_box(3.6, 3.65), 40, 0.09)
+    sa.lib().sph_shim_set_sync_interval(1)
+    run_reference_loop(sh.phase, lambda _: None, 4, [])
+    n0, extra = sh.n, 300
+    sh.alloc(n0 + extra, keep=n0)
+    rng = np.random.default_rng(5)
+    new = sh.parts[n0:]
+    new["x"] = rng.uniform(0.5, 3.0, extra).astype(np.float32)
+    new["y"] = rng.uniform(0.5, 3.0, extra).astype(np.float32)
+    new["x_prev"], new["y_prev"] = new["x"], new["y"]
+    new["id"] = np.arange(n0, n0 + extra)
+    sh.params.number_fluid_particles_local = n0 + extra
+    sa.lib().sph_shim_mark_host_dirty()
+    host = sh.parts.copy()
+    g = po.GatherSim(box, t, n0 + extra)
+    g.set_cap(sa.REF_MAX_BUCKET)
+    g.set_state(host.view(po.P32))
+    g.phase(po.PH_HASH)  # the lists of the changed state
+    for step in range(3):
+        for ph in FIRST_HALF:
+            sh.phase(ph)
+            g.phase(ph)
+        mid = sh.state()  # runs the recorded calls now, phase by phase
+        assert_bitexact(mid, g.get_state(), fields=("x", "y", "v_x", "v_y"),
+                        what=f"grown context, step {step}, after predict")
+        for ph in SECOND_HALF:
+            sh.phase(ph)
+            g.phase(ph)
+    assert g.counters()["bucket_overflow"] > 0
+    assert_bitexact(sh.state(), g.get_state(), what="grown context, 3 steps")
