@@ -1,22 +1,23 @@
-"""Dense tiles: the gather kernels' two candidate paths on one scene.
+"""Dense tiles: the gather kernels' candidate paths on one scene.
 
 A workgroup stages the candidate rows of its 256 sorted particles in LDS
 when they fit the library's tile (1,024-2,048 records by build; the density
 pass, whose records are positions only, twice as many in the same bytes); a
-workgroup whose tile would not fit, or that spans more than two cell rows,
-walks the same candidates in global memory in the same order.  A dense lattice
-(spacing 0.15h, about 44 particles per cell) makes tiles of 1,050-1,220
-records, and one workgroup spans several rows; it is compared bit for bit
-with the oracle.  The same scene then runs on the test build with
-256-record tiles (nearly every workgroup on the global-memory walk, XCD runs
-of one tile) in a child process, and must equal the default library bit for
-bit.  An overfull lattice (spacing 0.06h, about 280 particles per cell) takes
-the viscosity and relaxation workgroups off LDS (the density pass still
-stages most of them) and the reorder's ranking past its staged ids, and
-exceeds both of the reference's caps; it too is compared bit for bit, with the
-overflow counters.  A lattice at 0.1h (about 100 particles per cell) puts most
-tiles between the two capacities: density from LDS, the other gathers from
-global memory, bit for bit.
+workgroup whose rows fit only one at a time stages and walks them row by row;
+one that spans more than two cell rows, or whose rows do not fit even singly,
+walks the same candidates in global memory.  All three visit the same chunks
+in the same order.  A dense lattice (spacing 0.15h, about 44 particles per
+cell) makes tiles of 1,050-1,220 records, and one workgroup spans several
+rows; it is compared bit for bit with the oracle.  The same scene then runs on
+the test build with 256-record tiles (nearly every workgroup row by row or on
+the global-memory walk, XCD runs of one tile) in a child process, and must
+equal the default library bit for bit.  An overfull lattice (spacing 0.06h,
+about 280 particles per cell) stages the viscosity and relaxation rows one at
+a time (the density pass most tiles whole), takes the reorder's ranking past
+its staged ids, and exceeds both of the reference's caps; it too is compared
+bit for bit, with the overflow counters.  A lattice at 0.1h (about 100
+particles per cell) puts most tiles between the two capacities: density
+staged whole, the other gathers row by row, bit for bit.
 """
 import os
 import subprocess
@@ -73,7 +74,8 @@ def test_overfull_cells_bitexact_against_oracle():
     100-slot buckets and 400-entry neighbour lists, both counted).  Cells this
     large reach past the ids the reorder stages around each workgroup (its
     positions +-64), so the ranking's global-memory path runs, and every tile
-    is too large for the viscosity and relaxation passes' LDS tiles."""
+    is too large for the viscosity and relaxation passes' LDS tiles as a whole
+    (they stage its candidate rows one at a time)."""
     parts = sa.scene_lattice(48, 48, 0.06, 0.01, seed=5)
     tun, box = dam_break_tunables(), po.make_box(60, 40)
     s = sa.Solver(sa.Tunables.from_buffer_copy(bytes(tun)), sa.AABB.from_buffer_copy(bytes(box)),
@@ -106,7 +108,7 @@ def test_overfull_cells_bitexact_against_oracle():
 
 def test_tiles_between_the_density_and_record_capacities():
     """Spacing 0.1h: 16,384 particles, about 100 per cell, tiles of 1,200-1,800
-    candidates - staged by the density pass, read from global memory by the
+    candidates - staged whole by the density pass, row by row by the
     viscosity and relaxation passes - bit-exact against the oracle."""
     parts = sa.scene_lattice(128, 128, 0.1, 0.01, seed=7)
     tun, box = dam_break_tunables(), po.make_box(60, 40)
