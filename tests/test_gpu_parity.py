@@ -285,3 +285,30 @@ def test_reference_loop_through_shims_matches_step(tmp_path):
     s.set_bucket_cap(sa.REF_MAX_BUCKET)
     s.step(40)
     assert_bitexact(got, s.download(), what="shim loop vs sph_step")
+
+
+@pytest.mark.parametrize("flags", [["-s", "4", "-p"], ["-s", "0", "-u"]])
+def test_reference_loop_through_shims_at_1m(tmp_path, flags):
+    """The drop-in at the benchmark's size: the C driver's reference loop on
+    the 1024 x 1024 lattice in the 1200 x 800 box, 24 steps, copying positions
+    back every 4 steps (the reference's packing interval) or never, with the
+    reference's 100-slot buckets or uncapped (-u); its final state equals
+    sph_step's bit for bit."""
+    import os
+    import subprocess
+    exe = os.path.join(sa.PKG_DIR, "sph_main")
+    out = tmp_path / "out.bin"
+    cmd = [exe, *flags, "-b", "1200,800", "1024", "1024", "24", str(out)]
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0, r.stderr
+    got = np.fromfile(out, sa.PARTICLE)
+    tun = dam_break_tunables()
+    box = po.make_box(1200, 800)
+    parts = sa.scene_lattice(1024, 1024, 0.5, 0.0, 0)
+    s = sa.Solver(sa.Tunables.from_buffer_copy(bytes(tun)), sa.AABB.from_buffer_copy(bytes(box)),
+                  len(parts))
+    s.upload(parts)
+    s.set_bucket_cap(0 if "-u" in flags else sa.REF_MAX_BUCKET)
+    s.step(24)
+    assert_bitexact(got, s.download(), what="1M shim loop vs sph_step")
