@@ -256,6 +256,16 @@ int sph_set_scene_frozen(sph_ctx *ctx, const int *flags);
  * reference's cap never changes).  The 400-neighbour list cap is not
  * reproduced. */
 int sph_set_bucket_cap(sph_ctx *ctx, int cap);
+
+/* Launch shape, with no counterpart in the reference: lanes per particle of
+ * the fused step's gathers.  0 (the default) chooses by size - 4 when the
+ * context's 256-slot tiles number at most a quarter of the device's CUs
+ * (a scene of up to 16,384 particles on 256 CUs), where a launch's time is
+ * one workgroup's latency, else 1; 1 and 4 force a shape.  Results are bit
+ * for bit the same either way: four lanes split a particle's candidate tests
+ * and pair terms and apply the terms in the same order.  Any other value is
+ * SPH_EINVAL. */
+int sph_set_gather_lanes(sph_ctx *ctx, int lanes);
 /* Sloshing clock offset of one scene: its forcing is evaluated at
  * t = (steps since the last upload or load + steps) * dt, so a scene loaded
  * mid-trajectory keeps its phase. */

@@ -126,6 +126,7 @@ def lib():
             "sph_set_scene_forcing": (ip, [vp, ip, ctypes.c_float, ctypes.c_float]),
             "sph_set_scene_frozen": (ip, [vp, vp]),
             "sph_set_bucket_cap": (ip, [vp, ip]),
+            "sph_set_gather_lanes": (ip, [vp, ip]),
             "sph_upload": (ip, [vp, vp, ip]),
             "sph_set_tunables": (ip, [vp, ctypes.POINTER(Tunables)]),
             "sph_step": (ip, [vp, ip]),
@@ -308,6 +309,11 @@ class Solver:
         """The reference's per-cell bucket cap (src/hash.c:153-164); 0 turns
         it off (the default)."""
         _check(lib().sph_set_bucket_cap(self._h, int(cap)), "sph_set_bucket_cap")
+
+    def set_gather_lanes(self, lanes: int = 0):
+        """Lanes per particle of the fused step's gathers: 0 chooses by size
+        (4 for small contexts), 1 or 4 force a shape; results are the same."""
+        _check(lib().sph_set_gather_lanes(self._h, int(lanes)), "sph_set_gather_lanes")
 
     def set_scene_tunables(self, scene: int, t: Tunables):
         _check(lib().sph_set_scene_tunables(self._h, scene, ctypes.byref(t)), "sph_set_scene_tunables")

@@ -182,7 +182,7 @@ void enqueue_fused_step(sph_ctx *c, bool visc)
         Tick t(c, KV);
         sph_launch_visc_predict(c->st, S, true, visc, true, cap, c->dPA, c->BS, c->POS0, c->VEL,
                                 c->K0, c->start, c->UPOS, c->ukey, c->uslot, c->cnt, c->dC,
-                                c->dclock, c->UVEL);
+                                c->dclock, c->UVEL, c->lanes);
     }
     enqueue_sort_tail(c, c->ID0, c->K1);
     {
@@ -194,13 +194,13 @@ void enqueue_fused_step(sph_ctx *c, bool visc)
     {
         Tick t(c, KDE);
         sph_launch_density(c->st, S, cap, c->dP, c->BS, c->POS1, c->K1, c->start, c->RHO,
-                           c->MASK, c->dC);
+                           c->MASK, c->dC, c->lanes);
     }
     {
         Tick t(c, KRE);
         sph_launch_relax(c->st, S, true, cap, c->dP, c->BS, c->POS1, c->PREV, c->RHO, c->K1,
                          c->start, c->UPOS, c->UVEL, c->ukey, c->uslot, c->cnt, c->MASK, c->dC,
-                         c->dclock);
+                         c->dclock, c->lanes);
     }
     enqueue_sort_tail(c, c->ID1, c->K0);
     {
@@ -345,6 +345,7 @@ int sph_create_batch(sph_ctx **out, const sph_scene_desc *scenes, int nscenes)
     }
     c->cells = (unsigned)cells;
     c->slots = (unsigned)slots;
+    c->lanes = sph_gather_lanes(c->slots);
     int rc = SPH_OK;
     const size_t N = c->slots, M = (size_t)c->cells + 1;
     const size_t MP = sph_scan_padded((unsigned)M); // the scan works on whole tiles
@@ -640,6 +641,20 @@ int sph_set_bucket_cap(sph_ctx *c, int cap)
     return push_params(c);
 }
 
+int sph_set_gather_lanes(sph_ctx *c, int lanes)
+{
+    if (int rc = check_ctx(c))
+        return rc;
+    if (lanes != 0 && lanes != 1 && lanes != 4)
+        return set_err(SPH_EINVAL, "sph_set_gather_lanes: lanes %d is not 0, 1 or 4", lanes);
+    const int want = lanes ? lanes : sph_gather_lanes(c->slots);
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (want != c->lanes)
+        free_graphs(c); // the captured steps launch the other kernel variants
+    c->lanes = want;
+    return SPH_OK;
+}
+
 int sph_step_eager(sph_ctx *c, int nsteps)
 {
     if (int rc = check_ctx(c))
@@ -782,7 +797,7 @@ int sph_phase_viscosity(sph_ctx *c)
         return SPH_OK; // empty lists (src/fluid.c:202)
     sph_launch_visc_predict(c->st, c->slots, false, true, false, c->bucket_cap != 0, c->dP, c->BS,
                             c->POS0, c->VEL, c->K0, c->start, c->UVEL, nullptr, nullptr, c->cnt,
-                            c->dC, c->dclock, nullptr);
+                            c->dC, c->dclock, nullptr, c->lanes);
     HIPCHK(hipMemcpyAsync(c->VEL, c->UVEL, sizeof(float2) * (size_t)c->slots,
                           hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipGetLastError());
@@ -824,7 +839,7 @@ int sph_phase_hash(sph_ctx *c, int compute_density)
                           c->st));
     if (compute_density)
         sph_launch_density(c->st, S, c->bucket_cap != 0, c->dP, c->BS, c->POS0, c->K0, c->start,
-                           c->RHO, c->MASK, c->dC);
+                           c->RHO, c->MASK, c->dC, c->lanes);
     HIPCHK(hipGetLastError());
     c->lists_valid = true;
     c->hashes++;
@@ -843,7 +858,7 @@ int sph_phase_relax(sph_ctx *c)
         return set_err(SPH_ESTATE, "sph_phase_relax: no hash since upload");
     sph_launch_relax(c->st, c->slots, false, c->bucket_cap != 0, c->dP, c->BS, c->POS0, c->PREV,
                      c->RHO, c->K0, c->start, c->UPOS, nullptr, nullptr, nullptr, nullptr,
-                     nullptr, c->dC, c->dclock);
+                     nullptr, c->dC, c->dclock, c->lanes);
     HIPCHK(hipMemcpyAsync(c->POS0, c->UPOS, sizeof(float2) * (size_t)c->slots,
                           hipMemcpyDeviceToDevice, c->st));
     HIPCHK(hipGetLastError());

@@ -39,6 +39,7 @@ struct sph_ctx {
     sph_dev_params *dPA = nullptr;
     bool dPA_stale = false;
     unsigned bucket_cap = 0; // the reference's bucket cap (0: none)
+    int lanes = 1;           // lanes per particle of the gathers (sph_gather_lanes)
     float2 *XY = nullptr; // positions by id for sph_download_positions (allocated on first use)
     int *BS = nullptr; // workgroup -> scene
     sph_dev_counters *dC = nullptr;

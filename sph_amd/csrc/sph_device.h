@@ -371,6 +371,43 @@ struct sph_masks_replay {
     __device__ void put(unsigned, unsigned) {}
 };
 
+// Four lanes per particle (the walk's LP = 4): lanes 4p .. 4p + 3 of a wave
+// hold the same particle, and quad-wide DPP moves share values among them.
+// Each of the four takes one pair of a drain step and the pair terms are
+// then applied in canonical order by all four, so every sum is formed in
+// exactly the order of one lane per particle.
+struct sph_pair_term {
+    float a, b; // the term's two components (zero: no update)
+    int f;      // an extra flag of the term (the relaxation's 1e-6 nudge)
+};
+
+// A pair body split in two for LP = 4: term(idx, own, rec) evaluates one
+// pair's contribution, apply(t) adds it to the particle's sums; apply of a
+// zero term with f = 0 must leave the sums unchanged bit for bit.
+template <class Term, class Apply>
+struct sph_split_body {
+    Term term;
+    Apply apply;
+};
+
+__device__ __forceinline__ unsigned sph_quad_lane() { return __lane_id() & 3u; }
+
+template <int CTRL>
+__device__ __forceinline__ unsigned sph_dpp(unsigned x)
+{
+    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xF, 0xF, false);
+}
+
+// Lane j's term of this lane's quad (quad_perm j, j, j, j).
+template <int J>
+__device__ __forceinline__ sph_pair_term sph_quad_term(const sph_pair_term &t)
+{
+    constexpr int kCtrl = J * 0x55;
+    return sph_pair_term{__uint_as_float(sph_dpp<kCtrl>(__float_as_uint(t.a))),
+                         __uint_as_float(sph_dpp<kCtrl>(__float_as_uint(t.b))),
+                         (int)sph_dpp<kCtrl>((unsigned)t.f)};
+}
+
 // The same walk in two passes, for SIMD efficiency.  Pass 1 tests the
 // candidates 32 at a time (dist2(rec): squared distance, within h when not
 // above h2), branch-free, into a mask
@@ -402,8 +439,15 @@ struct sph_masks_replay {
 // DY0..DY1 limit the walk to those candidate rows, and ci counts the lane's
 // chunks across calls: walking the rows one call at a time (a tile staged row
 // by row) visits the same chunks in the same order as one call over all three.
-template <bool OWN = false, bool CAP = false, int DY0 = -1, int DY1 = 1, class Src, class Dist2,
-          class Body, class Masks>
+//
+// LP = 4: four lanes per particle (sph_split_body).  Lane q of the quad tests
+// the chunk's candidates 8q .. 8q + 7 and the quad ORs the four bytes into
+// the chunk mask, so all four hold the same masks and queue; a drain step
+// gives lane q the q-th pending pair of the head chunk, and the four terms
+// are applied by every lane in pair order.  Same chunks, same masks, same
+// sums as LP = 1, with a quarter of the serial work per lane.
+template <bool OWN = false, bool CAP = false, int DY0 = -1, int DY1 = 1, int LP = 1, class Src,
+          class Dist2, class Body, class Masks>
 __device__ __forceinline__ uint2 sph_walk_rows(unsigned self, const sph_rows &R, const Src &src,
                                                float h2, unsigned cap, Dist2 &&dist2, Body &&body,
                                                Masks &mk, unsigned &ci)
@@ -425,10 +469,33 @@ __device__ __forceinline__ uint2 sph_walk_rows(unsigned self, const sph_rows &R,
         unsigned q0 = m0, q1 = m1, q2 = m2, y0 = x0, y1 = x1, y2 = x2, u0 = t0, u1 = t1,
                  u2 = t2;
         while (q0) {
-            const unsigned t = (unsigned)__builtin_ctz(q0);
-            q0 &= q0 - 1u;
-            const unsigned idx = y0 + t;
-            body(idx, OWN && idx >= u0, src.at(idx));
+            if constexpr (LP == 1) {
+                const unsigned t = (unsigned)__builtin_ctz(q0);
+                q0 &= q0 - 1u;
+                const unsigned idx = y0 + t;
+                body(idx, OWN && idx >= u0, src.at(idx));
+            } else {
+                // lane q: the q-th lowest pending pair of the head chunk
+                const unsigned q = sph_quad_lane();
+                unsigned r = q0;
+                r = q >= 1u ? r & (r - 1u) : r;
+                r = q >= 2u ? r & (r - 1u) : r;
+                r = q >= 3u ? r & (r - 1u) : r;
+                const bool have = r != 0u;
+                const unsigned idx = y0 + (have ? (unsigned)__builtin_ctz(r) : 0u);
+                sph_pair_term tm = body.term(idx, OWN && idx >= u0, src.at(idx));
+                tm.a = have ? tm.a : 0.0f;
+                tm.b = have ? tm.b : 0.0f;
+                tm.f = have ? tm.f : 0;
+                body.apply(sph_quad_term<0>(tm));
+                body.apply(sph_quad_term<1>(tm));
+                body.apply(sph_quad_term<2>(tm));
+                body.apply(sph_quad_term<3>(tm));
+                q0 &= q0 - 1u;
+                q0 &= q0 - 1u;
+                q0 &= q0 - 1u;
+                q0 &= q0 - 1u;
+            }
             const bool next = q0 == 0; // chunk done: the queue moves up
             q0 = next ? q1 : q0;
             y0 = next ? y1 : y0;
@@ -472,15 +539,31 @@ __device__ __forceinline__ uint2 sph_walk_rows(unsigned self, const sph_rows &R,
                     // are padded by kSphWalkPad); the first candidate ends in
                     // the top pushed bit, so a bit reversal and a shift give
                     // candidate t bit t, and the padding's bits are cut off
-                    const unsigned groups = (m + 7u) >> 3;
-                    unsigned acc = 0;
-                    for (unsigned g = 0; g < groups; g++) {
-                        const unsigned base = c + 8u * g;
+                    if constexpr (LP == 1) {
+                        const unsigned groups = (m + 7u) >> 3;
+                        unsigned acc = 0;
+                        for (unsigned g = 0; g < groups; g++) {
+                            const unsigned base = c + 8u * g;
 #pragma unroll
-                        for (unsigned k = 0; k < 8u; k++)
-                            acc = sph_mask_push(acc, dist2(src.at(base + k)), h2);
+                            for (unsigned k = 0; k < 8u; k++)
+                                acc = sph_mask_push(acc, dist2(src.at(base + k)), h2);
+                        }
+                        mask = __builtin_bitreverse32(acc) >> (32u - 8u * groups);
+                    } else {
+                        // lane q: group q; the quad ORs the four bytes
+                        const unsigned q = sph_quad_lane();
+                        unsigned acc = 0;
+                        if (8u * q < m) {
+                            const unsigned base = c + 8u * q;
+#pragma unroll
+                            for (unsigned k = 0; k < 8u; k++)
+                                acc = sph_mask_push(acc, dist2(src.at(base + k)), h2);
+                        }
+                        unsigned part = (__builtin_bitreverse32(acc) >> 24) << (8u * q);
+                        part |= sph_dpp<0xB1>(part); // quad_perm 1, 0, 3, 2
+                        part |= sph_dpp<0x4E>(part); // quad_perm 2, 3, 0, 1
+                        mask = part;
                     }
-                    mask = __builtin_bitreverse32(acc) >> (32u - 8u * groups);
                     mask &= m < 32u ? (1u << m) - 1u : ~0u;
                     if (me - c < m) // the particle itself is no neighbour
                         mask &= ~(1u << (me - c));
@@ -528,11 +611,12 @@ __device__ __forceinline__ uint2 sph_walk_rows(unsigned self, const sph_rows &R,
 }
 
 // All three candidate rows in one call.
-template <bool OWN = false, bool CAP = false, class Src, class Dist2, class Body, class Masks>
+template <bool OWN = false, bool CAP = false, int LP = 1, class Src, class Dist2, class Body,
+          class Masks>
 __device__ __forceinline__ uint2 sph_walk_masked(unsigned self, const sph_rows &R, const Src &src,
                                                  float h2, unsigned cap, Dist2 &&dist2,
                                                  Body &&body, Masks &mk)
 {
     unsigned ci = 0; // chunks of this lane so far (all rows, walk order)
-    return sph_walk_rows<OWN, CAP>(self, R, src, h2, cap, dist2, body, mk, ci);
+    return sph_walk_rows<OWN, CAP, -1, 1, LP>(self, R, src, h2, cap, dist2, body, mk, ci);
 }

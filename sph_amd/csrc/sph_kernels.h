@@ -14,13 +14,19 @@ struct sph_reorder_fields {
 // Workgroup size of every particle kernel; scenes are padded to multiples.
 constexpr unsigned kSphTPB = 256;
 
-// cap: the reference's bucket cap is on (sph_dev_params::bucket_cap != 0)
+// Lanes per particle (1 or 4) of the gathers of a context with `slots`
+// padded slots, on the current device.
+int sph_gather_lanes(unsigned slots);
+
+// cap: the reference's bucket cap is on (sph_dev_params::bucket_cap != 0).
+// lanes: lanes per particle of the gathers, 1 or 4 (sph_gather_lanes); 4
+// applies to the fused step's variants (grav, visc and pred; fused relax).
 void sph_launch_visc_predict(hipStream_t st, unsigned slots, bool grav, bool visc, bool pred,
                              bool cap, const sph_dev_params *P, const int *BS, const float2 *POS,
                              const float2 *VEL, const unsigned *K, const unsigned *start,
                              float2 *opos, unsigned *okey, unsigned *oslot, unsigned *cnt,
                              sph_dev_counters *C, const unsigned long long *clock,
-                             float2 *held_vel);
+                             float2 *held_vel, int lanes);
 void sph_launch_gravity(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
                         float2 *VEL, float2 *RHO);
 void sph_launch_predict(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
@@ -51,13 +57,14 @@ void sph_launch_reorder4(hipStream_t st, unsigned slots, const sph_dev_params *P
                          sph_dev_counters *C);
 void sph_launch_density(hipStream_t st, unsigned slots, bool cap, const sph_dev_params *P,
                         const int *BS, const float2 *POS, const unsigned *K,
-                        const unsigned *start, float2 *RHO, uint4 *masks, sph_dev_counters *C);
+                        const unsigned *start, float2 *RHO, uint4 *masks, sph_dev_counters *C,
+                        int lanes);
 void sph_launch_relax(hipStream_t st, unsigned slots, bool fused, bool cap,
                       const sph_dev_params *P, const int *BS, const float2 *POS,
                       const float2 *PREV, const float2 *RHO, const unsigned *K,
                       const unsigned *start, float2 *opos, float2 *ovel, unsigned *okey,
                       unsigned *oslot, unsigned *cnt, const uint4 *masks, sph_dev_counters *C,
-                      unsigned long long *clock);
+                      unsigned long long *clock, int lanes);
 void sph_launch_pack_coords(hipStream_t st, unsigned slots, const sph_dev_params *P,
                             const int *BS, const float2 *POS, const int *ID, short *out);
 void sph_launch_neighbor_lists(hipStream_t st, unsigned slots, const sph_dev_params *P,

@@ -54,6 +54,8 @@ constexpr int kTPB = 256;
 // density -2%, relaxation -4%, the capped variants -3 to -7%, viscosity
 // unchanged.  The uncapped relaxation asks for 7, which keeps it at 72 VGPRs
 // (7 waves) with its row-by-row staging path; 6 let it grow to 74 (6 waves).
+// The four-lanes-per-particle variants run at most one workgroup per CU and
+// ask for one, which keeps them free of scratch.
 constexpr int kGatherMinBlocks = 6;
 #ifndef SPH_TILE_CAP
 #define SPH_TILE_CAP 1280
@@ -395,8 +397,8 @@ __device__ __forceinline__ GlobalSrc<Rec, Load> global_src(Load l)
 // plan staged one, else from global memory (same order, same results).  Every
 // thread of the workgroup calls this (the staging and its barrier need all of
 // them); `live` lanes walk.
-template <bool OWN, bool CAP, int CAPACITY, class Rec, class Load, class Test, class Body,
-          class Masks>
+template <bool OWN, bool CAP, int CAPACITY, int LP = 1, class Rec, class Load, class Test,
+          class Body, class Masks>
 __device__ __forceinline__ uint2 walk_group(const TilePlan &T, Rec *tile, Load load, bool live,
                                             unsigned self, const sph_rows &R, unsigned key,
                                             const sph_dev_params &P, Test &test, Body &body,
@@ -407,8 +409,8 @@ __device__ __forceinline__ uint2 walk_group(const TilePlan &T, Rec *tile, Load l
     if (T.ok) {
         stage_tile<CAPACITY>(T, mem, load);
         if (live)
-            fh = sph_walk_masked<OWN, CAP>(self, R, lds_src(mem, T, key, P), P.h2, P.bucket_cap,
-                                           test, body, mk);
+            fh = sph_walk_masked<OWN, CAP, LP>(self, R, lds_src(mem, T, key, P), P.h2,
+                                               P.bucket_cap, test, body, mk);
     } else if (T.rows_ok) {
         // one candidate row at a time: same chunks, same order
         unsigned ci = 0;
@@ -416,9 +418,9 @@ __device__ __forceinline__ uint2 walk_group(const TilePlan &T, Rec *tile, Load l
             constexpr int dy = decltype(row)::value;
             stage_row<CAPACITY>(T, dy, mem, load);
             if (live) {
-                const uint2 r =
-                    sph_walk_rows<OWN, CAP, dy, dy>(self, R, lds_row_src(mem, T, dy, key, P),
-                                                    P.h2, P.bucket_cap, test, body, mk, ci);
+                const uint2 r = sph_walk_rows<OWN, CAP, dy, dy, LP>(
+                    self, R, lds_row_src(mem, T, dy, key, P), P.h2, P.bucket_cap, test, body,
+                    mk, ci);
                 fh.x += r.x;
                 fh.y += r.y;
             }
@@ -428,8 +430,8 @@ __device__ __forceinline__ uint2 walk_group(const TilePlan &T, Rec *tile, Load l
         pass(std::integral_constant<int, 0>{});
         pass(std::integral_constant<int, 1>{});
     } else if (live) {
-        fh = sph_walk_masked<OWN, CAP>(self, R, global_src<Rec>(load), P.h2, P.bucket_cap, test,
-                                       body, mk);
+        fh = sph_walk_masked<OWN, CAP, LP>(self, R, global_src<Rec>(load), P.h2, P.bucket_cap,
+                                           test, body, mk);
     }
     return fh;
 }
@@ -447,8 +449,8 @@ __device__ __forceinline__ uint2 walk_group(const TilePlan &T, Rec *tile, Load l
 // PRED: x_prev = x; x += v*dt; boundary; then the new cell key and a slot
 //       in that cell (histogram of the next sort).  Without PRED the
 //       post-viscosity velocity is written to out_pos.
-template <bool GRAV, bool VISC, bool PRED, bool CAP>
-__global__ void __launch_bounds__(kTPB, kGatherMinBlocks)
+template <bool GRAV, bool VISC, bool PRED, bool CAP, int LP = 1>
+__global__ void __launch_bounds__(kTPB, LP == 1 ? kGatherMinBlocks : 1)
 k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
                const float2 *__restrict__ POS, const float2 *__restrict__ VEL,
                const unsigned *__restrict__ K, const unsigned *__restrict__ start,
@@ -459,11 +461,13 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
 {
     __shared__ TilePlan T;
     __shared__ float4 tile[VISC ? kTileCap + kSphWalkPad : 1];
+    constexpr unsigned kPer = kTPB / LP; // particles per workgroup
     const unsigned blk = xcd_block();
-    const sph_dev_params P = Pp[BS[blk]];
-    const unsigned s0 = blk * kTPB;
-    const unsigned s1 = min(s0 + kTPB, P.poff + (unsigned)P.n);
-    const unsigned s = s0 + threadIdx.x;
+    const sph_dev_params P = Pp[BS[blk / LP]];
+    const unsigned s0 = blk * kPer;
+    const unsigned s1 = min(s0 + kPer, P.poff + (unsigned)P.n);
+    const unsigned s = s0 + threadIdx.x / LP;
+    const bool lead = threadIdx.x % LP == 0; // the lane that stores the particle's results
     const float gdt = P.g_neg * P.dt;
     if (PRED)
         add_pad(P, s0, cnt);
@@ -473,7 +477,7 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
         // frozen scene: same position, same cell; its velocity waits in
         // held_vel for the relaxation kernel (the sort of a frozen scene is
         // the identity)
-        if (s < s1) {
+        if (s < s1 && lead) {
             const float2 a = POS[s];
             out_pos[s] = a;
             held_vel[s] = VEL[s];
@@ -516,12 +520,30 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
         };
         plan_tile_par(s0, s1, K, start, P, T, kTileCap);
         sph_masks_none mk;
-        const uint2 fh =
-            walk_group<false, CAP, kTileCap>(T, tile, load, live, s, R, key, P, test, body, mk);
+        uint2 fh;
+        if constexpr (LP == 1) {
+            fh = walk_group<false, CAP, kTileCap>(T, tile, load, live, s, R, key, P, test, body,
+                                                  mk);
+        } else {
+            // the same impulse as a term (zero when not pushed: nv - 0 == nv)
+            auto term = [&](unsigned, bool, float4 b) {
+                float ix, iy;
+                const bool push =
+                    sph_visc_impulse(a.x, a.y, avx, avy, b.x, b.y, b.z, b.w, P, ix, iy);
+                return sph_pair_term{push ? ix : 0.0f, push ? iy : 0.0f, 0};
+            };
+            auto apply = [&](const sph_pair_term &t) {
+                nvx = nvx - t.a * 0.5f;
+                nvy = nvy - t.b * 0.5f;
+            };
+            sph_split_body<decltype(term), decltype(apply)> sb{term, apply};
+            fh = walk_group<false, CAP, kTileCap, LP>(T, tile, load, live, s, R, key, P, test, sb,
+                                                      mk);
+        }
         full = fh.x;
-        half = fh.y;
+        half = lead ? fh.y : 0u; // a particle's half list counted once
     }
-    if (live) {
+    if (live && lead) {
         if (PRED) {
             float x = a.x + nvx * P.dt;
             float y = a.y + nvy * P.dt;
@@ -808,8 +830,8 @@ k_reorder(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
 // ---------------------------------------------------------------------------
 // Density (src/hash.c:190-194, 225-229 with src/fluid.c:527-539) in gather
 // form: each particle sums (1-r/h)^2 and (1-r/h)^3 over its neighbours.
-template <bool CAP>
-__global__ void __launch_bounds__(kTPB, kGatherMinBlocks)
+template <bool CAP, int LP = 1>
+__global__ void __launch_bounds__(kTPB, LP == 1 ? kGatherMinBlocks : 1)
 k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
           const float2 *__restrict__ POS, const unsigned *__restrict__ K,
           const unsigned *__restrict__ start, float2 *__restrict__ RHO,
@@ -820,11 +842,13 @@ k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     // tiles in the same LDS bytes, so denser rows stay in LDS at no cost in
     // occupancy (registers allow 7 workgroups per CU either way)
     __shared__ float2 tile[2 * kTileCap + kSphWalkPad];
+    constexpr unsigned kPer = kTPB / LP; // particles per workgroup
     const unsigned blk = xcd_block();
-    const sph_dev_params P = Pp[BS[blk]];
-    const unsigned s0 = blk * kTPB;
-    const unsigned s1 = min(s0 + kTPB, P.poff + (unsigned)P.n);
-    const unsigned s = s0 + threadIdx.x;
+    const sph_dev_params P = Pp[BS[blk / LP]];
+    const unsigned s0 = blk * kPer;
+    const unsigned s1 = min(s0 + kPer, P.poff + (unsigned)P.n);
+    const unsigned s = s0 + threadIdx.x / LP;
+    const bool lead = threadIdx.x % LP == 0; // the lane that stores the particle's results
     if (s0 >= s1 || P.frozen)
         return; // a padding workgroup, or a frozen scene
     // the lane's own particle and candidate rows, before the barriers
@@ -849,13 +873,33 @@ k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
         rhon = in ? rhon + OmR2 * (1.0f - ratio) : rhon;
     };
     sph_masks_record mk;
-    const uint2 fh =
-        walk_group<false, CAP, 2 * kTileCap>(T, tile, load, live, s, R, key, P, test, body, mk);
-    if (live) {
+    uint2 fh;
+    if constexpr (LP == 1) {
+        fh = walk_group<false, CAP, 2 * kTileCap>(T, tile, load, live, s, R, key, P, test, body,
+                                                  mk);
+    } else {
+        // the same sums as terms (zero outside h: rho + 0 == rho, as rho >= +0)
+        auto term = [&](unsigned, bool, float2 b) {
+            const float dx = a.x - b.x, ddy = a.y - b.y;
+            const float r2 = dx * dx + ddy * ddy;
+            const float ratio = sph_sqrt(r2) * P.h_recip;
+            const float OmR2 = (1.0f - ratio) * (1.0f - ratio);
+            const bool in = ratio < 1.0f;
+            return sph_pair_term{in ? OmR2 : 0.0f, in ? OmR2 * (1.0f - ratio) : 0.0f, 0};
+        };
+        auto apply = [&](const sph_pair_term &t) {
+            rho = rho + t.a;
+            rhon = rhon + t.b;
+        };
+        sph_split_body<decltype(term), decltype(apply)> sb{term, apply};
+        fh = walk_group<false, CAP, 2 * kTileCap, LP>(T, tile, load, live, s, R, key, P, test, sb,
+                                                      mk);
+    }
+    if (live && lead) {
         RHO[s] = make_float2(rho, rhon);
         masks[s] = mk.m; // for the relaxation of the same step
     }
-    const unsigned full = fh.x, half = fh.y;
+    const unsigned full = fh.x, half = lead ? fh.y : 0u;
     report_neighbors(full, half, C);
 }
 
@@ -863,8 +907,10 @@ k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
 // FUSED adds the velocity update (src/fluid.c:642-653): boundary, then
 // v = (x - x_prev)/dt clamped, and the histogram of the next sort; without
 // FUSED only the relaxed positions are written.
-template <bool FUSED, bool CAP>
-__global__ void __launch_bounds__(kTPB, CAP ? kGatherMinBlocks : kGatherMinBlocks + 1)
+template <bool FUSED, bool CAP, int LP = 1>
+__global__ void __launch_bounds__(kTPB, LP != 1 ? 1
+                                        : CAP  ? kGatherMinBlocks
+                                               : kGatherMinBlocks + 1)
 k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
         const float2 *__restrict__ POS, const float2 *__restrict__ PREV,
         const float2 *__restrict__ RHO, const unsigned *__restrict__ K,
@@ -876,11 +922,13 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
 {
     __shared__ TilePlan T;
     __shared__ float4 tile[kTileCap + kSphWalkPad];
+    constexpr unsigned kPer = kTPB / LP; // particles per workgroup
     const unsigned blk = xcd_block();
-    const sph_dev_params P = Pp[BS[blk]];
-    const unsigned s0 = blk * kTPB;
-    const unsigned s1 = min(s0 + kTPB, P.poff + (unsigned)P.n);
-    const unsigned s = s0 + threadIdx.x;
+    const sph_dev_params P = Pp[BS[blk / LP]];
+    const unsigned s0 = blk * kPer;
+    const unsigned s1 = min(s0 + kPer, P.poff + (unsigned)P.n);
+    const unsigned s = s0 + threadIdx.x / LP;
+    const bool lead = threadIdx.x % LP == 0; // the lane that stores the particle's results
     if (FUSED) {
         add_pad(P, s0, cnt);
         if (blockIdx.x == 0 && threadIdx.x == 0)
@@ -891,7 +939,7 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     if (FUSED && P.frozen) {
         // frozen scene: keep the position; out_vel already holds the
         // velocity parked by the prediction kernel at this same slot
-        if (s < s1) {
+        if (s < s1 && lead) {
             const float2 a = POS[s];
             out_pos[s] = a;
             const unsigned nk = sph_cell_key(a.x, a.y, P, &C->key_out_of_range);
@@ -940,14 +988,47 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
         nx = push ? nx - Dx : nx;
         ny = push ? ny - Dy : ny;
     };
-    if (FUSED) { // the density pass of this step walked the same neighbour set
-        sph_masks_replay mk{live ? masks[s] : make_uint4(0u, 0u, 0u, 0u)};
-        walk_group<true, CAP, kTileCap>(T, tile, load, live, s, R, key, P, test, body, mk);
+    if constexpr (LP == 1) {
+        if (FUSED) { // the density pass of this step walked the same neighbour set
+            sph_masks_replay mk{live ? masks[s] : make_uint4(0u, 0u, 0u, 0u)};
+            walk_group<true, CAP, kTileCap>(T, tile, load, live, s, R, key, P, test, body, mk);
+        } else {
+            sph_masks_none mk;
+            walk_group<true, CAP, kTileCap>(T, tile, load, live, s, R, key, P, test, body, mk);
+        }
     } else {
-        sph_masks_none mk;
-        walk_group<true, CAP, kTileCap>(T, tile, load, live, s, R, key, P, test, body, mk);
+        // the same updates as terms: the displacement (zero when not pushed:
+        // n - 0 == n) and the nudge flag, applied nudge first as in body
+        auto term = [&](unsigned, bool own, float4 b) {
+            const float dx = a.x - b.x, ddy = a.y - b.y;
+            const float r2 = dx * dx + ddy * ddy;
+            const float rr = sph_sqrt(r2);
+            const float r_recip = sph_rcp(rr); // 1.0f / rr
+            const float ratio = rr * P.h_recip;
+            const float OmR = 1.0f - ratio;
+            const bool nudge = rr <= 0.000001f && own;
+            const float D = sph_relax_D(P.dt, aP + b.z, aPn + b.w, OmR, P.k_spring, P.h, rr);
+            const float Dx = D * -dx * r_recip;
+            const float Dy = D * -ddy * r_recip;
+            const bool push = ratio < 1.0f && rr > 0.0f;
+            return sph_pair_term{push ? Dx : 0.0f, push ? Dy : 0.0f, nudge ? 1 : 0};
+        };
+        auto apply = [&](const sph_pair_term &t) {
+            nx = t.f ? nx + 0.000001f : nx;
+            ny = t.f ? ny + 0.000001f : ny;
+            nx = nx - t.a;
+            ny = ny - t.b;
+        };
+        sph_split_body<decltype(term), decltype(apply)> sb{term, apply};
+        if (FUSED) {
+            sph_masks_replay mk{live ? masks[s] : make_uint4(0u, 0u, 0u, 0u)};
+            walk_group<true, CAP, kTileCap, LP>(T, tile, load, live, s, R, key, P, test, sb, mk);
+        } else {
+            sph_masks_none mk;
+            walk_group<true, CAP, kTileCap, LP>(T, tile, load, live, s, R, key, P, test, sb, mk);
+        }
     }
-    if (!live)
+    if (!live || !lead)
         return;
     if (FUSED) {
         sph_boundary(nx, ny, P);
@@ -1221,16 +1302,43 @@ k_load_state(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     ID[s] = (int)id;
 }
 
+// Four lanes per particle when the context's 256-slot tiles number at most a
+// quarter of the device's CUs (a single scene of up to 16,384 particles on
+// 256 CUs): each gather launch then still has at most one workgroup per CU,
+// and a workgroup's latency - the whole launch's, with so few - is its
+// particles' walks, which four lanes cut to about a quarter.  Larger contexts
+// fill the device with one lane per particle, where the quad's repeated
+// per-particle work would cost throughput.  SPH_SMALL_SCENE_LANES=1 turns
+// the four-lane path off.
+#ifndef SPH_SMALL_SCENE_LANES
+#define SPH_SMALL_SCENE_LANES 4
+#endif
+int sph_gather_lanes(unsigned slots)
+{
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    const unsigned long long tiles = slots / kTPB;
+    return SPH_SMALL_SCENE_LANES == 4 && cus > 0 && 4ull * tiles <= (unsigned long long)cus ? 4
+                                                                                           : 1;
+}
+
 void sph_launch_visc_predict(hipStream_t st, unsigned slots, bool grav, bool visc, bool pred,
                              bool cap, const sph_dev_params *P, const int *BS, const float2 *POS,
                              const float2 *VEL, const unsigned *K, const unsigned *start,
                              float2 *opos, unsigned *okey, unsigned *oslot, unsigned *cnt,
                              sph_dev_counters *C, const unsigned long long *clock,
-                             float2 *held_vel)
+                             float2 *held_vel, int lanes)
 {
 #define VP(G, V, R, CAP)                                                                         \
     launch(k_visc_predict<G, V, R, CAP>, GRID(slots), kTPB, st, P, BS, POS, VEL, K, start, opos, \
            okey, oslot, cnt, C, clock, held_vel)
+#define VP4(CAP)                                                                                 \
+    launch(k_visc_predict<true, true, true, CAP, 4>, GRID(4 * (slots)), kTPB, st, P, BS, POS,    \
+           VEL, K, start, opos, okey, oslot, cnt, C, clock, held_vel)
 #define VPC(G, V, R)                                                                             \
     do {                                                                                         \
         if (cap)                                                                                 \
@@ -1238,8 +1346,14 @@ void sph_launch_visc_predict(hipStream_t st, unsigned slots, bool grav, bool vis
         else                                                                                     \
             VP(G, V, R, false);                                                                  \
     } while (0)
-    // without viscosity the walk does not run: no capped variant
-    if (grav && visc && pred)
+    // without viscosity the walk does not run: no capped variant; four lanes
+    // per particle for the fused step's variant only
+    if (grav && visc && pred && lanes == 4) {
+        if (cap)
+            VP4(true);
+        else
+            VP4(false);
+    } else if (grav && visc && pred)
         VPC(true, true, true);
     else if (grav && !visc && pred)
         VP(true, false, true, false);
@@ -1251,6 +1365,7 @@ void sph_launch_visc_predict(hipStream_t st, unsigned slots, bool grav, bool vis
         VP(false, false, true, false);
     else
         VPC(true, true, false);
+#undef VP4
 #undef VPC
 #undef VP
 }
@@ -1328,9 +1443,15 @@ void sph_launch_reorder4(hipStream_t st, unsigned slots, const sph_dev_params *P
 
 void sph_launch_density(hipStream_t st, unsigned slots, bool cap, const sph_dev_params *P,
                         const int *BS, const float2 *POS, const unsigned *K,
-                        const unsigned *start, float2 *RHO, uint4 *masks, sph_dev_counters *C)
+                        const unsigned *start, float2 *RHO, uint4 *masks, sph_dev_counters *C,
+                        int lanes)
 {
-    if (cap)
+    if (lanes == 4 && cap)
+        launch(k_density<true, 4>, GRID(4 * slots), kTPB, st, P, BS, POS, K, start, RHO, masks, C);
+    else if (lanes == 4)
+        launch(k_density<false, 4>, GRID(4 * slots), kTPB, st, P, BS, POS, K, start, RHO, masks,
+               C);
+    else if (cap)
         launch(k_density<true>, GRID(slots), kTPB, st, P, BS, POS, K, start, RHO, masks, C);
     else
         launch(k_density<false>, GRID(slots), kTPB, st, P, BS, POS, K, start, RHO, masks, C);
@@ -1341,12 +1462,20 @@ void sph_launch_relax(hipStream_t st, unsigned slots, bool fused, bool cap,
                       const float2 *PREV, const float2 *RHO, const unsigned *K,
                       const unsigned *start, float2 *opos, float2 *ovel, unsigned *okey,
                       unsigned *oslot, unsigned *cnt, const uint4 *masks, sph_dev_counters *C,
-                      unsigned long long *clock)
+                      unsigned long long *clock, int lanes)
 {
 #define RL(F, CAP)                                                                               \
     launch(k_relax<F, CAP>, GRID(slots), kTPB, st, P, BS, POS, PREV, RHO, K, start, opos, ovel,   \
            okey, oslot, cnt, masks, C, clock)
-    if (fused && cap)
+#define RL4(CAP)                                                                                 \
+    launch(k_relax<true, CAP, 4>, GRID(4 * (slots)), kTPB, st, P, BS, POS, PREV, RHO, K, start,  \
+           opos, ovel, okey, oslot, cnt, masks, C, clock)
+    // four lanes per particle for the fused step's variant only
+    if (fused && lanes == 4 && cap)
+        RL4(true);
+    else if (fused && lanes == 4)
+        RL4(false);
+    else if (fused && cap)
         RL(true, true);
     else if (fused)
         RL(true, false);
@@ -1354,6 +1483,7 @@ void sph_launch_relax(hipStream_t st, unsigned slots, bool fused, bool cap,
         RL(false, true);
     else
         RL(false, false);
+#undef RL4
 #undef RL
 }
 

@@ -1231,7 +1231,7 @@ int density_now(sph_ctx *c)
     if (!c->lists_valid)
         return merr(SPH_ESTATE, "surrogate: state not hashed");
     sph_launch_density(c->st, c->slots, c->bucket_cap != 0, c->dP, c->BS, c->POS0, c->K0,
-                       c->start, c->RHO, c->MASK, c->dC);
+                       c->start, c->RHO, c->MASK, c->dC, c->lanes);
     MCHK(hipGetLastError());
     return SPH_OK;
 }
