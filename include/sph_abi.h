@@ -259,9 +259,9 @@ int sph_set_bucket_cap(sph_ctx *ctx, int cap);
 
 /* Launch shape, with no counterpart in the reference: lanes per particle of
  * the fused step's gathers.  0 (the default) chooses by size - 4 when the
- * context's 256-slot tiles number at most a quarter of the device's CUs
- * (a scene of up to 16,384 particles on 256 CUs), where a launch's time is
- * one workgroup's latency, else 1; 1 and 4 force a shape.  Results are bit
+ * context has at most two 256-slot tiles per CU of the device (a scene of
+ * up to 131,072 particles on 256 CUs), where a launch's time is set by its
+ * slowest workgroups, else 1; 1 and 4 force a shape.  Results are bit
  * for bit the same either way: four lanes split a particle's candidate tests
  * and pair terms and apply the terms in the same order.  Any other value is
  * SPH_EINVAL. */

@@ -1302,14 +1302,17 @@ k_load_state(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     ID[s] = (int)id;
 }
 
-// Four lanes per particle when the context's 256-slot tiles number at most a
-// quarter of the device's CUs (a single scene of up to 16,384 particles on
-// 256 CUs): each gather launch then still has at most one workgroup per CU,
-// and a workgroup's latency - the whole launch's, with so few - is its
-// particles' walks, which four lanes cut to about a quarter.  Larger contexts
-// fill the device with one lane per particle, where the quad's repeated
-// per-particle work would cost throughput.  SPH_SMALL_SCENE_LANES=1 turns
-// the four-lane path off.
+// Four lanes per particle when the context has at most two 256-slot tiles
+// per CU (a single scene of up to 131,072 particles on 256 CUs).  There the
+// gather launches are short of work, and a launch's time is set by its
+// slowest workgroups: the dense bottom rows of a settling column, whose walks
+// four lanes cut to about a quarter.  Measured on dam breaks of 4k-1M
+// particles (graph steps 21-220 / 501-600, one lane against four): 4k 164 /
+// 87 us; 65k 164 / 91; 131k 96 against 77 and 239 against 135; 262k 102
+// against 112 and 247 against 175; 524k 121 against 183; 1M 170 against 323.
+// Past the crossover the quad's repeated per-particle work and four times
+// the tiles (staging, plans) cost more than the shorter walks gain.
+// SPH_SMALL_SCENE_LANES=1 turns the four-lane path off.
 #ifndef SPH_SMALL_SCENE_LANES
 #define SPH_SMALL_SCENE_LANES 4
 #endif
@@ -1322,8 +1325,8 @@ int sph_gather_lanes(unsigned slots)
         return 1;
     }
     const unsigned long long tiles = slots / kTPB;
-    return SPH_SMALL_SCENE_LANES == 4 && cus > 0 && 4ull * tiles <= (unsigned long long)cus ? 4
-                                                                                           : 1;
+    return SPH_SMALL_SCENE_LANES == 4 && cus > 0 && tiles <= 2ull * (unsigned long long)cus ? 4
+                                                                                             : 1;
 }
 
 void sph_launch_visc_predict(hipStream_t st, unsigned slots, bool grav, bool visc, bool pred,

@@ -1,8 +1,8 @@
 """Lanes per particle of the fused step's gathers (sph_set_gather_lanes).
 
-A context small enough that its gather launches leave most CUs idle (at most
-a quarter as many 256-slot tiles as CUs: the C1 dam break, the dense test
-lattices, small batches) runs four lanes per particle by default: the quad
+A context with at most two 256-slot tiles per CU (up to 131,072 particles
+on 256 CUs: the C1 dam break, the dense test lattices, small batches, the
+65k dam break) runs four lanes per particle by default: the quad
 splits each 32-candidate chunk's tests four ways and takes four pairs per
 drain step, and every lane applies the four pair terms in canonical order.
 Larger contexts run one lane per particle.  Both shapes must give the
