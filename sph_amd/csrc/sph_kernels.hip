@@ -54,8 +54,9 @@ constexpr int kTPB = 256;
 // density -2%, relaxation -4%, the capped variants -3 to -7%, viscosity
 // unchanged.  The uncapped relaxation asks for 7, which keeps it at 72 VGPRs
 // (7 waves) with its row-by-row staging path; 6 let it grow to 74 (6 waves).
-// The four-lanes-per-particle variants run at most one workgroup per CU and
-// ask for one, which keeps them free of scratch.
+// The four-lanes-per-particle variants (contexts of at most two tiles per
+// CU, so at most eight of their 64-particle workgroups per CU) set no bound:
+// their 66-88 VGPRs still allow 5-7 waves per SIMD, without scratch.
 constexpr int kGatherMinBlocks = 6;
 #ifndef SPH_TILE_CAP
 #define SPH_TILE_CAP 1280
@@ -450,15 +451,16 @@ __device__ __forceinline__ uint2 walk_group(const TilePlan &T, Rec *tile, Load l
 //       in that cell (histogram of the next sort).  Without PRED the
 //       post-viscosity velocity is written to out_pos.
 template <bool GRAV, bool VISC, bool PRED, bool CAP, int LP = 1>
-__global__ void __launch_bounds__(kTPB, LP == 1 ? kGatherMinBlocks : 1)
-k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
-               const float2 *__restrict__ POS, const float2 *__restrict__ VEL,
-               const unsigned *__restrict__ K, const unsigned *__restrict__ start,
-               float2 *__restrict__ out_pos, unsigned *__restrict__ out_key,
-               unsigned *__restrict__ out_slot, unsigned *__restrict__ cnt,
-               sph_dev_counters *__restrict__ C, const unsigned long long *__restrict__ clock,
-               float2 *__restrict__ held_vel)
+__device__ __forceinline__ void
+visc_predict_tile(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+                  const float2 *__restrict__ POS, const float2 *__restrict__ VEL,
+                  const unsigned *__restrict__ K, const unsigned *__restrict__ start,
+                  float2 *__restrict__ out_pos, unsigned *__restrict__ out_key,
+                  unsigned *__restrict__ out_slot, unsigned *__restrict__ cnt,
+                  sph_dev_counters *__restrict__ C, const unsigned long long *__restrict__ clock,
+                  float2 *__restrict__ held_vel)
 {
+
     __shared__ TilePlan T;
     __shared__ float4 tile[VISC ? kTileCap + kSphWalkPad : 1];
     constexpr unsigned kPer = kTPB / LP; // particles per workgroup
@@ -558,6 +560,35 @@ k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS
     }
     if (VISC)
         report_neighbors(full, half, C);
+}
+
+template <bool GRAV, bool VISC, bool PRED, bool CAP>
+__global__ void __launch_bounds__(kTPB, kGatherMinBlocks)
+k_visc_predict(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+               const float2 *__restrict__ POS, const float2 *__restrict__ VEL,
+               const unsigned *__restrict__ K, const unsigned *__restrict__ start,
+               float2 *__restrict__ out_pos, unsigned *__restrict__ out_key,
+               unsigned *__restrict__ out_slot, unsigned *__restrict__ cnt,
+               sph_dev_counters *__restrict__ C, const unsigned long long *__restrict__ clock,
+               float2 *__restrict__ held_vel)
+{
+    visc_predict_tile<GRAV, VISC, PRED, CAP, 1>(Pp, BS, POS, VEL, K, start, out_pos, out_key,
+                                                out_slot, cnt, C, clock, held_vel);
+}
+
+// The fused step's variant with four lanes per particle (sph_gather_lanes).
+template <bool CAP>
+__global__ void __launch_bounds__(kTPB, 1)
+k_visc_predict_quad(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+                    const float2 *__restrict__ POS, const float2 *__restrict__ VEL,
+                    const unsigned *__restrict__ K, const unsigned *__restrict__ start,
+                    float2 *__restrict__ out_pos, unsigned *__restrict__ out_key,
+                    unsigned *__restrict__ out_slot, unsigned *__restrict__ cnt,
+                    sph_dev_counters *__restrict__ C, const unsigned long long *__restrict__ clock,
+                    float2 *__restrict__ held_vel)
+{
+    visc_predict_tile<true, true, true, CAP, 4>(Pp, BS, POS, VEL, K, start, out_pos, out_key,
+                                                out_slot, cnt, C, clock, held_vel);
 }
 
 // src/fluid.c:398-413 as a separate pass (phase API).
@@ -831,12 +862,13 @@ k_reorder(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
 // Density (src/hash.c:190-194, 225-229 with src/fluid.c:527-539) in gather
 // form: each particle sums (1-r/h)^2 and (1-r/h)^3 over its neighbours.
 template <bool CAP, int LP = 1>
-__global__ void __launch_bounds__(kTPB, LP == 1 ? kGatherMinBlocks : 1)
-k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
-          const float2 *__restrict__ POS, const unsigned *__restrict__ K,
-          const unsigned *__restrict__ start, float2 *__restrict__ RHO,
-          uint4 *__restrict__ masks, sph_dev_counters *__restrict__ C)
+__device__ __forceinline__ void
+density_tile(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+             const float2 *__restrict__ POS, const unsigned *__restrict__ K,
+             const unsigned *__restrict__ start, float2 *__restrict__ RHO,
+             uint4 *__restrict__ masks, sph_dev_counters *__restrict__ C)
 {
+
     __shared__ TilePlan T;
     // positions only: twice the candidates of the other gathers' float4
     // tiles in the same LDS bytes, so denser rows stay in LDS at no cost in
@@ -903,23 +935,43 @@ k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     report_neighbors(full, half, C);
 }
 
+template <bool CAP>
+__global__ void __launch_bounds__(kTPB, kGatherMinBlocks)
+k_density(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+          const float2 *__restrict__ POS, const unsigned *__restrict__ K,
+          const unsigned *__restrict__ start, float2 *__restrict__ RHO,
+          uint4 *__restrict__ masks, sph_dev_counters *__restrict__ C)
+{
+    density_tile<CAP, 1>(Pp, BS, POS, K, start, RHO, masks, C);
+}
+
+// Four lanes per particle.
+template <bool CAP>
+__global__ void __launch_bounds__(kTPB, 1)
+k_density_quad(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+               const float2 *__restrict__ POS, const unsigned *__restrict__ K,
+               const unsigned *__restrict__ start, float2 *__restrict__ RHO,
+               uint4 *__restrict__ masks, sph_dev_counters *__restrict__ C)
+{
+    density_tile<CAP, 4>(Pp, BS, POS, K, start, RHO, masks, C);
+}
+
 // Double-density relaxation (src/fluid.c:541-611) in gather form.
 // FUSED adds the velocity update (src/fluid.c:642-653): boundary, then
 // v = (x - x_prev)/dt clamped, and the histogram of the next sort; without
 // FUSED only the relaxed positions are written.
 template <bool FUSED, bool CAP, int LP = 1>
-__global__ void __launch_bounds__(kTPB, LP != 1 ? 1
-                                        : CAP  ? kGatherMinBlocks
-                                               : kGatherMinBlocks + 1)
-k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
-        const float2 *__restrict__ POS, const float2 *__restrict__ PREV,
-        const float2 *__restrict__ RHO, const unsigned *__restrict__ K,
-        const unsigned *__restrict__ start, float2 *__restrict__ out_pos,
-        float2 *__restrict__ out_vel, unsigned *__restrict__ out_key,
-        unsigned *__restrict__ out_slot, unsigned *__restrict__ cnt,
-        const uint4 *__restrict__ masks, sph_dev_counters *__restrict__ C,
-        unsigned long long *__restrict__ clock)
+__device__ __forceinline__ void
+relax_tile(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+           const float2 *__restrict__ POS, const float2 *__restrict__ PREV,
+           const float2 *__restrict__ RHO, const unsigned *__restrict__ K,
+           const unsigned *__restrict__ start, float2 *__restrict__ out_pos,
+           float2 *__restrict__ out_vel, unsigned *__restrict__ out_key,
+           unsigned *__restrict__ out_slot, unsigned *__restrict__ cnt,
+           const uint4 *__restrict__ masks, sph_dev_counters *__restrict__ C,
+           unsigned long long *__restrict__ clock)
 {
+
     __shared__ TilePlan T;
     __shared__ float4 tile[kTileCap + kSphWalkPad];
     constexpr unsigned kPer = kTPB / LP; // particles per workgroup
@@ -1044,6 +1096,37 @@ k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
         out_slot[s] = wave_slot(cnt, nk);
     }
     out_pos[s] = make_float2(nx, ny);
+}
+
+template <bool FUSED, bool CAP>
+__global__ void __launch_bounds__(kTPB, CAP ? kGatherMinBlocks : kGatherMinBlocks + 1)
+k_relax(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+        const float2 *__restrict__ POS, const float2 *__restrict__ PREV,
+        const float2 *__restrict__ RHO, const unsigned *__restrict__ K,
+        const unsigned *__restrict__ start, float2 *__restrict__ out_pos,
+        float2 *__restrict__ out_vel, unsigned *__restrict__ out_key,
+        unsigned *__restrict__ out_slot, unsigned *__restrict__ cnt,
+        const uint4 *__restrict__ masks, sph_dev_counters *__restrict__ C,
+        unsigned long long *__restrict__ clock)
+{
+    relax_tile<FUSED, CAP, 1>(Pp, BS, POS, PREV, RHO, K, start, out_pos, out_vel, out_key, out_slot,
+                              cnt, masks, C, clock);
+}
+
+// The fused variant with four lanes per particle.
+template <bool CAP>
+__global__ void __launch_bounds__(kTPB, 1)
+k_relax_quad(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
+             const float2 *__restrict__ POS, const float2 *__restrict__ PREV,
+             const float2 *__restrict__ RHO, const unsigned *__restrict__ K,
+             const unsigned *__restrict__ start, float2 *__restrict__ out_pos,
+             float2 *__restrict__ out_vel, unsigned *__restrict__ out_key,
+             unsigned *__restrict__ out_slot, unsigned *__restrict__ cnt,
+             const uint4 *__restrict__ masks, sph_dev_counters *__restrict__ C,
+             unsigned long long *__restrict__ clock)
+{
+    relax_tile<true, CAP, 4>(Pp, BS, POS, PREV, RHO, K, start, out_pos, out_vel, out_key, out_slot,
+                             cnt, masks, C, clock);
 }
 
 // Reference coordinate packing (src/fluid.c:356-363), written by id.
@@ -1340,7 +1423,7 @@ void sph_launch_visc_predict(hipStream_t st, unsigned slots, bool grav, bool vis
     launch(k_visc_predict<G, V, R, CAP>, GRID(slots), kTPB, st, P, BS, POS, VEL, K, start, opos, \
            okey, oslot, cnt, C, clock, held_vel)
 #define VP4(CAP)                                                                                 \
-    launch(k_visc_predict<true, true, true, CAP, 4>, GRID(4 * (slots)), kTPB, st, P, BS, POS,    \
+    launch(k_visc_predict_quad<CAP>, GRID(4 * (slots)), kTPB, st, P, BS, POS,    \
            VEL, K, start, opos, okey, oslot, cnt, C, clock, held_vel)
 #define VPC(G, V, R)                                                                             \
     do {                                                                                         \
@@ -1450,9 +1533,10 @@ void sph_launch_density(hipStream_t st, unsigned slots, bool cap, const sph_dev_
                         int lanes)
 {
     if (lanes == 4 && cap)
-        launch(k_density<true, 4>, GRID(4 * slots), kTPB, st, P, BS, POS, K, start, RHO, masks, C);
+        launch(k_density_quad<true>, GRID(4 * slots), kTPB, st, P, BS, POS, K, start, RHO, masks,
+               C);
     else if (lanes == 4)
-        launch(k_density<false, 4>, GRID(4 * slots), kTPB, st, P, BS, POS, K, start, RHO, masks,
+        launch(k_density_quad<false>, GRID(4 * slots), kTPB, st, P, BS, POS, K, start, RHO, masks,
                C);
     else if (cap)
         launch(k_density<true>, GRID(slots), kTPB, st, P, BS, POS, K, start, RHO, masks, C);
@@ -1471,7 +1555,7 @@ void sph_launch_relax(hipStream_t st, unsigned slots, bool fused, bool cap,
     launch(k_relax<F, CAP>, GRID(slots), kTPB, st, P, BS, POS, PREV, RHO, K, start, opos, ovel,   \
            okey, oslot, cnt, masks, C, clock)
 #define RL4(CAP)                                                                                 \
-    launch(k_relax<true, CAP, 4>, GRID(4 * (slots)), kTPB, st, P, BS, POS, PREV, RHO, K, start,  \
+    launch(k_relax_quad<CAP>, GRID(4 * (slots)), kTPB, st, P, BS, POS, PREV, RHO, K, start,  \
            opos, ovel, okey, oslot, cnt, masks, C, clock)
     // four lanes per particle for the fused step's variant only
     if (fused && lanes == 4 && cap)
