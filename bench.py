@@ -558,9 +558,9 @@ def capped_line(s, upload, steps, warmup, repeats):
                               "counters": counters}}
 
 
-def c1_line():
+def c1_line(cpu=True):
     """The N = 4096 configuration (64 x 64 lattice, box 60 x 40, 1000 steps):
-    the GPU through the step graph, and the reference's own C step
+    the GPU through the step graph, and (cpu) the reference's own C step
     (oracle/_ref, -O3 -ffast-math) on one host core over the same 1000
     steps."""
     tun, box, parts = workload(64, (60.0, 40.0), seed=20260101)
@@ -577,7 +577,7 @@ def c1_line():
     out = {"particles": n, "steps": "2-1000 (graph)", "value": n * 999 / el,
            "unit": "particle-steps/s", "ms_per_step": el * 1e3 / 999}
     from oracle import pyoracle as po
-    if po.have_compiled_ref(fast=True):
+    if cpu and po.have_compiled_ref(fast=True):
         sim = po.CompiledRef(box, tun, n, fast=True)
         sim.load(parts)
         sim.step(1)
@@ -674,6 +674,9 @@ def main():
                          "the headline's window; 550 profiles the late regime)")
     ap.add_argument("--bucket-cap", type=int, default=0,
                     help="with --kernels-only: run under this bucket cap (the reference's: 100)")
+    ap.add_argument("--only-c1", action="store_true",
+                    help="run only the N = 4096 line (with --no-cpu: the GPU leg alone) and "
+                         "print it")
     args = ap.parse_args()
     if args.profile_summary:
         print(profile_summary(args.profile_summary, args.last))
@@ -702,6 +705,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs one scene on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.only_c1:
+        print(json.dumps({"c1": c1_line(cpu=not args.no_cpu)}))
+        return
 
     # host-core baseline of the batched configuration, before any GPU use
     # (forked workers)
