@@ -39,6 +39,12 @@ HIP_FLAGS = [
 C_FLAGS = ["-O2", "-fPIC", "-std=c11", "-ffp-contract=off", "-fno-fast-math", "-Wall"]
 
 HIP_SRCS = ["sph_kernels.hip", "sph_ctx.hip", "sph_mlp.hip"]
+# Extra flags per source.  The gather kernels are scheduled for instruction-
+# level parallelism (LLVM's max-ilp strategy): the same instructions and
+# registers within the same occupancy, bit-identical results; measured 2.4%
+# faster on the 1M dam break's late steps, 5.6% on the 4096-particle one,
+# 0.8% on the 256-scene batch, the 1M early steps unchanged.
+SRC_FLAGS = {"sph_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
 C_SRCS = ["scene.c", "sph_shim.c"]
 VARIANT_DIR = os.path.join(PKG, "variants")
 # (LDS tile capacity, tiles per XCD run) of the gather kernels; (1280, 16) is
@@ -67,7 +73,8 @@ def _objects(defines=()):
     objs = {}
     for src in HIP_SRCS:
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
-        _run([HIPCC, *HIP_FLAGS, *defines, "-c", os.path.join(CSRC, src), "-o", obj])
+        _run([HIPCC, *HIP_FLAGS, *SRC_FLAGS.get(src, []), *defines, "-c", os.path.join(CSRC, src),
+              "-o", obj])
         objs[src] = obj
     for src in C_SRCS:
         obj = os.path.join(BUILD, src.replace(".c", ".o"))
@@ -95,8 +102,8 @@ def build_variants(objs, verbose=False):
     os.makedirs(VARIANT_DIR, exist_ok=True)
     for cap, run in VARIANTS + CHECK_VARIANTS:
         kobj = os.path.join(BUILD, f"sph_kernels_t{cap}_x{run}.o")
-        _run([HIPCC, *HIP_FLAGS, f"-DSPH_TILE_CAP={cap}", f"-DSPH_XCD_CHUNK={run}", "-c",
-              os.path.join(CSRC, "sph_kernels.hip"), "-o", kobj])
+        _run([HIPCC, *HIP_FLAGS, *SRC_FLAGS["sph_kernels.hip"], f"-DSPH_TILE_CAP={cap}",
+              f"-DSPH_XCD_CHUNK={run}", "-c", os.path.join(CSRC, "sph_kernels.hip"), "-o", kobj])
         rest = [o for src, o in objs.items() if src != "sph_kernels.hip"]
         out = os.path.join(VARIANT_DIR, variant_name(cap, run))
         _link([kobj, *rest], out)

@@ -236,7 +236,8 @@ def test_gather_kernels_use_no_scratch_memory():
     for any instance of the three gather kernels."""
     from sph_amd import build as bd
     src = os.path.join(bd.CSRC, "sph_kernels.hip")
-    r = subprocess.run([bd.HIPCC, *bd.HIP_FLAGS, "--cuda-device-only", "-c", src, "-o", os.devnull,
+    r = subprocess.run([bd.HIPCC, *bd.HIP_FLAGS, *bd.SRC_FLAGS.get("sph_kernels.hip", []),
+                        "--cuda-device-only", "-c", src, "-o", os.devnull,
                         "-Rpass-analysis=kernel-resource-usage"],
                        capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-2000:]
