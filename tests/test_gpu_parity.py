@@ -314,13 +314,15 @@ def test_reference_loop_through_shims_at_1m(tmp_path, flags):
     assert_bitexact(got, s.download(), what="1M shim loop vs sph_step")
 
 
+@pytest.mark.parametrize("lanes", [0, 1])
 @pytest.mark.parametrize("seed", [11, 12, 13, 14, 15, 16])
-def test_random_scenes_and_tunables(seed):
+def test_random_scenes_and_tunables(seed, lanes):
     """Random scenes over a wide parameter range - smoothing radius 0.5-2,
     time step 1/240-1/30, negative and large spring constants, viscosity up
     to sigma 5 and beta 2, either mover, random velocities up to the clamp,
     some coincident particles - 25 fused steps (graph and eager alternating)
-    bit-exact against the oracle's gather form."""
+    bit-exact against the oracle's gather form, with the gathers' size rule
+    (four lanes per particle here) and with one lane."""
     rng = np.random.default_rng(seed)
     n = int(rng.integers(200, 3000))
     h = float(rng.choice([0.5, 0.75, 1.0, 1.3, 2.0]))
@@ -345,8 +347,9 @@ def test_random_scenes_and_tunables(seed):
     p["x_prev"], p["y_prev"] = p["x"], p["y"]
     p["id"] = np.arange(n)
     s, g = make_pair(p, t, box)
+    s.set_gather_lanes(lanes)
     for step in range(1, 26):
         s.step(1, graph=step % 2 == 0)
         g.step(1)
-    assert_bitexact(s.download(), g.get_state(), what=f"seed {seed}, h {h}, n {n}")
+    assert_bitexact(s.download(), g.get_state(), what=f"seed {seed}, lanes {lanes}, h {h}, n {n}")
     assert s.counters()["nonfinite"] == g.counters().get("nonfinite", 0) == 0

@@ -40,16 +40,20 @@ def pair(preset="x"):
     return s, g, t, box, parts, info
 
 
+@pytest.mark.parametrize("lanes", [0, 1])
 @pytest.mark.parametrize("preset", ["x", "y", "a", "b"])
-def test_reference_presets_bitexact(preset):
+def test_reference_presets_bitexact(preset, lanes):
+    """lanes 0: the size rule (four lanes per particle for this scene); 1: one."""
     s, g, t, box, parts, info = pair(preset)
+    s.set_gather_lanes(lanes)
     done = 0
     for upto in (1, 2, 5, 20, 60):
         print(f"preset {preset}: steps to {upto}", flush=True)
         s.step(upto - done)
         g.step(upto - done)
         done = upto
-        assert_bitexact(s.download(), g.get_state(), what=f"preset {preset} step {upto}")
+        assert_bitexact(s.download(), g.get_state(),
+                        what=f"preset {preset}, lanes {lanes}, step {upto}")
     c, oc = s.counters(), g.counters()
     assert c["nonfinite"] == 0 and c["bucket_overflow"] == oc["bucket_overflow"] == 0
     assert c["max_cell_count"] == oc["max_cell_count"]
@@ -294,10 +298,13 @@ def test_shim_mark_host_dirty_with_added_particles(shim):
         assert_bitexact(sh.state(), g.get_state(), what=f"after adding {extra} particles")
 
 
-def test_coincident_particles_nudge_owner():
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_coincident_particles_nudge_owner(lanes):
     """Particles closer than 1e-6 take the reference's nudge of +1e-6 on the
     half-list owner only (src/fluid.c:583-586): pairs in one cell (lower id
-    owns) and across a cell border (the particle in the backward cell owns)."""
+    owns) and across a cell border (the particle in the backward cell owns).
+    With one and with four lanes per particle (the nudge is a flag of the
+    quad's pair terms, applied in order)."""
     t = sc.preset_tunables(0)
     box = sa.make_box(10, 10)
     p = np.zeros(6, sa.PARTICLE)
@@ -307,9 +314,10 @@ def test_coincident_particles_nudge_owner():
     p["id"] = [4, 1, 0, 5, 2, 3]
     s = sa.Solver(t, box, len(p))
     s.upload(p)
+    s.set_gather_lanes(lanes)
     g = po.GatherSim(box, t, len(p))
     g.set_state(p.view(po.P32))
     for step in range(3):
         s.step(1)
         g.step(1)
-        assert_bitexact(s.download(), g.get_state(), what=f"step {step}")
+        assert_bitexact(s.download(), g.get_state(), what=f"lanes {lanes}, step {step}")
