@@ -153,13 +153,18 @@ struct Tick {
 void enqueue_sort_tail(sph_ctx *c, const int *id_src, unsigned *key_dst)
 {
     const unsigned m = c->cells + 1;
-    {
-        Tick t(c, KSR);
-        sph_launch_scan_reduce(c->st, c->cnt, m, c->bsum);
-    }
-    {
+    if (sph_scan_single_fits(m)) { // a small grid: the whole scan in one launch
         Tick t(c, KSA);
-        sph_launch_scan_apply(c->st, c->cnt, m, c->bsum, c->start);
+        sph_launch_scan_single(c->st, c->cnt, m, c->start);
+    } else {
+        {
+            Tick t(c, KSR);
+            sph_launch_scan_reduce(c->st, c->cnt, m, c->bsum);
+        }
+        {
+            Tick t(c, KSA);
+            sph_launch_scan_apply(c->st, c->cnt, m, c->bsum, c->start);
+        }
     }
     {
         Tick t(c, KSC);

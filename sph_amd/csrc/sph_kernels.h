@@ -41,6 +41,9 @@ void sph_launch_key_hist(hipStream_t st, unsigned slots, const sph_dev_params *P
 unsigned sph_scan_blocks(unsigned m);
 size_t sph_scan_padded(unsigned m);
 void sph_launch_scan_reduce(hipStream_t st, const unsigned *cnt, unsigned m, unsigned *bsum);
+// Small count arrays (sph_scan_single_fits(m)): the same scan in one launch.
+bool sph_scan_single_fits(unsigned m);
+void sph_launch_scan_single(hipStream_t st, unsigned *cnt, unsigned m, unsigned *start);
 void sph_launch_scan_apply(hipStream_t st, unsigned *cnt, unsigned m, const unsigned *bsum,
                            unsigned *start);
 void sph_launch_scatter(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,

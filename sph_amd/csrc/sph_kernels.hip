@@ -743,6 +743,40 @@ k_scan_apply(uint4 *__restrict__ cnt, const unsigned *__restrict__ bsum,
     t[threadIdx.x + kScanThreads] = make_uint4(0u, 0u, 0u, 0u);
 }
 
+// The whole scan of a small count array (sph_scan_single_fits: at most
+// kScanSingle counts, whole tiles) in one workgroup: the same starts and the
+// same cleared counts as k_scan_reduce and k_scan_apply, in one launch where
+// they take two.  Thread t holds counts [4t, 4t + 4).
+constexpr unsigned kScanSingleThreads = 1024;
+constexpr unsigned kScanSingle = 4 * kScanSingleThreads;
+
+__global__ void __launch_bounds__(kScanSingleThreads)
+k_scan_single(uint4 *__restrict__ cnt, unsigned mp, uint4 *__restrict__ start)
+{
+    constexpr unsigned kWaves = kScanSingleThreads / kWave;
+    __shared__ unsigned wsum[kWaves];
+    const unsigned t = threadIdx.x, lane = t & (kWave - 1u), w = t / kWave;
+    const bool live = 4u * t < mp;
+    const uint4 c = live ? cnt[t] : make_uint4(0u, 0u, 0u, 0u);
+    const unsigned tot = sum4(c);
+    unsigned inc = tot;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const unsigned y = (unsigned)__shfl_up((int)inc, o, kWave);
+        inc += lane >= (unsigned)o ? y : 0u;
+    }
+    if (lane == kWave - 1u)
+        wsum[w] = inc;
+    __syncthreads();
+    unsigned r = inc - tot;
+    for (unsigned i = 0; i < w; i++)
+        r += wsum[i];
+    if (live) {
+        start[t] = make_uint4(r, r + c.x, r + c.x + c.y, r + c.x + c.y + c.z);
+        cnt[t] = make_uint4(0u, 0u, 0u, 0u); // cleared for the next sort
+    }
+}
+
 // Drop (id, source index) into the cell ranges in slot order.
 __global__ void __launch_bounds__(kTPB)
 k_scatter(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
@@ -1499,6 +1533,20 @@ void sph_launch_scan_apply(hipStream_t st, unsigned *cnt, unsigned m, const unsi
 {
     launch(k_scan_apply, dim3(sph_scan_blocks(m)), kScanThreads, st,
            reinterpret_cast<uint4 *>(cnt), bsum, reinterpret_cast<uint4 *>(start));
+}
+
+#ifndef SPH_SCAN_SINGLE
+#define SPH_SCAN_SINGLE 1
+#endif
+bool sph_scan_single_fits(unsigned m)
+{
+    return SPH_SCAN_SINGLE && sph_scan_padded(m) <= kScanSingle;
+}
+
+void sph_launch_scan_single(hipStream_t st, unsigned *cnt, unsigned m, unsigned *start)
+{
+    launch(k_scan_single, dim3(1), kScanSingleThreads, st, reinterpret_cast<uint4 *>(cnt),
+           (unsigned)sph_scan_padded(m), reinterpret_cast<uint4 *>(start));
 }
 
 void sph_launch_scatter(hipStream_t st, unsigned slots, const sph_dev_params *P, const int *BS,
