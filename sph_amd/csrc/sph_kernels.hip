@@ -180,12 +180,12 @@ __device__ __forceinline__ unsigned wave_slot(unsigned *cnt, unsigned key)
 // segment-bounding cell start each, and thread 0 assembles the plan from
 // those with no memory access of its own.  Ends with a barrier.  The group
 // is staged when its rows hold at most `cap` records.
-__device__ void plan_tile_par(unsigned s0, unsigned s1, const unsigned *__restrict__ K,
+__device__ void plan_tile_par(unsigned key_first, unsigned key_last,
                               const unsigned *__restrict__ start, const sph_dev_params &P,
                               TilePlan &T, unsigned cap)
 {
     const unsigned sx = P.sx, sy = P.sy;
-    const unsigned kf = K[s0] - P.cell_base, kl = K[s1 - 1] - P.cell_base;
+    const unsigned kf = key_first - P.cell_base, kl = key_last - P.cell_base;
     const unsigned jA = kf / sx, jB = kl / sx;
     const unsigned t = threadIdx.x;
     if (t < 12u && jB <= jA + 1) {
@@ -465,10 +465,16 @@ visc_predict_tile(const sph_dev_params *__restrict__ Pp, const int *__restrict__
     __shared__ float4 tile[VISC ? kTileCap + kSphWalkPad : 1];
     constexpr unsigned kPer = kTPB / LP; // particles per workgroup
     const unsigned blk = xcd_block();
-    const sph_dev_params P = Pp[BS[blk / LP]];
     const unsigned s0 = blk * kPer;
-    const unsigned s1 = min(s0 + kPer, P.poff + (unsigned)P.n);
     const unsigned s = s0 + threadIdx.x / LP;
+    // loads that need no scene parameter go out before the parameters (every
+    // slot of the grid is inside the arrays; a value is used where its slot
+    // is live)
+    const float2 a0 = POS[s], av0 = VEL[s];
+    const unsigned ks = VISC ? K[s] : 0u;
+    const unsigned kf0 = VISC ? K[s0] : 0u, kl0 = VISC ? K[s0 + kPer - 1] : 0u;
+    const sph_dev_params P = Pp[BS[blk / LP]];
+    const unsigned s1 = min(s0 + kPer, P.poff + (unsigned)P.n);
     const bool lead = threadIdx.x % LP == 0; // the lane that stores the particle's results
     const float gdt = P.g_neg * P.dt;
     if (PRED)
@@ -494,9 +500,9 @@ visc_predict_tile(const sph_dev_params *__restrict__ Pp, const int *__restrict__
     const float sdt = slosh ? sph_slosh_accel(P, *clock) * P.dt : 0.0f;
     // the lane's own particle and candidate rows, loaded before the planning
     // barriers
-    const float2 a = POS[s], av = VEL[s];
+    const float2 a = a0, av = av0;
     const bool live = s < s1;
-    const unsigned key = VISC && live ? K[s] : P.cell_base; // pad lanes: any valid cell
+    const unsigned key = VISC && live ? ks : P.cell_base; // pad lanes: any valid cell
     const sph_rows R = VISC ? sph_row_bounds(key, P, start) : sph_rows{};
     const float avx = slosh ? av.x + sdt : av.x;
     const float avy = GRAV ? av.y + gdt : av.y;
@@ -520,7 +526,7 @@ visc_predict_tile(const sph_dev_params *__restrict__ Pp, const int *__restrict__
             nvx = push ? nvx - ix * 0.5f : nvx;
             nvy = push ? nvy - iy * 0.5f : nvy;
         };
-        plan_tile_par(s0, s1, K, start, P, T, kTileCap);
+        plan_tile_par(kf0, s1 - s0 == kPer ? kl0 : K[s1 - 1], start, P, T, kTileCap);
         sph_masks_none mk;
         uint2 fh;
         if constexpr (LP == 1) {
@@ -910,19 +916,22 @@ density_tile(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     __shared__ float2 tile[2 * kTileCap + kSphWalkPad];
     constexpr unsigned kPer = kTPB / LP; // particles per workgroup
     const unsigned blk = xcd_block();
-    const sph_dev_params P = Pp[BS[blk / LP]];
     const unsigned s0 = blk * kPer;
-    const unsigned s1 = min(s0 + kPer, P.poff + (unsigned)P.n);
     const unsigned s = s0 + threadIdx.x / LP;
+    // the lane's own particle and the tile's end keys, before the scene
+    // parameters (every slot of the grid is inside the arrays)
+    const float2 a = POS[s];
+    const unsigned ks = K[s], kf0 = K[s0], kl0 = K[s0 + kPer - 1];
+    const sph_dev_params P = Pp[BS[blk / LP]];
+    const unsigned s1 = min(s0 + kPer, P.poff + (unsigned)P.n);
     const bool lead = threadIdx.x % LP == 0; // the lane that stores the particle's results
     if (s0 >= s1 || P.frozen)
         return; // a padding workgroup, or a frozen scene
-    // the lane's own particle and candidate rows, before the barriers
-    const float2 a = POS[s];
+    // the candidate rows, before the barriers
     const bool live = s < s1;
-    const unsigned key = live ? K[s] : P.cell_base; // pad lanes: any valid cell
+    const unsigned key = live ? ks : P.cell_base; // pad lanes: any valid cell
     const sph_rows R = sph_row_bounds(key, P, start);
-    plan_tile_par(s0, s1, K, start, P, T, 2 * kTileCap);
+    plan_tile_par(kf0, s1 - s0 == kPer ? kl0 : K[s1 - 1], start, P, T, 2 * kTileCap);
     float rho = 0.0f, rhon = 0.0f;
     auto load = [&](unsigned g) { return POS[g]; };
     auto test = [&](float2 b) {
@@ -1010,10 +1019,13 @@ relax_tile(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     __shared__ float4 tile[kTileCap + kSphWalkPad];
     constexpr unsigned kPer = kTPB / LP; // particles per workgroup
     const unsigned blk = xcd_block();
-    const sph_dev_params P = Pp[BS[blk / LP]];
     const unsigned s0 = blk * kPer;
-    const unsigned s1 = min(s0 + kPer, P.poff + (unsigned)P.n);
     const unsigned s = s0 + threadIdx.x / LP;
+    // the lane's own key and the tile's end keys, before the scene
+    // parameters (every slot of the grid is inside the arrays)
+    const unsigned ks = K[s], kf0 = K[s0], kl0 = K[s0 + kPer - 1];
+    const sph_dev_params P = Pp[BS[blk / LP]];
+    const unsigned s1 = min(s0 + kPer, P.poff + (unsigned)P.n);
     const bool lead = threadIdx.x % LP == 0; // the lane that stores the particle's results
     if (FUSED) {
         add_pad(P, s0, cnt);
@@ -1039,9 +1051,9 @@ relax_tile(const sph_dev_params *__restrict__ Pp, const int *__restrict__ BS,
     const float2 a = POS[s];
     const float2 ar = RHO[s];
     const bool live = s < s1;
-    const unsigned key = live ? K[s] : P.cell_base; // pad lanes: any valid cell
+    const unsigned key = live ? ks : P.cell_base; // pad lanes: any valid cell
     const sph_rows R = sph_row_bounds(key, P, start);
-    plan_tile_par(s0, s1, K, start, P, T, kTileCap);
+    plan_tile_par(kf0, s1 - s0 == kPer ? kl0 : K[s1 - 1], start, P, T, kTileCap);
     // a neighbour's record carries its pressure and near-pressure
     // (src/fluid.c:563-564), evaluated once per record rather than per pair
     auto load = [&](unsigned g) {
