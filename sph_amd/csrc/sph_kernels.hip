@@ -173,13 +173,13 @@ __device__ __forceinline__ unsigned wave_slot(unsigned *cnt, unsigned key)
     return __shfl(base, leader, 64) + rank;
 }
 
-// The tile plan of the sorted particles [s0, s1) of a workgroup: the
-// segments of the candidate rows of its (at most two) cell rows, i.e. for
-// each row the sorted range of cells c0-1 .. c1+1.  Made by the whole
-// workgroup: every thread reads the two end keys, threads 0..11 read one
-// segment-bounding cell start each, and thread 0 assembles the plan from
-// those with no memory access of its own.  Ends with a barrier.  The group
-// is staged when its rows hold at most `cap` records.
+// The tile plan of the sorted particles [s0, s1) of a workgroup, given the
+// keys of s0 and s1 - 1 (the caller reads them early): the segments of the
+// candidate rows of its (at most two) cell rows, i.e. for each row the
+// sorted range of cells c0-1 .. c1+1.  Made by the whole workgroup: threads
+// 0..11 read one segment-bounding cell start each, and thread 0 assembles
+// the plan from those with no memory access of its own.  Ends with a
+// barrier.  The group is staged when its rows hold at most `cap` records.
 __device__ void plan_tile_par(unsigned key_first, unsigned key_last,
                               const unsigned *__restrict__ start, const sph_dev_params &P,
                               TilePlan &T, unsigned cap)
