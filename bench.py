@@ -322,13 +322,13 @@ def calibration_copy(mbytes):
     hip.hipFree(b)
 
 
-def profile_summary(db, last=0):
+def profile_summary(db, last=0, skip=0):
     """Text summary of a rocprofv3 results database (its default output):
     per kernel, the dispatch count and mean / total duration from the kernel
     trace, and the mean per dispatch of every collected counter (FETCH_SIZE
     and WRITE_SIZE are in KiB).  `last` keeps only the library's last N
     dispatches of the trace (the steady state; the runtime's own blit
-    kernels are left out)."""
+    kernels are left out), after dropping its final `skip` dispatches."""
     import sqlite3
     con = sqlite3.connect(db)
     names = {r[0] for r in con.execute("select name from sqlite_master")}
@@ -338,7 +338,8 @@ def profile_summary(db, last=0):
         rows = con.execute("select name, dispatch_id, start, end, duration from kernels "
                            "order by start").fetchall()
         if last:  # the library's last N dispatches (runtime blit kernels left out)
-            rows = [r for r in rows if not short(r[0]).startswith("__amd")][-last:]
+            rows = [r for r in rows if not short(r[0]).startswith("__amd")]
+            rows = rows[:len(rows) - skip][-last:]
         agg = {}
         for name, _, _, _, dur in rows:
             agg.setdefault(short(name), []).append(dur)
@@ -652,6 +653,9 @@ def main():
                     help="also run a device copy of this many MB (profiler calibration)")
     ap.add_argument("--profile-summary", metavar="DB",
                     help="print a summary of a rocprofv3 results database and exit")
+    ap.add_argument("--skip", type=int, default=0,
+                    help="with --profile-summary and --last N: drop the final M dispatches "
+                         "before taking the last N")
     ap.add_argument("--last", type=int, default=0,
                     help="with --profile-summary: keep the last N trace dispatches")
     ap.add_argument("--counter-summary", metavar="DB",
@@ -679,7 +683,7 @@ def main():
                          "print it")
     args = ap.parse_args()
     if args.profile_summary:
-        print(profile_summary(args.profile_summary, args.last))
+        print(profile_summary(args.profile_summary, args.last, args.skip))
         return
     if args.counter_summary:
         print(counter_summary(args.counter_summary, args.last))

@@ -84,3 +84,28 @@ def test_counter_summary_means_and_derived_figures(tmp_path):
     assert "dispatches=1" in b
     last = bench.counter_summary(str(db), last=1).splitlines()
     assert len(last) == 1 and last[0].startswith("k_b<2> ")
+
+
+def test_profile_summary_window(tmp_path):
+    """--last N after --skip M: the library's dispatches before the final M,
+    runtime blit kernels left out."""
+    import sqlite3
+    db = tmp_path / "run_results.db"
+    con = sqlite3.connect(db)
+    con.execute("create table kernels (name text, dispatch_id int, start int, end int, "
+                "duration int)")
+    rows, t = [], 0
+    for d, (k, dur) in enumerate([("k_a(int)", 1000), ("__amd_rocclr_copyBuffer", 50),
+                                  ("k_a(int)", 2000), ("k_b(int)", 3000), ("k_a(int)", 4000),
+                                  ("k_b(int)", 5000)]):
+        rows.append((k, d, t, t + dur, dur))
+        t += dur + 10
+    con.executemany("insert into kernels values (?,?,?,?,?)", rows)
+    con.commit()
+    out = bench.profile_summary(str(db), last=2, skip=2).splitlines()
+    a = next(l for l in out if l.startswith("k_a "))
+    b = next(l for l in out if l.startswith("k_b "))
+    assert a.split()[1:3] == ["1", "2.00"] and b.split()[1:3] == ["1", "3.00"]
+    assert out[-1].endswith("(2 dispatches)")
+    tail = bench.profile_summary(str(db), last=2).splitlines()
+    assert next(l for l in tail if l.startswith("k_a ")).split()[1:3] == ["1", "4.00"]
